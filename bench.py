@@ -1,0 +1,200 @@
+"""Benchmark of the exact NTT multiply on one MI355X.
+
+Workload (one "step"): one exact product of two 2^26-limb integers
+(uniform random 64-bit limbs, SplitMix64 seeds 0x48475801/0x48475802,
+top limb non-zero), transform length L = 2^27 over the three 63-bit primes,
+with operands and product resident in HBM.  value = output limbs (2^27) per
+second over the timed steps.
+
+Also reported:
+  roofline      the dominant kernel's algorithmic HBM bytes per launch over
+                its mean duration (HIP events on its launch stream, recorded
+                by the library around every kernel), against 8 TB/s;
+  cpu_baseline  the oracle (C restatement of the reference's NTT multiply,
+                1 thread) timed on this host on a bounded sample;
+  latency       p50 / p99 of per-step wall time.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+METRIC = "NTT-multiply limbs/s batched vs per-call on GPU; %HBM roofline; p99 latency"
+
+
+def kernel_bytes(name: str, L: int, na: int, nb: int, nops: int) -> float | None:
+    """Algorithmic HBM bytes of one launch: every word the kernel must read
+    or write once (3 primes, 8-byte words)."""
+    K = 3
+    runs = L // 16
+    if name == "fwd_col_first":
+        return 8 * (na + nb if nops == 2 else na) + 8 * K * L * nops
+    if name == "fwd_col":
+        return 16 * K * L * nops
+    if name in ("row_mul",):
+        return 8 * K * L * 3          # read both spectra, write the product spectrum
+    if name in ("row_sqr",):
+        return 8 * K * L * 2
+    if name == "inv_col":
+        return 16 * K * L
+    if name == "inv_col_crt":
+        return 8 * K * L + 8 * (na + nb) + 20 * runs
+    return None
+
+
+def cpu_baseline(sample_bits: int = 20) -> dict:
+    """Time the oracle's mpz-*/ntt restatement on one 2^sample_bits x
+    2^sample_bits multiply, single-threaded."""
+    from oracle import oracle as O
+    O.build()
+    n = 1 << sample_bits
+    a = O.splitmix64(0x48475801, n)
+    b = O.splitmix64(0x48475802, n)
+    t0 = time.perf_counter()
+    O.mpz_mul_ntt(a, b)
+    dt = time.perf_counter() - t0
+    return {
+        "value": (2 * n) / dt,
+        "unit": "limbs/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"one {n}x{n}-limb multiply (L=2^{sample_bits + 1}) through the oracle's restatement of "
+                  f"mpz-*/ntt, 1 thread, {dt:.2f} s; host {platform.processor() or platform.machine()}, "
+                  f"{os.cpu_count()} cpus",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--bits", type=int, default=26, help="operand size 2^bits limbs each")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-bits", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); products are independent per GPU "
+              f"and no multi-GPU path is part of this build", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import hypergeometrica_amd as h
+    from hypergeometrica_amd import _lib
+
+    L_ = _lib.lib(0)
+    n = 1 << args.bits
+    na = nb = n
+    dev = torch.device("cuda:0")
+    A = torch.empty(na, dtype=torch.int64, device=dev)
+    B = torch.empty(nb, dtype=torch.int64, device=dev)
+    OUT = torch.empty(na + nb, dtype=torch.int64, device=dev)
+    h.fill_splitmix64(A.data_ptr(), na, 0x48475801)
+    h.fill_splitmix64(B.data_ptr(), nb, 0x48475802)
+    torch.cuda.synchronize()
+
+    def step():
+        _lib.check(L_.hgx_mul_device(A.data_ptr(), na, B.data_ptr(), nb, OUT.data_ptr(), None))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print(f"# warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+
+    # timed region: K steps, wall clock, synchronised on both sides
+    per_step = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        per_step.append(time.perf_counter() - s0)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    limbs = na + nb
+    value = limbs * args.steps / wall
+    print(f"# timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # per-kernel durations: HIP events on each kernel's launch stream
+    _lib.timing_enable(True)
+    for _ in range(args.steps):
+        step()
+    kt = _lib.timing_read()
+    _lib.timing_enable(False)
+    L = h.product_length(na, nb)
+    kernels = {}
+    for name, (cnt, tot) in kt.items():
+        avg = tot / cnt
+        byts = kernel_bytes(name, L, na, nb, 2)
+        kernels[name] = {"launches_per_step": cnt / args.steps, "avg_ms": round(avg, 4),
+                         "ms_per_step": round(tot / args.steps, 4),
+                         "alg_bytes": byts,
+                         "GBps": round(byts / (avg * 1e-3) / 1e9, 1) if byts else None}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    d = kernels[dom]
+    achieved = d["alg_bytes"] / (d["avg_ms"] * 1e-3) if d["alg_bytes"] else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "kernel": dom,
+                "achieved": round(achieved / 1e9, 1) if achieved else None,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK, 4) if achieved else None,
+                "traffic": traffic}
+    # whole-multiply fraction against the fused-minimum bytes B_mul
+    P = -(-(L.bit_length() - 1) // 12)
+    b_mul = 16 * (na + nb) + 3 * L * (48 * P - 16)
+    per_step_sorted = sorted(per_step)
+    p50 = per_step_sorted[len(per_step_sorted) // 2]
+    p99 = per_step_sorted[min(len(per_step_sorted) - 1, int(0.99 * len(per_step_sorted)))]
+
+    cpu = None
+    if not args.no_cpu:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_bits)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "limbs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": f"single exact multiply, 2^{args.bits} x 2^{args.bits} uniform 64-bit limbs, "
+                               f"L=2^{L.bit_length() - 1}, 3 primes, operands resident in HBM",
+                   "limbs_per_step": limbs, "transform_length": L},
+        "roofline": roofline,
+        "multiply_roofline": {"alg_bytes": b_mul, "frac": round(b_mul / (ms_per_step * 1e-3) / HBM_PEAK, 4)},
+        "latency_ms": {"p50": round(1e3 * p50, 3), "p99": round(1e3 * p99, 3)},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,112 @@
+"""ctypes binding of libhgx.so (the C ABI of include/hgx.h).
+
+There is no CPU path: if the library is missing or no device is usable,
+every call raises HgxError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhgx.so")
+
+HGX_OK, HGX_EINVAL, HGX_ETOOBIG, HGX_ENOMEM, HGX_EDEVICE, HGX_ENOTINIT = 0, -1, -2, -3, -4, -5
+
+# every symbol the header declares (checked by the CPU test suite)
+EXPORTS = (
+    "hgx_init", "hgx_shutdown", "hgx_last_error", "hgx_num_moduli", "hgx_modulus",
+    "hgx_product_length", "hgx_max_transform_bits", "hgx_mul", "hgx_sqr", "hgx_mul_batch",
+    "hgx_mul_device", "hgx_sqr_device", "hgx_mul_batch_device", "hgx_ntt_forward",
+    "hgx_ntt_reverse", "hgx_submit", "hgx_poll", "hgx_wait", "hgx_fill_splitmix64",
+    "hgx_timing_enable", "hgx_timing_read",
+)
+
+
+class HgxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hgx error {code}: {msg}")
+        self.code = code
+
+
+class MulDesc(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_void_p), ("na", ctypes.c_size_t), ("b", ctypes.c_void_p),
+                ("nb", ctypes.c_size_t), ("out", ctypes.c_void_p)]
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double)]
+
+
+_u64 = ctypes.c_uint64
+_sz = ctypes.c_size_t
+_vp = ctypes.c_void_p
+_lib = None
+_init_lock = threading.Lock()
+_initialised = False
+
+
+def load() -> ctypes.CDLL:
+    """Load libhgx.so (no device call)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HgxError(HGX_ENOTINIT, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.hgx_init.argtypes = [ctypes.c_int, _sz]
+        L.hgx_shutdown.argtypes = []
+        L.hgx_shutdown.restype = None
+        L.hgx_last_error.restype = ctypes.c_char_p
+        L.hgx_num_moduli.restype = ctypes.c_int
+        L.hgx_modulus.argtypes = [ctypes.c_int]
+        L.hgx_modulus.restype = _u64
+        L.hgx_product_length.argtypes = [_sz, _sz]
+        L.hgx_product_length.restype = _sz
+        L.hgx_max_transform_bits.restype = ctypes.c_int
+        L.hgx_mul.argtypes = [_vp, _sz, _vp, _sz, _vp]
+        L.hgx_sqr.argtypes = [_vp, _sz, _vp]
+        L.hgx_mul_batch.argtypes = [ctypes.POINTER(MulDesc), _sz]
+        L.hgx_mul_device.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
+        L.hgx_sqr_device.argtypes = [_vp, _sz, _vp, _vp]
+        L.hgx_mul_batch_device.argtypes = [ctypes.POINTER(MulDesc), _sz, _vp]
+        L.hgx_ntt_forward.argtypes = [_vp, _sz, ctypes.c_int]
+        L.hgx_ntt_reverse.argtypes = [_vp, _sz, ctypes.c_int]
+        L.hgx_submit.argtypes = [ctypes.POINTER(MulDesc), ctypes.POINTER(_u64)]
+        L.hgx_poll.argtypes = [_u64]
+        L.hgx_wait.argtypes = [_u64]
+        L.hgx_fill_splitmix64.argtypes = [_vp, _sz, _u64, ctypes.c_int, _vp]
+        L.hgx_timing_enable.argtypes = [ctypes.c_int]
+        L.hgx_timing_read.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        msg = load().hgx_last_error()
+        raise HgxError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def lib(device: int = 0, budget: int = 0) -> ctypes.CDLL:
+    """The library, initialised on DEVICE (once per process)."""
+    global _initialised
+    L = load()
+    if not _initialised:
+        with _init_lock:
+            if not _initialised:
+                check(L.hgx_init(device, budget))
+                _initialised = True
+    return L
+
+
+def timing_enable(on: bool = True) -> None:
+    check(lib().hgx_timing_enable(1 if on else 0))
+
+
+def timing_read() -> dict:
+    """{kernel name: (launches, total ms)} of the calls since timing_enable."""
+    buf = (KernelTime * 64)()
+    n = check(lib().hgx_timing_read(buf, 64))
+    return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms)) for i in range(n)}

@@ -1,0 +1,729 @@
+// hgx_kernels.hpp -- gfx950 kernels of the exact NTT multiply.
+//
+// Transform structure (equivalent to ntt-forward / ntt-reverse,
+// src/number-theoretic-transform.lisp:40-143, same residues out):
+//
+//   A length-L decimation-in-frequency transform with bit-reversed output is
+//   split into passes over HBM.  A "column pass" on blocks of length B treats
+//   each block as a 256 x (B/256) matrix (row m, column t; element
+//   t + (B/256) m), runs a 256-point DIF down every column (output row m'
+//   holds frequency q = bitrev8(m')), and multiplies that element by
+//   w_B^(t q).  Each row of the block is then a contiguous sub-block of
+//   length B/256 whose own bit-reversed DIF finishes the transform, so the
+//   next pass recurses on it; no transposes are needed.  The final "row
+//   pass" transforms contiguous blocks of at most 4096 words in LDS.  The
+//   inverse runs the same passes in reverse with w^-1 (decimation in time,
+//   twiddles applied before each column transform).
+//
+//   Every pass moves one 4096-word tile (32 KiB) per workgroup through LDS:
+//   a coalesced load, radix-16/8/4/2 rounds in registers with LDS exchanges,
+//   and a coalesced store.  Column tiles are 16 adjacent columns x 256 rows
+//   (16 consecutive words = 128 B per row segment); row tiles are 4096
+//   consecutive words.
+//
+// Multiply pipeline (one L, three primes):
+//   k_fwd_col<FIRST>  limbs -> residues mod p_k for all k, first column pass
+//   k_fwd_col         further column passes (in place)
+//   k_row<MUL|SQR>    last forward pass of both operands, pointwise product,
+//                     first inverse pass, per prime
+//   k_inv_col         inverse column passes (in place)
+//   k_inv_col<LAST>   outermost inverse pass for all three primes, CRT
+//                     (Garner form) of every coefficient and the carry sum
+//                     of every 16-limb run, with each run's overflow saved
+//   k_carry_*         resolves the run-to-run carries (block scan)
+//   For L <= 4096 one k_row<..., SINGLE> does the whole transform work.
+#pragma once
+#include "modarith.hpp"
+
+#define HGX_TILE 4096
+#define HGX_NT 256
+#define HGX_EPT 16  // elements of a full tile per thread
+#define HGX_PAD(x) ((x) + ((x) >> 4))
+#define HGX_TILE_LDS (HGX_TILE + HGX_TILE / 16)
+#define HGX_RUN 16  // limbs per carry run
+
+enum { MODE_MUL = 0, MODE_SQR = 1, MODE_FWD = 2, MODE_INV = 3 };
+
+// Moduli and in-tile root tables.  tw_fwd[k][i] = w_4096^i * R mod p_k and
+// tw_inv[k][i] = w_4096^-i * R mod p_k for i < 2048, with
+// w_4096 = g^((p-1)/4096) (primitive-root-from-generator, src/moduli.lisp:182-187).
+struct PrimeSet {
+  u64 p[3];
+  u32 c[3];
+  const u64 *tw_fwd[3];
+  const u64 *tw_inv[3];
+  // Garner constants in Montgomery form: p0^-1 mod p1, p0^-1 mod p2, p1^-1 mod p2
+  u64 g01, g02, g12;
+};
+
+// Two-level root table of one column pass on blocks of length 2^b:
+// w_B^e = TH[e >> lb] * TL[e & (2^lb - 1)], both in Montgomery form.
+struct ColTw {
+  const u64 *TL[3];
+  const u64 *TH[3];
+  int lb;
+};
+
+// One multiplication as the kernels see it: operands, product and work
+// arrays in device memory.  A launch covers one job (passed by value) or a
+// table of jobs of the same transform length (a batch); blockIdx.x selects
+// job = blockIdx.x >> jt_bits and its tile = blockIdx.x & (2^jt_bits - 1).
+struct MulJob {
+  const u64 *a;
+  const u64 *b;
+  u64 na, nb;
+  u64 *out;       // product limbs
+  u64 nout;       // limbs to store (na + nb); positions at or above are zero
+  u64 *res0;      // 3 L residues of operand a (prime k at + k L); the product spectrum lands here
+  u64 *res1;      // 3 L residues of operand b
+  u64 *ovf;       // 2 words per run: what the run's sum carries past its 16 limbs
+  u32 *flags;     // per run: limbs 2..15 of the run are all ones
+  uint8_t *maps;  // per run: carry map (kill / propagate / generate)
+  uint8_t *agg;   // per block of 1024 runs: combined map
+  uint8_t *blkin; // per block of 1024 runs: carry into the block
+};
+
+struct JobRef {
+  MulJob one;
+  const MulJob *table;  // null: every block works on ONE
+};
+
+__device__ __forceinline__ MulJob load_job(const JobRef &jr, u32 j) { return jr.table ? jr.table[j] : jr.one; }
+
+__device__ __forceinline__ u32 bitrev8(u32 m) { return __builtin_bitreverse32(m) >> 24; }
+
+__device__ __forceinline__ u64 twiddle2(const u64 *TH, const u64 *TL, int lb, u64 e, u64 p, u32 c) {
+  u64 hi = TH[e >> lb];
+  u64 lo = TL[e & ((1ull << lb) - 1)];
+  return mont_mul(hi, lo, p, c);
+}
+
+// ---------------------------------------------------------------------------
+// In-tile rounds.  A round does R radix-2 stages (A .. A+R-1 of S_BITS) on
+// groups of 2^R elements spaced D = S >> (A+R) rows apart, one group per
+// thread iteration, values in registers.
+// ---------------------------------------------------------------------------
+template <int S_BITS>
+struct RoundPlan {
+  static constexpr int NR = (S_BITS + 3) / 4;
+  static constexpr int bits(int i) { return S_BITS / NR + (i < S_BITS % NR ? 1 : 0); }
+  static constexpr int start(int i) {
+    int a = 0;
+    for (int k = 0; k < i; k++) a += bits(k);
+    return a;
+  }
+};
+
+template <int V>
+struct Log2 { static constexpr int value = 1 + Log2<V / 2>::value; };
+template <>
+struct Log2<1> { static constexpr int value = 0; };
+
+// COLFAST: column tile (16 columns, LDS index m*16 + t, lanes run along t).
+// Otherwise: row tile (LDS index t*S + m, lanes run along m).
+template <int S_BITS, int A, int R, bool INV, bool COLFAST>
+__device__ __forceinline__ void tile_round(u64 *lds, int wbits, const u64 *__restrict__ tw, u64 p, u32 c) {
+  constexpr int S = 1 << S_BITS;
+  constexpr int D = S >> (A + R);
+  constexpr int LD = Log2<D>::value;
+  constexpr int RR = 1 << R;
+  constexpr int GPC_BITS = S_BITS - R;
+  const int ngroups = 1 << (wbits + GPC_BITS);
+  for (int gid = threadIdx.x; gid < ngroups; gid += HGX_NT) {
+    int t, g;
+    if (COLFAST) {
+      t = gid & ((1 << wbits) - 1);
+      g = gid >> wbits;
+    } else {
+      g = gid & ((1 << GPC_BITS) - 1);
+      t = gid >> GPC_BITS;
+    }
+    const int goff = g & (D - 1);
+    const int base = ((g >> LD) << (LD + R)) + goff;
+    u64 x[RR];
+#pragma unroll
+    for (int j = 0; j < RR; j++) {
+      const int m = base + j * D;
+      const int li = COLFAST ? (m * 16 + t) : (t * S + m);
+      x[j] = lds[HGX_PAD(li)];
+    }
+    if (!INV) {
+#pragma unroll
+      for (int l = 0; l < R; l++) {
+        const int half = 1 << (R - 1 - l);
+        const int h = half * D;
+        const int tstride = 2048 / h;
+#pragma unroll
+        for (int j = 0; j < RR; j++) {
+          if (j & half) continue;
+          const u64 u = x[j], v = x[j + half];
+          x[j] = mod_add(u, v, p);
+          if (D == 1 && (j & (half - 1)) == 0) {
+            x[j + half] = mod_sub(u, v, p);  // twiddle w^0 = 1
+          } else {
+            const int e = goff + (j & (half - 1)) * D;
+            const u64 w = tw[e * tstride];
+            x[j + half] = mont_mul(u + (p - v), w, p, c);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int l = R - 1; l >= 0; l--) {
+        const int half = 1 << (R - 1 - l);
+        const int h = half * D;
+        const int tstride = 2048 / h;
+#pragma unroll
+        for (int j = 0; j < RR; j++) {
+          if (j & half) continue;
+          const u64 u = x[j];
+          u64 v;
+          if (D == 1 && (j & (half - 1)) == 0) {
+            v = x[j + half];  // twiddle w^0 = 1
+          } else {
+            const int e = goff + (j & (half - 1)) * D;
+            v = mont_mul(x[j + half], tw[e * tstride], p, c);
+          }
+          x[j] = mod_add(u, v, p);
+          x[j + half] = mod_sub(u, v, p);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < RR; j++) {
+      const int m = base + j * D;
+      const int li = COLFAST ? (m * 16 + t) : (t * S + m);
+      lds[HGX_PAD(li)] = x[j];
+    }
+  }
+}
+
+template <int S_BITS, int I, bool COLFAST>
+struct FwdRounds {
+  static __device__ __forceinline__ void run(u64 *lds, int wbits, const u64 *tw, u64 p, u32 c) {
+    if constexpr (I < RoundPlan<S_BITS>::NR) {
+      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), false, COLFAST>(lds, wbits, tw, p, c);
+      __syncthreads();
+      FwdRounds<S_BITS, I + 1, COLFAST>::run(lds, wbits, tw, p, c);
+    }
+  }
+};
+
+template <int S_BITS, int I, bool COLFAST>
+struct InvRounds {  // I counts down from NR-1
+  static __device__ __forceinline__ void run(u64 *lds, int wbits, const u64 *tw, u64 p, u32 c) {
+    if constexpr (I >= 0) {
+      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), true, COLFAST>(lds, wbits, tw, p, c);
+      __syncthreads();
+      InvRounds<S_BITS, I - 1, COLFAST>::run(lds, wbits, tw, p, c);
+    }
+  }
+};
+
+template <int S_BITS, bool COLFAST>
+__device__ __forceinline__ void tile_forward(u64 *lds, int wbits, const u64 *tw, u64 p, u32 c) {
+  FwdRounds<S_BITS, 0, COLFAST>::run(lds, wbits, tw, p, c);
+}
+template <int S_BITS, bool COLFAST>
+__device__ __forceinline__ void tile_inverse(u64 *lds, int wbits, const u64 *tw, u64 p, u32 c) {
+  InvRounds<S_BITS, RoundPlan<S_BITS>::NR - 1, COLFAST>::run(lds, wbits, tw, p, c);
+}
+
+// ---------------------------------------------------------------------------
+// CRT of one coefficient (Garner form) and the carry sum of one run.
+// c = r0 + p0 (v1 + p1 v2) is the unique value below p0 p1 p2 with the given
+// residues, i.e. the same number as the reference's
+// (mod (sum f_k r_k) composite), src/ntt-multiply.lisp:242-252.
+// ---------------------------------------------------------------------------
+struct U192 { u64 w0, w1, w2; };
+
+__device__ __forceinline__ U192 garner3(u64 r0, u64 r1, u64 r2, const PrimeSet &ps) {
+  const u64 p0 = ps.p[0], p1 = ps.p[1], p2 = ps.p[2];
+  // v1 = (r1 - r0) p0^-1 mod p1
+  const u64 r0m1 = (r0 >= p1) ? r0 - p1 : r0;
+  const u64 v1 = mont_mul(mod_sub(r1, r0m1, p1), ps.g01, p1, ps.c[1]);
+  // v2 = ((r2 - r0) p0^-1 - v1) p1^-1 mod p2
+  const u64 r0m2 = (r0 >= p2) ? r0 - p2 : r0;
+  const u64 v1m2 = (v1 >= p2) ? v1 - p2 : v1;
+  u64 t = mont_mul(mod_sub(r2, r0m2, p2), ps.g02, p2, ps.c[2]);
+  const u64 v2 = mont_mul(mod_sub(t, v1m2, p2), ps.g12, p2, ps.c[2]);
+  // u = v1 + p1 * v2  (< p1 p2 < 2^126)
+  u64 ulo, uhi;
+  mul64x64(p1, v2, ulo, uhi);
+  ulo += v1;
+  uhi += (ulo < v1) ? 1 : 0;
+  // c = r0 + p0 * u  (< p0 p1 p2 < 2^188)
+  u64 a0, a1, b0, b1;
+  mul64x64(p0, ulo, a0, a1);
+  mul64x64(p0, uhi, b0, b1);
+  U192 c;
+  c.w0 = a0 + r0;
+  u64 cy = (c.w0 < a0) ? 1 : 0;
+  u64 m1 = a1 + b0;
+  u64 cy2 = (m1 < a1) ? 1 : 0;
+  c.w1 = m1 + cy;
+  cy2 += (c.w1 < m1) ? 1 : 0;
+  c.w2 = b1 + cy2;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Carry sum of one run of up to 16 coefficients given their three residues:
+// writes the run's digits into LDS (run-major) and returns the overflow past
+// the run.  add-big-digit, src/ntt-multiply.lisp:10-51, adds the same
+// coefficients at the same positions.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void run_crt_sum(const u64 (&r)[3][HGX_RUN], int rl, const PrimeSet &ps, u64 *lds,
+                                            int run, u64 &ov0, u64 &ov1, u32 &allones) {
+  u64 a0 = 0, a1 = 0;
+  u64 andhi = ~0ull;
+#pragma unroll
+  for (int j = 0; j < HGX_RUN; j++) {
+    if (j < rl) {
+      const U192 cf = garner3(r[0][j], r[1][j], r[2][j], ps);
+      const u64 s0 = a0 + cf.w0;
+      const u64 cy = (s0 < a0) ? 1 : 0;
+      const u64 s1 = a1 + cf.w1;
+      u64 cy1 = (s1 < a1) ? 1 : 0;
+      const u64 s1b = s1 + cy;
+      cy1 += (s1b < s1) ? 1 : 0;
+      lds[HGX_PAD(run * rl + j)] = s0;
+      if (j >= 2) andhi &= s0;
+      a0 = s1b;
+      a1 = cf.w2 + cy1;
+    }
+  }
+  ov0 = a0;
+  ov1 = a1;
+  allones = (andhi == ~0ull) ? 1u : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Column pass, forward.  Tile = 16 adjacent columns of one block of length
+// 2^bbits.  FIRST: the input is the operand's limbs (zero above its length);
+// each workgroup reduces them mod every prime (as %make-ntt-work,
+// src/number-theoretic-transform.lisp:7-22) and writes all three residue
+// arrays.  Otherwise: in place on the residue array of prime kbase + y.
+// grid = (jobs << jt_bits, FIRST ? 1 : nprimes, noperands)
+template <bool FIRST>
+__global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
+                                                      int kbase) {
+  __shared__ u64 lds[HGX_TILE_LDS];
+  const int tid = threadIdx.x;
+  const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
+  const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
+  const int tpb_bits = bbits - 12;  // tiles per block
+  const u64 blk = tile >> tpb_bits;
+  const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
+  const u64 rs = 1ull << (bbits - 8);
+  const u64 gbase = (blk << bbits) + t0;
+  const int op = blockIdx.z;
+  u64 *res = op ? J.res1 : J.res0;
+
+  if (FIRST) {
+    const u64 *limbs = op ? J.b : J.a;
+    const u64 n = op ? J.nb : J.na;
+    u64 v[HGX_EPT];
+#pragma unroll
+    for (int j = 0; j < HGX_EPT; j++) {
+      const int x = tid + HGX_NT * j;
+      const u64 pos = gbase + (x & 15) + rs * (u64)(x >> 4);
+      v[j] = (pos < n) ? limbs[pos] : 0;
+    }
+    for (int k = 0; k < 3; k++) {
+      const u64 p = ps.p[k];
+      const u32 c = ps.c[k];
+#pragma unroll
+      for (int j = 0; j < HGX_EPT; j++) {
+        const int x = tid + HGX_NT * j;
+        lds[HGX_PAD(x)] = mod_reduce64(v[j], p);
+      }
+      __syncthreads();
+      tile_forward<8, true>(lds, 4, ps.tw_fwd[k], p, c);
+      u64 *dst = res + (u64)k * L;
+#pragma unroll
+      for (int j = 0; j < HGX_EPT; j++) {
+        const int x = tid + HGX_NT * j;
+        const u32 m = x >> 4;
+        const u64 t = t0 + (x & 15);
+        const u64 e = t * bitrev8(m);
+        const u64 w = twiddle2(tw.TH[k], tw.TL[k], tw.lb, e, p, c);
+        dst[gbase + (x & 15) + rs * m] = mont_mul(lds[HGX_PAD(x)], w, p, c);
+      }
+      __syncthreads();
+    }
+  } else {
+    const int k = kbase + blockIdx.y;
+    const u64 p = ps.p[k];
+    const u32 c = ps.c[k];
+    u64 *arr = res + (u64)blockIdx.y * L;
+#pragma unroll
+    for (int j = 0; j < HGX_EPT; j++) {
+      const int x = tid + HGX_NT * j;
+      lds[HGX_PAD(x)] = arr[gbase + (x & 15) + rs * (u64)(x >> 4)];
+    }
+    __syncthreads();
+    tile_forward<8, true>(lds, 4, ps.tw_fwd[k], p, c);
+#pragma unroll
+    for (int j = 0; j < HGX_EPT; j++) {
+      const int x = tid + HGX_NT * j;
+      const u32 m = x >> 4;
+      const u64 t = t0 + (x & 15);
+      const u64 e = t * bitrev8(m);
+      const u64 w = twiddle2(tw.TH[k], tw.TL[k], tw.lb, e, p, c);
+      arr[gbase + (x & 15) + rs * m] = mont_mul(lds[HGX_PAD(x)], w, p, c);
+    }
+  }
+}
+
+// Column pass, inverse.  Twiddles w_B^-(t q) (times the final scale on the
+// outermost pass) are applied at load, then the 256-point DIT.  LAST: all
+// three primes are transformed in turn, each 16-column row of the tile is a
+// run of 16 consecutive coefficients, and the CRT + carry sum is fused in.
+// grid = (jobs << jt_bits, LAST ? 1 : nprimes)
+template <bool LAST>
+__global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
+                                                      int kbase) {
+  __shared__ u64 lds[HGX_TILE_LDS];
+  const int tid = threadIdx.x;
+  const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
+  const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
+  const int tpb_bits = bbits - 12;
+  const u64 blk = tile >> tpb_bits;
+  const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
+  const u64 rs = 1ull << (bbits - 8);
+  const u64 gbase = (blk << bbits) + t0;
+
+  if (!LAST) {
+    const int k = kbase + blockIdx.y;
+    const u64 p = ps.p[k];
+    const u32 c = ps.c[k];
+    u64 *arr = J.res0 + (u64)blockIdx.y * L;
+#pragma unroll
+    for (int j = 0; j < HGX_EPT; j++) {
+      const int x = tid + HGX_NT * j;
+      const u32 m = x >> 4;
+      const u64 t = t0 + (x & 15);
+      const u64 e = t * bitrev8(m);
+      const u64 w = twiddle2(tw.TH[k], tw.TL[k], tw.lb, e, p, c);
+      lds[HGX_PAD(x)] = mont_mul(arr[gbase + (x & 15) + rs * m], w, p, c);
+    }
+    __syncthreads();
+    tile_inverse<8, true>(lds, 4, ps.tw_inv[k], p, c);
+#pragma unroll
+    for (int j = 0; j < HGX_EPT; j++) {
+      const int x = tid + HGX_NT * j;
+      arr[gbase + (x & 15) + rs * (u64)(x >> 4)] = lds[HGX_PAD(x)];
+    }
+  } else {
+    u64 r[3][HGX_RUN];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const u64 p = ps.p[k];
+      const u32 c = ps.c[k];
+      const u64 *arr = J.res0 + (u64)k * L;
+#pragma unroll
+      for (int j = 0; j < HGX_EPT; j++) {
+        const int x = tid + HGX_NT * j;
+        const u32 m = x >> 4;
+        const u64 t = t0 + (x & 15);
+        const u64 e = t * bitrev8(m);
+        const u64 w = twiddle2(tw.TH[k], tw.TL[k], tw.lb, e, p, c);
+        lds[HGX_PAD(x)] = mont_mul(arr[gbase + (x & 15) + rs * m], w, p, c);
+      }
+      __syncthreads();
+      tile_inverse<8, true>(lds, 4, ps.tw_inv[k], p, c);
+      // run tid = tile row tid = 16 consecutive coefficients
+#pragma unroll
+      for (int j = 0; j < HGX_RUN; j++) r[k][j] = lds[HGX_PAD(tid * 16 + j)];
+      __syncthreads();
+    }
+    u64 ov0, ov1;
+    u32 allones;
+    run_crt_sum(r, 16, ps, lds, tid, ov0, ov1, allones);
+    const u64 run_g = (gbase + rs * (u64)tid) >> 4;
+    J.ovf[2 * run_g] = ov0;
+    J.ovf[2 * run_g + 1] = ov1;
+    J.flags[run_g] = allones;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < HGX_EPT; j++) {
+      const int x = tid + HGX_NT * j;
+      const u64 pos = gbase + (x & 15) + rs * (u64)(x >> 4);
+      if (pos < J.nout) J.out[pos] = lds[HGX_PAD(x)];
+    }
+  }
+}
+
+// Row pass: contiguous blocks of 2^S_BITS words; a tile is
+// min(L, 4096) consecutive words (2^tw_bits).
+//   MODE_MUL / MODE_SQR, !SINGLE: last forward pass of both operands (or the
+//     one operand), pointwise product (a*b/R by REDC), first inverse pass.
+//   MODE_FWD / MODE_INV: transform-only row pass on res0 (prime kbase + y).
+//   SINGLE (L <= 4096): the limbs come in, all three primes are done in
+//     turn, and the CRT + carry sum goes out (as k_inv_col<true>).
+struct RowParams {
+  u64 L;
+  int tw_bits;
+  int kbase;
+  u64 scale[3];  // Montgomery-form factor applied to the pointwise product when SINGLE
+};
+
+template <int S_BITS, int MODE, bool SINGLE>
+__global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRef jr, int jt_bits) {
+  constexpr bool TWO = (MODE == MODE_MUL);
+  __shared__ u64 ldsA[HGX_TILE_LDS];
+  __shared__ u64 ldsB[TWO ? HGX_TILE_LDS : 1];
+  const int tid = threadIdx.x;
+  const int TW = 1 << rp.tw_bits;
+  const int wbits = rp.tw_bits - S_BITS;
+  const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
+  const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
+  const u64 gbase = tile << rp.tw_bits;
+
+  if (!SINGLE) {
+    const int k = rp.kbase + blockIdx.y;
+    const u64 p = ps.p[k];
+    const u32 c = ps.c[k];
+    u64 *A = J.res0 + (u64)blockIdx.y * rp.L + gbase;
+    u64 *Bp = TWO ? (J.res1 + (u64)blockIdx.y * rp.L + gbase) : nullptr;
+    for (int x = tid; x < TW; x += HGX_NT) {
+      ldsA[HGX_PAD(x)] = A[x];
+      if (TWO) ldsB[HGX_PAD(x)] = Bp[x];
+    }
+    __syncthreads();
+    if (MODE == MODE_MUL || MODE == MODE_SQR || MODE == MODE_FWD) {
+      tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], p, c);
+      if (TWO) tile_forward<S_BITS, false>(ldsB, wbits, ps.tw_fwd[k], p, c);
+    }
+    if (MODE == MODE_MUL || MODE == MODE_SQR) {
+      for (int x = tid; x < TW; x += HGX_NT) {
+        const u64 a = ldsA[HGX_PAD(x)];
+        const u64 b = TWO ? ldsB[HGX_PAD(x)] : a;
+        ldsA[HGX_PAD(x)] = mont_mul(a, b, p, c);
+      }
+      __syncthreads();
+    }
+    if (MODE != MODE_FWD) tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], p, c);
+    for (int x = tid; x < TW; x += HGX_NT) A[x] = ldsA[HGX_PAD(x)];
+  } else {
+    const int rl = TW < HGX_RUN ? TW : HGX_RUN;
+    const int nruns = TW / rl;
+    u64 r[3][HGX_RUN];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const u64 p = ps.p[k];
+      const u32 c = ps.c[k];
+      for (int x = tid; x < TW; x += HGX_NT) {
+        const u64 pos = gbase + x;
+        ldsA[HGX_PAD(x)] = (pos < J.na) ? mod_reduce64(J.a[pos], p) : 0;
+        if (TWO) ldsB[HGX_PAD(x)] = (pos < J.nb) ? mod_reduce64(J.b[pos], p) : 0;
+      }
+      __syncthreads();
+      tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], p, c);
+      if (TWO) tile_forward<S_BITS, false>(ldsB, wbits, ps.tw_fwd[k], p, c);
+      for (int x = tid; x < TW; x += HGX_NT) {
+        const u64 a = ldsA[HGX_PAD(x)];
+        const u64 b = TWO ? ldsB[HGX_PAD(x)] : a;
+        ldsA[HGX_PAD(x)] = mont_mul(mont_mul(a, b, p, c), rp.scale[k], p, c);
+      }
+      __syncthreads();
+      tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], p, c);
+      if (tid < nruns) {
+#pragma unroll
+        for (int j = 0; j < HGX_RUN; j++)
+          if (j < rl) r[k][j] = ldsA[HGX_PAD(tid * rl + j)];
+      }
+      __syncthreads();
+    }
+    u64 ov0 = 0, ov1 = 0;
+    u32 allones = 0;
+    if (tid < nruns) {
+      run_crt_sum(r, rl, ps, ldsA, tid, ov0, ov1, allones);
+      const u64 run_g = (gbase / rl) + tid;
+      J.ovf[2 * run_g] = ov0;
+      J.ovf[2 * run_g + 1] = ov1;
+      J.flags[run_g] = allones;
+    }
+    __syncthreads();
+    for (int x = tid; x < TW; x += HGX_NT) {
+      const u64 pos = gbase + x;
+      if (pos < J.nout) J.out[pos] = ldsA[HGX_PAD(x)];
+    }
+  }
+}
+
+// Transform-only inverse when L <= 4096 (ntt-reverse on one tile, with the
+// 1/L factor applied at the end).
+template <int S_BITS>
+__global__ __launch_bounds__(HGX_NT) void k_row_inv_scaled(PrimeSet ps, RowParams rp, JobRef jr) {
+  __shared__ u64 ldsA[HGX_TILE_LDS];
+  const int tid = threadIdx.x;
+  const int TW = 1 << rp.tw_bits;
+  const int wbits = rp.tw_bits - S_BITS;
+  const int k = rp.kbase;
+  const u64 p = ps.p[k];
+  const u32 c = ps.c[k];
+  u64 *A = jr.one.res0 + ((u64)blockIdx.x << rp.tw_bits);
+  for (int x = tid; x < TW; x += HGX_NT) ldsA[HGX_PAD(x)] = A[x];
+  __syncthreads();
+  tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], p, c);
+  for (int x = tid; x < TW; x += HGX_NT) A[x] = mont_mul(ldsA[HGX_PAD(x)], rp.scale[k], p, c);
+}
+
+// ---------------------------------------------------------------------------
+// Run-to-run carry resolution.  Run r holds digits D (16 limbs, written by the
+// CRT pass) and receives X = ovf[r-1] plus a carry bit e from run r-1.  Its
+// carry out is 1 iff limbs 2..15 are all ones and D_lo2 + X + e >= 2^128, so
+// each run is a map e -> e' that kills (0), propagates (2) or generates (3).
+// The carry into a run is the value of the nearest non-propagating map before
+// it (0 if none): a last-non-P scan, done per 1024 runs and then across blocks.
+// ---------------------------------------------------------------------------
+#define HGX_CARRY_RUNS_PER_THREAD 4
+#define HGX_CARRY_RUNS_PER_BLOCK (HGX_NT * HGX_CARRY_RUNS_PER_THREAD)
+
+__device__ __forceinline__ u32 run_map(const u64 *out, u64 n_out, const u64 *ovf, const u32 *flags, u64 r,
+                                       int rl) {
+  u64 x0 = 0, x1 = 0;
+  if (r > 0) {
+    x0 = ovf[2 * (r - 1)];
+    x1 = ovf[2 * (r - 1) + 1];
+  }
+  const u64 base = r * (u64)rl;
+  const u64 d0 = (base < n_out) ? out[base] : 0;
+  const u64 d1 = (rl > 1 && base + 1 < n_out) ? out[base + 1] : 0;
+  const u32 hi_ones = flags[r];
+  u64 s0 = d0 + x0;
+  u64 cy = (s0 < d0) ? 1 : 0;
+  u64 s1 = d1 + x1;
+  u64 c1 = (s1 < d1) ? 1 : 0;
+  u64 s1b = s1 + cy;
+  c1 += (s1b < s1) ? 1 : 0;
+  const u32 f0 = c1 ? 1u : 0u;
+  const u32 f1 = (c1 || (s0 == ~0ull && s1b == ~0ull)) ? 1u : 0u;
+  if (!hi_ones) return 0u;
+  return f0 | (f1 << 1);
+}
+
+// combine(earlier, later): later unless it propagates
+__device__ __forceinline__ u32 map_combine(u32 a, u32 b) { return (b == 2u) ? a : b; }
+
+__device__ u32 block_exclusive_last(u32 v, u32 *sh) {
+  // inclusive Hillis-Steele scan of map_combine over the block, then shift
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    u32 other = (tid >= off) ? sh[tid - off] : 2u;
+    __syncthreads();
+    sh[tid] = map_combine(other, sh[tid]);
+    __syncthreads();
+  }
+  u32 ex = (tid > 0) ? sh[tid - 1] : 2u;
+  __syncthreads();
+  return ex;
+}
+
+// per-run maps and each block's combined map.
+// grid = jobs << bpj_bits (bpj = blocks of 1024 runs per job)
+__global__ __launch_bounds__(HGX_NT) void k_carry_map(JobRef jr, int bpj_bits, u64 nruns, int rl) {
+  __shared__ u32 sh[HGX_NT];
+  const int tid = threadIdx.x;
+  const MulJob J = load_job(jr, blockIdx.x >> bpj_bits);
+  const u64 blk = blockIdx.x & ((1u << bpj_bits) - 1);
+  const u64 r0 = blk * HGX_CARRY_RUNS_PER_BLOCK + (u64)tid * HGX_CARRY_RUNS_PER_THREAD;
+  u32 summary = 2u;
+#pragma unroll
+  for (int i = 0; i < HGX_CARRY_RUNS_PER_THREAD; i++) {
+    const u64 r = r0 + i;
+    if (r < nruns) {
+      const u32 m = run_map(J.out, J.nout, J.ovf, J.flags, r, rl);
+      J.maps[r] = (uint8_t)m;
+      summary = map_combine(summary, m);
+    }
+  }
+  sh[tid] = summary;
+  __syncthreads();
+  for (int s = 1; s < HGX_NT; s <<= 1) {
+    if ((tid & (2 * s - 1)) == 0) sh[tid] = map_combine(sh[tid], sh[tid + s]);
+    __syncthreads();
+  }
+  if (tid == 0) J.agg[blk] = (uint8_t)sh[0];
+}
+
+// carry into every block of runs; one workgroup of 1024 threads per job
+__global__ __launch_bounds__(1024) void k_carry_blocks(JobRef jr, u64 nblocks) {
+  __shared__ u32 sh[1024];
+  const int tid = threadIdx.x;
+  const MulJob J = load_job(jr, blockIdx.x);
+  const u64 chunk = (nblocks + 1023) / 1024;
+  const u64 b0 = (u64)tid * chunk;
+  u64 b1 = b0 + chunk;
+  if (b1 > nblocks) b1 = nblocks;
+  u32 summary = 2u;
+  for (u64 b = b0; b < b1; b++) summary = map_combine(summary, J.agg[b]);
+  u32 state = block_exclusive_last(summary, sh);
+  for (u64 b = b0; b < b1; b++) {
+    J.blkin[b] = (state == 3u) ? 1 : 0;
+    state = map_combine(state, J.agg[b]);
+  }
+}
+
+// add ovf[r-1] and the carry bit into every run (grid as k_carry_map)
+__global__ __launch_bounds__(HGX_NT) void k_carry_apply(JobRef jr, int bpj_bits, u64 nruns, int rl) {
+  __shared__ u32 sh[HGX_NT];
+  const int tid = threadIdx.x;
+  const MulJob J = load_job(jr, blockIdx.x >> bpj_bits);
+  const u64 blk = blockIdx.x & ((1u << bpj_bits) - 1);
+  const u64 r0 = blk * HGX_CARRY_RUNS_PER_BLOCK + (u64)tid * HGX_CARRY_RUNS_PER_THREAD;
+  u32 summary = 2u;
+#pragma unroll
+  for (int i = 0; i < HGX_CARRY_RUNS_PER_THREAD; i++) {
+    const u64 r = r0 + i;
+    if (r < nruns) summary = map_combine(summary, J.maps[r]);
+  }
+  const u32 ex = block_exclusive_last(summary, sh);
+  u64 e = (ex == 2u) ? (u64)J.blkin[blk] : (ex == 3u ? 1ull : 0ull);
+  for (int i = 0; i < HGX_CARRY_RUNS_PER_THREAD; i++) {
+    const u64 r = r0 + i;
+    if (r >= nruns) break;
+    u64 x0 = 0, x1 = 0;
+    if (r > 0) {
+      x0 = J.ovf[2 * (r - 1)];
+      x1 = J.ovf[2 * (r - 1) + 1];
+    }
+    const u64 base = r * (u64)rl;
+    if (x0 | x1 | e) {
+      u64 carry = e;
+      for (int j = 0; j < rl; j++) {
+        const u64 pos = base + j;
+        if (pos >= J.nout) break;
+        const u64 add = (j == 0) ? x0 : (j == 1 ? x1 : 0);
+        if (j >= 2 && carry == 0) break;
+        const u64 d = J.out[pos];
+        const u64 s = d + add;
+        u64 cy = (s < d) ? 1 : 0;
+        const u64 s2 = s + carry;
+        cy += (s2 < s) ? 1 : 0;
+        J.out[pos] = s2;
+        carry = cy;
+      }
+    }
+    e = (J.maps[r] >> e) & 1u;
+  }
+}
+
+// SplitMix64 stream: out[i] = mix(seed + (i+1) * 0x9E3779B97F4A7C15)
+__global__ void k_splitmix64(u64 *out, u64 n, u64 seed, int force_top) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u64 z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  if (force_top && i == n - 1 && z == 0) z = 1;
+  out[i] = z;
+}

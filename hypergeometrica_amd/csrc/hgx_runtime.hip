@@ -1,0 +1,946 @@
+// hgx_runtime.hip -- host side of the C ABI declared in include/hgx.h.
+//
+// One device context per process: the prime set and its root tables are
+// built once at hgx_init and stay resident (the reference builds **scheme**
+// once at load, src/ntt-multiply.lisp:8); per-transform-length plans (the
+// two-level root tables of the column passes) are built on first use and
+// cached; work arrays grow on demand under the HBM budget.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/hgx.h"
+#include "hgx_kernels.hpp"
+
+typedef unsigned __int128 u128;
+
+namespace {
+
+constexpr int kMaxBits = 32;   // largest transform: 2^32 words per residue array
+constexpr int kRowMaxBits = 12;
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(HGX_EDEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- host modular arithmetic (table construction only) ----
+u64 mulmod(u64 a, u64 b, u64 m) { return (u64)(((u128)a * b) % m); }
+u64 powmod(u64 a, u64 e, u64 m) {
+  u64 r = 1 % m;
+  a %= m;
+  while (e) {
+    if (e & 1) r = mulmod(r, a, m);
+    a = mulmod(a, a, m);
+    e >>= 1;
+  }
+  return r;
+}
+u64 invmod(u64 a, u64 m) { return powmod(a, m - 2, m); }  // m prime
+u64 to_mont(u64 a, u64 m) { return (u64)((((u128)(a % m)) << 64) % m); }
+
+bool is_prime_small(u64 n) {
+  if (n < 2) return false;
+  for (u64 d = 2; d * d <= n; d++)
+    if (n % d == 0) return false;
+  return true;
+}
+
+// smallest prime generator of Z/pZ* (find-finite-field-generator,
+// src/moduli.lisp:173-180); p - 1 = c * 2^55 with small c.
+u64 find_generator(u64 p, u64 c) {
+  std::vector<u64> f{2};
+  u64 r = c;
+  for (u64 d = 2; d * d <= r; d++)
+    if (r % d == 0) {
+      if (d != 2) f.push_back(d);
+      while (r % d == 0) r /= d;
+    }
+  if (r > 1 && r != 2) f.push_back(r);
+  for (u64 g = 2;; g++) {
+    if (!is_prime_small(g)) continue;
+    bool ok = true;
+    for (u64 q : f)
+      if (powmod(g, (p - 1) / q, p) == 1) {
+        ok = false;
+        break;
+      }
+    if (ok) return g;
+  }
+}
+
+// The moduli of default-moduli (src/moduli.lisp:257-264): the 63-bit primes
+// i * 2^55 + 1, descending.
+const u64 kModuli[3] = {0x6280000000000001ull, 0x5700000000000001ull, 0x4180000000000001ull};
+
+struct ColPassTables {
+  int bbits = 0, lb = 0;
+  u64 *TLf[3], *THf[3], *TLi[3], *THi[3], *THi_mul[3], *THi_tr[3];
+};
+
+struct Plan {
+  int n = 0, ncol = 0, srow = 0;
+  ColPassTables col[8];
+  u64 *dev = nullptr;  // one allocation holding every table
+  size_t bytes = 0;
+};
+
+struct Workspace {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct Context {
+  bool ready = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ws_done = nullptr;  // last use of the shared workspace
+  size_t budget = 0, allocated = 0;
+  PrimeSet ps{};
+  u64 gen[3]{};
+  u64 *tables = nullptr;
+  std::map<int, std::unique_ptr<Plan>> plans;
+  Workspace ws, stage, jobs_dev, pack;
+  MulJob *jobs_host = nullptr;  // pinned job-table staging
+  size_t jobs_host_cap = 0;
+  hipEvent_t jobs_copied = nullptr;
+  std::recursive_mutex mu;
+};
+
+Context g_ctx;
+
+// ---- per-kernel timing (benchmarks) ----
+struct TimedLaunch {
+  const char *name;
+  hipEvent_t a, b;
+};
+bool g_timing = false;
+std::vector<TimedLaunch> g_pending;
+std::map<std::string, std::pair<uint64_t, double>> g_kstats;
+
+struct KTimer {
+  hipStream_t s;
+  bool on;
+  KTimer(const char *name, hipStream_t st) : s(st), on(g_timing) {
+    if (!on) return;
+    TimedLaunch t{name, nullptr, nullptr};
+    (void)hipEventCreate(&t.a);
+    (void)hipEventCreate(&t.b);
+    (void)hipEventRecord(t.a, s);
+    g_pending.push_back(t);
+  }
+  ~KTimer() {
+    if (on) (void)hipEventRecord(g_pending.back().b, s);
+  }
+};
+
+void timing_drain() {
+  for (auto &t : g_pending) {
+    (void)hipEventSynchronize(t.b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t.a, t.b);
+    auto &st = g_kstats[t.name];
+    st.first += 1;
+    st.second += ms;
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  g_pending.clear();
+}
+
+#define HGX_TIMED(name, s) KTimer kt_##__LINE__(name, s)
+
+int dev_alloc(void **p, size_t bytes) {
+  if (g_ctx.allocated + bytes > g_ctx.budget)
+    return fail(HGX_ENOMEM, "HBM budget exceeded: need " + std::to_string(bytes) + " more bytes, " +
+                                std::to_string(g_ctx.budget - g_ctx.allocated) + " left");
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(HGX_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  g_ctx.allocated += bytes;
+  return HGX_OK;
+}
+
+void dev_free(void *p, size_t bytes) {
+  if (!p) return;
+  hipFree(p);
+  g_ctx.allocated -= bytes;
+}
+
+int ensure(Workspace &w, size_t bytes) {
+  if (w.bytes >= bytes) return HGX_OK;
+  HIPCHK(hipEventSynchronize(g_ctx.ws_done));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  dev_free(w.ptr, w.bytes);
+  w.ptr = nullptr;
+  w.bytes = 0;
+  size_t want = std::max(bytes, (size_t)1 << 20);
+  int rc = dev_alloc(&w.ptr, want);
+  if (rc) return rc;
+  w.bytes = want;
+  return HGX_OK;
+}
+
+// ---- plans ----
+int build_plan(int n, Plan **out) {
+  auto it = g_ctx.plans.find(n);
+  if (it != g_ctx.plans.end()) {
+    *out = it->second.get();
+    return HGX_OK;
+  }
+  auto plan = std::make_unique<Plan>();
+  plan->n = n;
+  int rem = n;
+  while (rem > kRowMaxBits) {
+    plan->ncol++;
+    rem -= 8;
+  }
+  plan->srow = rem;
+  const u64 L = 1ull << n;
+  std::vector<u64> host;
+  struct Off { size_t TLf, THf, TLi, THi, THm, THt; };
+  std::vector<Off> offs(plan->ncol * 3);
+  for (int i = 0; i < plan->ncol; i++) {
+    const int b = n - 8 * i;
+    const int lb = b / 2, hb = b - lb;
+    plan->col[i].bbits = b;
+    plan->col[i].lb = lb;
+    for (int k = 0; k < 3; k++) {
+      const u64 p = kModuli[k];
+      const u64 w = powmod(g_ctx.gen[k], (p - 1) >> b, p);
+      const u64 wi = invmod(w, p);
+      const u64 wl = powmod(w, 1ull << lb, p), wil = powmod(wi, 1ull << lb, p);
+      const u64 Linv = invmod(L % p, p);
+      const u64 Rm = (u64)((((u128)1) << 64) % p);
+      const u64 kmul = mulmod(Rm, Linv, p);  // R / L: undoes the pointwise REDC and scales by 1/L
+      const u64 ktr = Linv;                   // 1 / L
+      Off &o = offs[i * 3 + k];
+      auto push_pow = [&](u64 base, size_t cnt, u64 scale) {
+        size_t at = host.size();
+        u64 x = scale % p;
+        for (size_t j = 0; j < cnt; j++) {
+          host.push_back(to_mont(x, p));
+          x = mulmod(x, base, p);
+        }
+        return at;
+      };
+      o.TLf = push_pow(w, 1ull << lb, 1);
+      o.THf = push_pow(wl, 1ull << hb, 1);
+      o.TLi = push_pow(wi, 1ull << lb, 1);
+      o.THi = push_pow(wil, 1ull << hb, 1);
+      o.THm = (i == 0) ? push_pow(wil, 1ull << hb, kmul) : 0;
+      o.THt = (i == 0) ? push_pow(wil, 1ull << hb, ktr) : 0;
+    }
+  }
+  if (!host.empty()) {
+    plan->bytes = host.size() * sizeof(u64);
+    int rc = dev_alloc((void **)&plan->dev, plan->bytes);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(plan->dev, host.data(), plan->bytes, hipMemcpyHostToDevice));
+    for (int i = 0; i < plan->ncol; i++)
+      for (int k = 0; k < 3; k++) {
+        const Off &o = offs[i * 3 + k];
+        plan->col[i].TLf[k] = plan->dev + o.TLf;
+        plan->col[i].THf[k] = plan->dev + o.THf;
+        plan->col[i].TLi[k] = plan->dev + o.TLi;
+        plan->col[i].THi[k] = plan->dev + o.THi;
+        plan->col[i].THi_mul[k] = plan->dev + o.THm;
+        plan->col[i].THi_tr[k] = plan->dev + o.THt;
+      }
+  }
+  *out = plan.get();
+  g_ctx.plans[n] = std::move(plan);
+  return HGX_OK;
+}
+
+ColTw coltw(const ColPassTables &c, int which) {
+  // which: 0 forward, 1 inverse, 2 inverse with multiply scale, 3 inverse with 1/L
+  ColTw t{};
+  t.lb = c.lb;
+  for (int k = 0; k < 3; k++) {
+    t.TL[k] = (which == 0) ? c.TLf[k] : c.TLi[k];
+    t.TH[k] = (which == 0) ? c.THf[k] : (which == 1 ? c.THi[k] : (which == 2 ? c.THi_mul[k] : c.THi_tr[k]));
+  }
+  return t;
+}
+
+// ---- row-kernel dispatch ----
+template <int MODE, bool SINGLE>
+int launch_row(int sbits, dim3 grid, hipStream_t s, const RowParams &rp, const JobRef &jr, int jt_bits) {
+  static const char *names[4] = {SINGLE ? "row_single_mul" : "row_mul", SINGLE ? "row_single_sqr" : "row_sqr",
+                                 "row_fwd", "row_inv"};
+  HGX_TIMED(names[MODE], s);
+  switch (sbits) {
+#define HGX_ROW_CASE(S) \
+  case S:               \
+    hipLaunchKernelGGL((k_row<S, MODE, SINGLE>), grid, dim3(HGX_NT), 0, s, g_ctx.ps, rp, jr, jt_bits); break;
+    HGX_ROW_CASE(1) HGX_ROW_CASE(2) HGX_ROW_CASE(3) HGX_ROW_CASE(4) HGX_ROW_CASE(5) HGX_ROW_CASE(6)
+    HGX_ROW_CASE(7) HGX_ROW_CASE(8) HGX_ROW_CASE(9) HGX_ROW_CASE(10) HGX_ROW_CASE(11) HGX_ROW_CASE(12)
+#undef HGX_ROW_CASE
+    default:
+      return fail(HGX_EINVAL, "row pass size out of range");
+  }
+  return HGX_OK;
+}
+
+int launch_row_inv_scaled(int sbits, dim3 grid, hipStream_t s, const RowParams &rp, const JobRef &jr) {
+  HGX_TIMED("row_inv_scaled", s);
+  switch (sbits) {
+#define HGX_ROW_CASE(S) \
+  case S:               \
+    hipLaunchKernelGGL((k_row_inv_scaled<S>), grid, dim3(HGX_NT), 0, s, g_ctx.ps, rp, jr); break;
+    HGX_ROW_CASE(1) HGX_ROW_CASE(2) HGX_ROW_CASE(3) HGX_ROW_CASE(4) HGX_ROW_CASE(5) HGX_ROW_CASE(6)
+    HGX_ROW_CASE(7) HGX_ROW_CASE(8) HGX_ROW_CASE(9) HGX_ROW_CASE(10) HGX_ROW_CASE(11) HGX_ROW_CASE(12)
+#undef HGX_ROW_CASE
+    default:
+      return fail(HGX_EINVAL, "row pass size out of range");
+  }
+  return HGX_OK;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int log2_ceil(u64 x) {
+  int n = 0;
+  while ((1ull << n) < x) n++;
+  return n;
+}
+
+// Work memory of one multiply of transform length 2^n.
+struct WorkLayout {
+  size_t res0, res1, ovf, flags, maps, agg, blkin, total;
+};
+WorkLayout work_layout(int n, bool sqr) {
+  const u64 L = 1ull << n;
+  const u64 rl = std::min<u64>(L, HGX_RUN);
+  const u64 nruns = L / rl;
+  const u64 nblk = (nruns + HGX_CARRY_RUNS_PER_BLOCK - 1) / HGX_CARRY_RUNS_PER_BLOCK;
+  WorkLayout w{};
+  size_t at = 0;
+  const bool single = n <= kRowMaxBits;
+  w.res0 = at;
+  at += single ? 0 : align_up(3 * L * sizeof(u64));
+  w.res1 = at;
+  at += (single || sqr) ? 0 : align_up(3 * L * sizeof(u64));
+  w.ovf = at;
+  at += align_up(2 * nruns * sizeof(u64));
+  w.flags = at;
+  at += align_up(nruns * sizeof(u32));
+  w.maps = at;
+  at += align_up(nruns);
+  w.agg = at;
+  at += align_up(nblk);
+  w.blkin = at;
+  at += align_up(nblk);
+  w.total = at;
+  return w;
+}
+
+MulJob make_job(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, int n, bool sqr, char *base) {
+  const WorkLayout wl = work_layout(n, sqr);
+  MulJob J{};
+  J.a = a;
+  J.na = na;
+  J.b = sqr ? a : b;
+  J.nb = sqr ? na : nb;
+  J.out = out;
+  J.nout = na + (sqr ? na : nb);
+  J.res0 = (u64 *)(base + wl.res0);
+  J.res1 = (u64 *)(base + wl.res1);
+  J.ovf = (u64 *)(base + wl.ovf);
+  J.flags = (u32 *)(base + wl.flags);
+  J.maps = (uint8_t *)(base + wl.maps);
+  J.agg = (uint8_t *)(base + wl.agg);
+  J.blkin = (uint8_t *)(base + wl.blkin);
+  return J;
+}
+
+// Enqueue NJOBS products of transform length 2^n (all squares or all
+// general products) on stream s.  JR holds the job by value (one job) or a
+// device table of NJOBS jobs.
+int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t s) {
+  if (n > kMaxBits) return fail(HGX_ETOOBIG, "transform length 2^" + std::to_string(n) + " above 2^32");
+  Plan *plan = nullptr;
+  int rc = build_plan(n, &plan);
+  if (rc) return rc;
+  const u64 L = 1ull << n;
+  const u64 rl = std::min<u64>(L, HGX_RUN);
+  const u64 nruns = L / rl;
+  const int nops = sqr ? 1 : 2;
+
+  if (n <= kRowMaxBits) {
+    RowParams rp{};
+    rp.L = L;
+    rp.tw_bits = n;
+    rp.kbase = 0;
+    for (int k = 0; k < 3; k++) {
+      const u64 p = kModuli[k];
+      // pointwise value a*b/R times this (Montgomery) factor = a*b/L
+      const u64 Rm = (u64)((((u128)1) << 64) % p);
+      rp.scale[k] = to_mont(mulmod(Rm, invmod(L % p, p), p), p);
+    }
+    rc = sqr ? launch_row<MODE_SQR, true>(n, dim3(njobs), s, rp, jr, 0)
+             : launch_row<MODE_MUL, true>(n, dim3(njobs), s, rp, jr, 0);
+    if (rc) return rc;
+  } else {
+    const int jt = n - 12;  // tiles per job
+    const unsigned grid = njobs << jt;
+    {
+      HGX_TIMED("fwd_col_first", s);
+      hipLaunchKernelGGL((k_fwd_col<true>), dim3(grid, 1, nops), dim3(HGX_NT), 0, s, g_ctx.ps,
+                         coltw(plan->col[0], 0), jr, jt, L, plan->col[0].bbits, 0);
+    }
+    for (int i = 1; i < plan->ncol; i++) {
+      HGX_TIMED("fwd_col", s);
+      hipLaunchKernelGGL((k_fwd_col<false>), dim3(grid, 3, nops), dim3(HGX_NT), 0, s, g_ctx.ps,
+                         coltw(plan->col[i], 0), jr, jt, L, plan->col[i].bbits, 0);
+    }
+    RowParams rp{};
+    rp.L = L;
+    rp.tw_bits = 12;
+    rp.kbase = 0;
+    rc = sqr ? launch_row<MODE_SQR, false>(plan->srow, dim3(grid, 3), s, rp, jr, jt)
+             : launch_row<MODE_MUL, false>(plan->srow, dim3(grid, 3), s, rp, jr, jt);
+    if (rc) return rc;
+    for (int i = plan->ncol - 1; i >= 1; i--) {
+      HGX_TIMED("inv_col", s);
+      hipLaunchKernelGGL((k_inv_col<false>), dim3(grid, 3), dim3(HGX_NT), 0, s, g_ctx.ps, coltw(plan->col[i], 1), jr,
+                         jt, L, plan->col[i].bbits, 0);
+    }
+    HGX_TIMED("inv_col_crt", s);
+    hipLaunchKernelGGL((k_inv_col<true>), dim3(grid, 1), dim3(HGX_NT), 0, s, g_ctx.ps, coltw(plan->col[0], 2), jr, jt,
+                       L, plan->col[0].bbits, 0);
+  }
+  const u64 nblk = (nruns + HGX_CARRY_RUNS_PER_BLOCK - 1) / HGX_CARRY_RUNS_PER_BLOCK;
+  const int bpj = log2_ceil(nblk);
+  {
+    HGX_TIMED("carry_map", s);
+    hipLaunchKernelGGL(k_carry_map, dim3(njobs << bpj), dim3(HGX_NT), 0, s, jr, bpj, nruns, (int)rl);
+  }
+  {
+    HGX_TIMED("carry_blocks", s);
+    hipLaunchKernelGGL(k_carry_blocks, dim3(njobs), dim3(1024), 0, s, jr, nblk);
+  }
+  {
+    HGX_TIMED("carry_apply", s);
+    hipLaunchKernelGGL(k_carry_apply, dim3(njobs << bpj), dim3(HGX_NT), 0, s, jr, bpj, nruns, (int)rl);
+  }
+  HIPCHK(hipGetLastError());
+  return HGX_OK;
+}
+
+// Forward or inverse transform of one residue array in place (device).
+int enqueue_transform(u64 *d, size_t nwords, int mod_num, bool inverse, hipStream_t s) {
+  const int n = log2_ceil(nwords);
+  if ((1ull << n) != nwords) return fail(HGX_EINVAL, "transform length must be a power of two");
+  if (n > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32");
+  if (n == 0) return HGX_OK;  // length 1: both transforms are the identity
+  Plan *plan = nullptr;
+  int rc = build_plan(n, &plan);
+  if (rc) return rc;
+  const u64 L = nwords;
+  const int jt = n >= 12 ? n - 12 : 0;
+  const unsigned tiles = 1u << jt;
+  JobRef jr{};
+  jr.one.res0 = d;
+  RowParams rp{};
+  rp.L = L;
+  rp.tw_bits = n >= 12 ? 12 : n;
+  rp.kbase = mod_num;
+  if (!inverse) {
+    for (int i = 0; i < plan->ncol; i++) {
+      HGX_TIMED("fwd_col", s);
+      hipLaunchKernelGGL((k_fwd_col<false>), dim3(tiles, 1, 1), dim3(HGX_NT), 0, s, g_ctx.ps, coltw(plan->col[i], 0),
+                         jr, jt, L, plan->col[i].bbits, mod_num);
+    }
+    rc = launch_row<MODE_FWD, false>(plan->srow, dim3(tiles, 1), s, rp, jr, jt);
+    if (rc) return rc;
+  } else {
+    if (plan->ncol == 0) {
+      const u64 p = kModuli[mod_num];
+      const u64 Rm = (u64)((((u128)1) << 64) % p);
+      rp.scale[mod_num] = to_mont(mulmod(Rm, invmod(L % p, p), p), p);  // times 1/L after REDC
+      rc = launch_row_inv_scaled(plan->srow, dim3(tiles), s, rp, jr);
+      if (rc) return rc;
+    } else {
+      rc = launch_row<MODE_INV, false>(plan->srow, dim3(tiles, 1), s, rp, jr, jt);
+      if (rc) return rc;
+      for (int i = plan->ncol - 1; i >= 0; i--) {
+        HGX_TIMED("inv_col", s);
+        hipLaunchKernelGGL((k_inv_col<false>), dim3(tiles, 1), dim3(HGX_NT), 0, s, g_ctx.ps,
+                           coltw(plan->col[i], i == 0 ? 3 : 1), jr, jt, L, plan->col[i].bbits, mod_num);
+      }
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return HGX_OK;
+}
+
+int check_ready() {
+  if (!g_ctx.ready) return fail(HGX_ENOTINIT, "hgx_init has not been called");
+  return HGX_OK;
+}
+
+// Order work on a caller stream after the previous use of the shared
+// workspace, and record this use.
+int ws_acquire(hipStream_t s) {
+  HIPCHK(hipStreamWaitEvent(s, g_ctx.ws_done, 0));
+  return HGX_OK;
+}
+int ws_release(hipStream_t s) {
+  HIPCHK(hipEventRecord(g_ctx.ws_done, s));
+  return HGX_OK;
+}
+
+int mul_device_locked(const u64 *d_a, size_t na, const u64 *d_b, size_t nb, u64 *d_out, bool sqr, hipStream_t s) {
+  const int n = log2_ceil((u64)hgx_product_length(na, nb));
+  if (n > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32");
+  const WorkLayout wl = work_layout(n, sqr);
+  int rc = ws_acquire(s);
+  if (rc) return rc;
+  rc = ensure(g_ctx.ws, wl.total);
+  if (rc) return rc;
+  JobRef jr{};
+  jr.one = make_job(d_a, na, d_b, nb, d_out, n, sqr, (char *)g_ctx.ws.ptr);
+  rc = enqueue_group(jr, 1, n, sqr, s);
+  if (rc) return rc;
+  return ws_release(s);
+}
+
+// Batched products of device operands: grouped by (transform length,
+// square), one launch sequence per group over a device job table.  Groups
+// whose work memory exceeds the budget are split.
+int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) {
+  struct Item { size_t idx; int n; bool sqr; size_t bytes; };
+  std::vector<Item> items;
+  items.reserve(count);
+  for (size_t i = 0; i < count; i++) {
+    const hgx_mul_desc &d = descs[i];
+    if (!d.a || !d.b || !d.out || d.na == 0 || d.nb == 0) return fail(HGX_EINVAL, "bad descriptor " + std::to_string(i));
+    const bool sqr = (d.a == d.b && d.na == d.nb);
+    const int n = log2_ceil((u64)hgx_product_length(d.na, d.nb));
+    if (n > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32 in descriptor " + std::to_string(i));
+    items.push_back({i, n, sqr, work_layout(n, sqr).total});
+  }
+  std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
+    return x.n != y.n ? x.n < y.n : x.sqr < y.sqr;
+  });
+  int rc = ws_acquire(s);
+  if (rc) return rc;
+  size_t pos = 0;
+  while (pos < items.size()) {
+    // a chunk: consecutive items that fit the budget together
+    const size_t budget_left = g_ctx.budget - (g_ctx.allocated - g_ctx.ws.bytes - g_ctx.jobs_dev.bytes);
+    size_t end = pos, bytes = 0;
+    while (end < items.size()) {
+      const size_t add = items[end].bytes;
+      const size_t tbl = align_up((end - pos + 1) * sizeof(MulJob));
+      if (end > pos && bytes + add + tbl + (2u << 20) > budget_left) break;
+      bytes += add;
+      end++;
+    }
+    rc = ensure(g_ctx.ws, bytes);
+    if (rc) return rc;
+    rc = ensure(g_ctx.jobs_dev, align_up((end - pos) * sizeof(MulJob)));
+    if (rc) return rc;
+    // fill the job table (pinned) and copy it to the device
+    HIPCHK(hipEventSynchronize(g_ctx.jobs_copied));
+    if (g_ctx.jobs_host_cap < end - pos) {
+      if (g_ctx.jobs_host) (void)hipHostFree(g_ctx.jobs_host);
+      g_ctx.jobs_host_cap = std::max<size_t>(end - pos, 1024);
+      HIPCHK(hipHostMalloc((void **)&g_ctx.jobs_host, g_ctx.jobs_host_cap * sizeof(MulJob), hipHostMallocDefault));
+    }
+    char *base = (char *)g_ctx.ws.ptr;
+    for (size_t i = pos; i < end; i++) {
+      const hgx_mul_desc &d = descs[items[i].idx];
+      g_ctx.jobs_host[i - pos] = make_job(d.a, d.na, d.b, d.nb, d.out, items[i].n, items[i].sqr, base);
+      base += items[i].bytes;
+    }
+    MulJob *table = (MulJob *)g_ctx.jobs_dev.ptr;
+    HIPCHK(hipMemcpyAsync(table, g_ctx.jobs_host, (end - pos) * sizeof(MulJob), hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(g_ctx.jobs_copied, s));
+    size_t g = pos;
+    while (g < end) {
+      size_t h = g;
+      while (h < end && items[h].n == items[g].n && items[h].sqr == items[g].sqr) h++;
+      JobRef jr{};
+      jr.table = table + (g - pos);
+      rc = enqueue_group(jr, (unsigned)(h - g), items[g].n, items[g].sqr, s);
+      if (rc) return rc;
+      g = h;
+    }
+    pos = end;
+    if (pos < items.size()) {
+      // the next chunk reuses the work memory
+      HIPCHK(hipStreamSynchronize(s));
+    }
+  }
+  return ws_release(s);
+}
+
+// ---- request server (continuous batching) ----
+struct Request {
+  hgx_mul_desc d;
+  std::atomic<int> state{0};  // 0 pending, 1 done, <0 error
+  std::string err;
+};
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *hgx_last_error(void) { return g_err.c_str(); }
+
+int hgx_init(int device, size_t hbm_budget_bytes) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  if (g_ctx.ready) return HGX_OK;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(HGX_EINVAL, "no such device");
+  HIPCHK(hipSetDevice(device));
+  g_ctx.device = device;
+  size_t freeb = 0, totalb = 0;
+  HIPCHK(hipMemGetInfo(&freeb, &totalb));
+  g_ctx.budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(0.9 * (double)freeb);
+  HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&g_ctx.ws_done, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(g_ctx.ws_done, g_ctx.stream));
+  HIPCHK(hipEventCreateWithFlags(&g_ctx.jobs_copied, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(g_ctx.jobs_copied, g_ctx.stream));
+  // prime set: moduli, Proth factors, generators, in-tile root tables
+  std::vector<u64> host(3 * 2 * 2048);
+  PrimeSet ps{};
+  for (int k = 0; k < 3; k++) {
+    const u64 p = kModuli[k];
+    ps.p[k] = p;
+    ps.c[k] = (u32)((p - 1) >> 55);
+    g_ctx.gen[k] = find_generator(p, ps.c[k]);
+    const u64 w = powmod(g_ctx.gen[k], (p - 1) >> 12, p);
+    const u64 wi = invmod(w, p);
+    u64 x = 1, y = 1;
+    for (int i = 0; i < 2048; i++) {
+      host[(k * 2 + 0) * 2048 + i] = to_mont(x, p);
+      host[(k * 2 + 1) * 2048 + i] = to_mont(y, p);
+      x = mulmod(x, w, p);
+      y = mulmod(y, wi, p);
+    }
+  }
+  ps.g01 = to_mont(invmod(kModuli[0] % kModuli[1], kModuli[1]), kModuli[1]);
+  ps.g02 = to_mont(invmod(kModuli[0] % kModuli[2], kModuli[2]), kModuli[2]);
+  ps.g12 = to_mont(invmod(kModuli[1] % kModuli[2], kModuli[2]), kModuli[2]);
+  int rc = dev_alloc((void **)&g_ctx.tables, host.size() * sizeof(u64));
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(g_ctx.tables, host.data(), host.size() * sizeof(u64), hipMemcpyHostToDevice));
+  for (int k = 0; k < 3; k++) {
+    ps.tw_fwd[k] = g_ctx.tables + (k * 2 + 0) * 2048;
+    ps.tw_inv[k] = g_ctx.tables + (k * 2 + 1) * 2048;
+  }
+  g_ctx.ps = ps;
+  g_ctx.ready = true;
+  return HGX_OK;
+}
+
+void hgx_shutdown(void) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  if (!g_ctx.ready) return;
+  hipStreamSynchronize(g_ctx.stream);
+  for (auto &kv : g_ctx.plans) dev_free(kv.second->dev, kv.second->bytes);
+  g_ctx.plans.clear();
+  dev_free(g_ctx.ws.ptr, g_ctx.ws.bytes);
+  g_ctx.ws = Workspace{};
+  dev_free(g_ctx.stage.ptr, g_ctx.stage.bytes);
+  g_ctx.stage = Workspace{};
+  dev_free(g_ctx.jobs_dev.ptr, g_ctx.jobs_dev.bytes);
+  g_ctx.jobs_dev = Workspace{};
+  dev_free(g_ctx.pack.ptr, g_ctx.pack.bytes);
+  g_ctx.pack = Workspace{};
+  if (g_ctx.jobs_host) (void)hipHostFree(g_ctx.jobs_host);
+  g_ctx.jobs_host = nullptr;
+  g_ctx.jobs_host_cap = 0;
+  (void)hipEventDestroy(g_ctx.jobs_copied);
+  dev_free(g_ctx.tables, 3 * 2 * 2048 * sizeof(u64));
+  g_ctx.tables = nullptr;
+  hipEventDestroy(g_ctx.ws_done);
+  hipStreamDestroy(g_ctx.stream);
+  g_ctx.ready = false;
+}
+
+int hgx_num_moduli(void) { return 3; }
+uint64_t hgx_modulus(int k) { return (k >= 0 && k < 3) ? kModuli[k] : 0; }
+int hgx_max_transform_bits(void) { return kMaxBits; }
+
+size_t hgx_product_length(size_t na, size_t nb) {
+  // least-power-of-two->=, src/math-utilities.lisp:24-28
+  size_t n = na + nb;
+  size_t L = 1;
+  while (L < n) L <<= 1;
+  return L;
+}
+
+int hgx_mul_device(const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, uint64_t *d_out, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!d_a || !d_b || !d_out || na == 0 || nb == 0) return fail(HGX_EINVAL, "null operand or zero length");
+  hipStream_t s = stream ? (hipStream_t)stream : g_ctx.stream;
+  const bool sqr = (d_a == d_b && na == nb);
+  return mul_device_locked(d_a, na, d_b, nb, d_out, sqr, s);
+}
+
+int hgx_sqr_device(const uint64_t *d_a, size_t na, uint64_t *d_out, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!d_a || !d_out || na == 0) return fail(HGX_EINVAL, "null operand or zero length");
+  hipStream_t s = stream ? (hipStream_t)stream : g_ctx.stream;
+  return mul_device_locked(d_a, na, d_a, na, d_out, true, s);
+}
+
+// host-memory product: stage operands into device memory, multiply, copy back
+static int mul_host_locked(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out, bool sqr) {
+  hipStream_t s = g_ctx.stream;
+  const size_t nout = na + nb;
+  const size_t bytes_a = align_up(na * 8), bytes_b = sqr ? 0 : align_up(nb * 8), bytes_o = align_up(nout * 8);
+  int rc = ws_acquire(s);
+  if (rc) return rc;
+  rc = ensure(g_ctx.stage, bytes_a + bytes_b + bytes_o);
+  if (rc) return rc;
+  char *base = (char *)g_ctx.stage.ptr;
+  u64 *d_a = (u64 *)base, *d_b = sqr ? d_a : (u64 *)(base + bytes_a), *d_o = (u64 *)(base + bytes_a + bytes_b);
+  HIPCHK(hipMemcpyAsync(d_a, a, na * 8, hipMemcpyHostToDevice, s));
+  if (!sqr) HIPCHK(hipMemcpyAsync(d_b, b, nb * 8, hipMemcpyHostToDevice, s));
+  rc = mul_device_locked(d_a, na, d_b, sqr ? na : nb, d_o, sqr, s);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(out, d_o, nout * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return HGX_OK;
+}
+
+int hgx_mul(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!a || !b || !out || na == 0 || nb == 0) return fail(HGX_EINVAL, "null operand or zero length");
+  return mul_host_locked(a, na, b, nb, out, a == b && na == nb);
+}
+
+int hgx_sqr(const uint64_t *a, size_t na, uint64_t *out) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!a || !out || na == 0) return fail(HGX_EINVAL, "null operand or zero length");
+  return mul_host_locked(a, na, a, na, out, true);
+}
+
+int hgx_mul_batch(const hgx_mul_desc *descs, size_t count) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (count == 0) return HGX_OK;
+  if (!descs) return fail(HGX_EINVAL, "null descriptor array");
+  // pack every operand into one device buffer, run the device batch, unpack
+  size_t in_words = 0, out_words = 0;
+  for (size_t i = 0; i < count; i++) {
+    const hgx_mul_desc &d = descs[i];
+    if (!d.a || !d.b || !d.out || d.na == 0 || d.nb == 0) return fail(HGX_EINVAL, "bad descriptor " + std::to_string(i));
+    const bool sqr = (d.a == d.b && d.na == d.nb);
+    in_words += d.na + (sqr ? 0 : d.nb);
+    out_words += d.na + d.nb;
+  }
+  hipStream_t s = g_ctx.stream;
+  rc = ws_acquire(s);
+  if (rc) return rc;
+  rc = ensure(g_ctx.pack, align_up((in_words + out_words) * 8));
+  if (rc) return rc;
+  std::vector<u64> hin(in_words);
+  std::vector<hgx_mul_desc> dd(count);
+  u64 *din = (u64 *)g_ctx.pack.ptr, *dout = din + in_words;
+  size_t at = 0, ot = 0;
+  for (size_t i = 0; i < count; i++) {
+    const hgx_mul_desc &d = descs[i];
+    const bool sqr = (d.a == d.b && d.na == d.nb);
+    std::memcpy(&hin[at], d.a, d.na * 8);
+    dd[i].a = din + at;
+    dd[i].na = d.na;
+    at += d.na;
+    if (sqr) {
+      dd[i].b = dd[i].a;
+    } else {
+      std::memcpy(&hin[at], d.b, d.nb * 8);
+      dd[i].b = din + at;
+      at += d.nb;
+    }
+    dd[i].nb = d.nb;
+    dd[i].out = dout + ot;
+    ot += d.na + d.nb;
+  }
+  HIPCHK(hipMemcpyAsync(din, hin.data(), in_words * 8, hipMemcpyHostToDevice, s));
+  rc = batch_device_locked(dd.data(), count, s);
+  if (rc) return rc;
+  std::vector<u64> hout(out_words);
+  HIPCHK(hipMemcpyAsync(hout.data(), dout, out_words * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  ot = 0;
+  for (size_t i = 0; i < count; i++) {
+    std::memcpy(descs[i].out, &hout[ot], (descs[i].na + descs[i].nb) * 8);
+    ot += descs[i].na + descs[i].nb;
+  }
+  return HGX_OK;
+}
+
+int hgx_mul_batch_device(const hgx_mul_desc *descs, size_t count, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (count == 0) return HGX_OK;
+  if (!descs) return fail(HGX_EINVAL, "null descriptor array");
+  hipStream_t s = stream ? (hipStream_t)stream : g_ctx.stream;
+  return batch_device_locked(descs, count, s);
+}
+
+int hgx_ntt_forward(uint64_t *data, size_t n, int mod_num) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!data || n == 0 || mod_num < 0 || mod_num > 2) return fail(HGX_EINVAL, "bad transform arguments");
+  hipStream_t s = g_ctx.stream;
+  rc = ws_acquire(s);
+  if (rc) return rc;
+  rc = ensure(g_ctx.stage, align_up(n * 8));
+  if (rc) return rc;
+  u64 *d = (u64 *)g_ctx.stage.ptr;
+  HIPCHK(hipMemcpyAsync(d, data, n * 8, hipMemcpyHostToDevice, s));
+  rc = enqueue_transform(d, n, mod_num, false, s);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(data, d, n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return ws_release(s);
+}
+
+int hgx_ntt_reverse(uint64_t *data, size_t n, int mod_num) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!data || n == 0 || mod_num < 0 || mod_num > 2) return fail(HGX_EINVAL, "bad transform arguments");
+  hipStream_t s = g_ctx.stream;
+  rc = ws_acquire(s);
+  if (rc) return rc;
+  rc = ensure(g_ctx.stage, align_up(n * 8));
+  if (rc) return rc;
+  u64 *d = (u64 *)g_ctx.stage.ptr;
+  HIPCHK(hipMemcpyAsync(d, data, n * 8, hipMemcpyHostToDevice, s));
+  rc = enqueue_transform(d, n, mod_num, true, s);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(data, d, n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return ws_release(s);
+}
+
+int hgx_fill_splitmix64(uint64_t *d_out, size_t n, uint64_t seed, int force_top, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!d_out) return fail(HGX_EINVAL, "null output");
+  if (n == 0) return HGX_OK;
+  hipStream_t s = stream ? (hipStream_t)stream : g_ctx.stream;
+  hipLaunchKernelGGL(k_splitmix64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_out, (u64)n, seed, force_top);
+  HIPCHK(hipGetLastError());
+  return HGX_OK;
+}
+
+// Request server: submitted products are executed in arrival order by a
+// worker thread (replaced by the continuous-batching server).
+static std::mutex g_req_mu;
+static std::unordered_map<hgx_ticket, std::shared_ptr<Request>> g_reqs;
+static std::atomic<hgx_ticket> g_next_ticket{1};
+
+int hgx_submit(const hgx_mul_desc *desc, hgx_ticket *ticket) {
+  if (!desc || !ticket) return fail(HGX_EINVAL, "null descriptor or ticket");
+  auto r = std::make_shared<Request>();
+  r->d = *desc;
+  const hgx_ticket t = g_next_ticket++;
+  {
+    std::lock_guard<std::mutex> lk(g_req_mu);
+    g_reqs[t] = r;
+  }
+  std::thread([r]() {
+    int rc = hgx_mul(r->d.a, r->d.na, r->d.b, r->d.nb, r->d.out);
+    if (rc) r->err = g_err;
+    r->state = rc ? rc : 1;
+  }).detach();
+  *ticket = t;
+  return HGX_OK;
+}
+
+int hgx_poll(hgx_ticket ticket) {
+  std::shared_ptr<Request> r;
+  {
+    std::lock_guard<std::mutex> lk(g_req_mu);
+    auto it = g_reqs.find(ticket);
+    if (it == g_reqs.end()) return fail(HGX_EINVAL, "unknown ticket");
+    r = it->second;
+  }
+  int st = r->state.load();
+  if (st < 0) g_err = r->err;
+  return st;
+}
+
+int hgx_wait(hgx_ticket ticket) {
+  for (;;) {
+    int st = hgx_poll(ticket);
+    if (st != 0) {
+      std::lock_guard<std::mutex> lk(g_req_mu);
+      g_reqs.erase(ticket);
+      return st == 1 ? HGX_OK : st;
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+int hgx_timing_enable(int on) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  timing_drain();
+  g_kstats.clear();
+  g_timing = on != 0;
+  return HGX_OK;
+}
+
+int hgx_timing_read(hgx_kernel_time *out, int max_entries) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  timing_drain();
+  int n = 0;
+  for (auto &kv : g_kstats) {
+    if (n >= max_entries || !out) break;
+    std::memset(out[n].name, 0, sizeof out[n].name);
+    std::strncpy(out[n].name, kv.first.c_str(), sizeof out[n].name - 1);
+    out[n].launches = kv.second.first;
+    out[n].total_ms = kv.second.second;
+    n++;
+  }
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+//
+// hgx.h -- C ABI of the MI355X exact NTT big-integer multiply.
+//
+// This is the drop-in boundary for Hypergeometrica's NTT multiply.  The
+// reference (Common Lisp) dispatches every product of two operands of more
+// than *ntt-multiply-threshold* limbs through the generic MPZ-* method
+// (src/multiply.lisp:12-49) to MPZ-SQUARE for (eq x y) and to MPZ-*/NTT
+// otherwise.  An sb-alien shim redefines those two functions to call
+// hgx_sqr / hgx_mul; no caller changes.
+//
+// Data layout is the reference's storage layout: little-endian uint64 limbs
+// in caller-owned memory (src/vec.lisp:17-65, src/ram-vec.lisp:52-59).
+// Every entry point returns HGX_OK (0) or a negative HGX_E* code; the
+// message of the last failure on the calling thread is hgx_last_error().
+// All entry points are thread-safe.
+#ifndef HGX_H
+#define HGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  HGX_OK = 0,
+  HGX_EINVAL = -1,  // bad argument (null pointer, zero length, bad index)
+  HGX_ETOOBIG = -2,  // transform length above the supported maximum
+  HGX_ENOMEM = -3,  // device memory or the HBM budget exhausted
+  HGX_EDEVICE = -4,  // HIP runtime or kernel failure
+  HGX_ENOTINIT = -5  // hgx_init has not succeeded
+};
+
+// One multiplication: OUT receives NA + NB limbs of A * B.  A request with
+// A == B and NA == NB is computed as a square (as src/multiply.lisp:43-44
+// routes (eq x y)).
+typedef struct {
+  const uint64_t *a;
+  size_t na;
+  const uint64_t *b;
+  size_t nb;
+  uint64_t *out;
+} hgx_mul_desc;
+
+typedef uint64_t hgx_ticket;
+
+// Library set-up: select DEVICE, build the resident prime / root tables
+// (the scheme of src/moduli.lisp:298-353, src/ntt-multiply.lisp:8) and cap
+// device work memory at HBM_BUDGET_BYTES (0 = 90% of free memory).
+// Replaces the load-time construction of **scheme**.
+int hgx_init(int device, size_t hbm_budget_bytes);
+void hgx_shutdown(void);
+const char *hgx_last_error(void);
+
+// Scheme introspection: the moduli of default-moduli, src/moduli.lisp:257-264
+// (descending), and the transform length chosen for a product,
+// least-power-of-two->= (na + nb), src/ntt-multiply.lisp:176.
+int hgx_num_moduli(void);
+uint64_t hgx_modulus(int k);
+size_t hgx_product_length(size_t na, size_t nb);
+int hgx_max_transform_bits(void);
+
+// Exact product of two non-negative integers in host memory.
+// Replaces MPZ-*/NTT, src/ntt-multiply.lisp:174-254 (magnitude part; the
+// sign is sign(x)*sign(y), applied by the caller as at :254).  Writes exactly
+// NA + NB limbs to OUT (the reference's L-limb result trimmed by
+// optimize-mpz, src/multiply.lisp:19, holds the same value).
+int hgx_mul(const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
+            uint64_t *out);
+
+// Exact square.  Replaces MPZ-SQUARE, src/ntt-multiply.lisp:101-172.
+// Writes exactly 2 * NA limbs.
+int hgx_sqr(const uint64_t *a, size_t na, uint64_t *out);
+
+// Many independent products in one call, grouped on the device by
+// transform length (the uneven multiplies of one binary-splitting level,
+// COMBINE at src/binary-splitting.lisp:112-121).  Equivalent to COUNT calls
+// of hgx_mul / hgx_sqr with identical output.
+int hgx_mul_batch(const hgx_mul_desc *descs, size_t count);
+
+// Same products, with every pointer in device memory; runs on STREAM
+// (a hipStream_t, or NULL for the library's stream) and returns once the
+// work is queued.  Inputs resident in HBM; used by benchmarks and by
+// device-resident callers.
+int hgx_mul_device(const uint64_t *d_a, size_t na, const uint64_t *d_b,
+                   size_t nb, uint64_t *d_out, void *stream);
+int hgx_sqr_device(const uint64_t *d_a, size_t na, uint64_t *d_out,
+                   void *stream);
+int hgx_mul_batch_device(const hgx_mul_desc *descs, size_t count,
+                         void *stream);
+
+// Forward transform of N canonical residues (each < modulus MOD_NUM) in
+// place, output in bit-reversed order.  Same result as NTT-FORWARD,
+// src/number-theoretic-transform.lisp:40-90, with **scheme** and MOD_NUM.
+int hgx_ntt_forward(uint64_t *data, size_t n, int mod_num);
+
+// Inverse transform (input in bit-reversed order, output natural order,
+// scaled by 1/N).  Same result as NTT-REVERSE,
+// src/number-theoretic-transform.lisp:93-143.
+int hgx_ntt_reverse(uint64_t *data, size_t n, int mod_num);
+
+// Continuous-batching request server (no reference counterpart: the
+// reference calls MPZ-* synchronously).  hgx_submit queues one product and
+// returns at once; requests that arrive while others run are admitted as
+// device memory (the HBM budget) frees up, and their work overlaps on the
+// device.  Host buffers must stay valid until the ticket completes.
+// hgx_poll returns 1 when done, 0 while pending, <0 on failure;
+// hgx_wait blocks and returns HGX_OK or the request's error.
+int hgx_submit(const hgx_mul_desc *desc, hgx_ticket *ticket);
+int hgx_poll(hgx_ticket ticket);
+int hgx_wait(hgx_ticket ticket);
+
+// Device SplitMix64 fill for benchmarks and tests:
+// d_out[i] = mix(seed + (i + 1) * 0x9E3779B97F4A7C15); when FORCE_TOP is
+// non-zero the last limb is forced non-zero.
+int hgx_fill_splitmix64(uint64_t *d_out, size_t n, uint64_t seed,
+                        int force_top, void *stream);
+
+// Per-kernel device timing of later calls, for benchmarks: HIP events are
+// recorded around every kernel on the stream it is launched on.  Plays the
+// role of the reference's *verbose* phase timers ("Forward...", "Reverse...",
+// "CRT...", src/ntt-multiply.lisp:185-252).  hgx_timing_enable(1) clears the
+// totals and starts recording; hgx_timing_read waits for the recorded work
+// and returns the number of kernel kinds written to OUT.
+typedef struct {
+  char name[32];
+  uint64_t launches;
+  double total_ms;
+} hgx_kernel_time;
+
+int hgx_timing_enable(int on);
+int hgx_timing_read(hgx_kernel_time *out, int max_entries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
+

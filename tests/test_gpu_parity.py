@@ -1,0 +1,167 @@
+"""GPU parity: the MI355X library (through the C ABI) against the oracle,
+the golden fixtures, GMP and exact integers.  Integer work: bit-exact."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from tests import _gmp
+from tests._checksum import Q, mod_q
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def residues(oracle, k, n, seed):
+    m = oracle.moduli()[k]
+    x = oracle.splitmix64(seed, n)
+    return np.array([int(v) % m for v in x], dtype=np.uint64) if n < 4096 else (x % np.uint64(m))
+
+
+# ---------------------------------------------------------------- transforms
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 4, 5, 8, 11, 12, 13, 16, 17, 20, 21])
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_ntt_forward_reverse_match_oracle(hgx, oracle, k, nb):
+    say(f"ntt k={k} n=2^{nb}")
+    x = residues(oracle, k, 1 << nb, 1000 * k + nb)
+    want = oracle.ntt_forward(x, k)
+    got = hgx.ntt_forward(x, k)
+    assert np.array_equal(got, want)
+    back = hgx.ntt_reverse(got, k)
+    assert np.array_equal(back, oracle.ntt_reverse(want, k))
+    assert np.array_equal(back, x)
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+@pytest.mark.parametrize("nb", [6, 8, 10])
+def test_ntt_golden_vectors(hgx, k, nb):
+    v = np.load(os.path.join(GOLD, "ntt_vectors.npz"))
+    assert np.array_equal(hgx.ntt_forward(v[f"in_{k}_{nb}"], k), v[f"fwd_{k}_{nb}"])
+    assert np.array_equal(hgx.ntt_reverse(v[f"fwd_{k}_{nb}"], k), v[f"in_{k}_{nb}"])
+
+
+def test_ntt_rejects_bad_arguments(hgx):
+    from hypergeometrica_amd import HgxError
+    with pytest.raises(HgxError):
+        hgx.ntt_forward(np.zeros(6, dtype=np.uint64), 0)
+    with pytest.raises(HgxError):
+        hgx.ntt_forward(np.zeros(8, dtype=np.uint64), 3)
+
+
+# ---------------------------------------------------------------- products
+
+SIZES = [(1, 1), (1, 2), (3, 5), (100, 101), (101, 101), (777, 3), (2048, 2048), (2048, 2049), (1000, 7000),
+         (4096, 4096), (8192, 1), (65536, 65536), (65541, 32768), (100000, 31072)]
+
+
+@pytest.mark.parametrize("na,nb", SIZES)
+def test_mul_matches_oracle(hgx, oracle, na, nb):
+    say(f"mul {na}x{nb}")
+    a = oracle.splitmix64(0x48475800 + na, na)
+    b = oracle.splitmix64(0x48475900 + nb, nb)
+    got = hgx.mul_limbs(a, b)
+    want = oracle.mpz_mul_ntt(a, b)
+    assert np.array_equal(got, want[: na + nb])
+
+
+@pytest.mark.parametrize("n", [1, 2, 50, 101, 2048, 2049, 30000, 65536])
+def test_sqr_matches_oracle(hgx, oracle, n):
+    say(f"sqr {n}")
+    a = oracle.splitmix64(0x53515200 + n, n)
+    got = hgx.sqr_limbs(a)
+    want = oracle.mpz_square(a)
+    assert np.array_equal(got, want[: 2 * n])
+
+
+def test_golden_products(hgx):
+    d = np.load(os.path.join(GOLD, "mul_vectors.npz"))
+    n = len([k for k in d.files if k.startswith("p_")])
+    for i in range(n):
+        a, b, p = d[f"a_{i}"], d[f"b_{i}"], d[f"p_{i}"]
+        assert np.array_equal(hgx.mul_limbs(a, b), p), i
+        if np.array_equal(a, b):
+            assert np.array_equal(hgx.sqr_limbs(a), p), i
+
+
+@pytest.mark.parametrize("n", [1, 7, 16, 17, 2048, 2049, 65536, 300000, 1 << 20])
+def test_all_ones_carry_chains(hgx, oracle, n):
+    # (2^(64n) - 1)^2 has ~n all-ones limbs above a zero run: the longest carry chains
+    say(f"all-ones {n}")
+    a = np.full(n, 2**64 - 1, dtype=np.uint64)
+    got = hgx.sqr_limbs(a)
+    want = np.zeros(2 * n, dtype=np.uint64)
+    want[0] = 1
+    want[n] = 2**64 - 2
+    want[n + 1:] = 2**64 - 1
+    assert np.array_equal(got, want)
+    got2 = hgx.mul_limbs(a, a.copy())
+    assert np.array_equal(got2, want)
+
+
+@pytest.mark.parametrize("na,nb", [(3, 5), (5000, 3), (70000, 70000)])
+def test_all_ones_times_mixed(hgx, oracle, na, nb):
+    a = np.full(na, 2**64 - 1, dtype=np.uint64)
+    b = oracle.splitmix64(na + nb, nb)
+    got = hgx.mul_limbs(a, b)
+    if na + nb <= 200000:
+        assert np.array_equal(got, oracle.mpz_mul_ntt(a, b)[: na + nb])
+
+
+@pytest.mark.skipif(not _gmp.available(), reason="libgmp.so.10 not present")
+@pytest.mark.parametrize("na,nb", [(1 << 18, 1 << 18), ((1 << 20) + 3, 12345), (1 << 21, 1 << 20)])
+def test_large_products_match_gmp(hgx, oracle, na, nb):
+    say(f"gmp {na}x{nb}")
+    a = oracle.splitmix64(na, na)
+    b = oracle.splitmix64(nb + 1, nb)
+    assert np.array_equal(hgx.mul_limbs(a, b), _gmp.mul(a, b))
+
+
+def test_headline_size_checksum(hgx, oracle):
+    # 2^26 x 2^26 limbs, L = 2^27: product residue mod 2^61-1 equals the
+    # product of the operands' residues; top and bottom limbs exact.
+    n = 1 << 26
+    say("checksum 2^26 x 2^26")
+    a = oracle.splitmix64(0x48475801, n)
+    b = oracle.splitmix64(0x48475802, n)
+    got = hgx.mul_limbs(a, b)
+    assert mod_q(got) == (mod_q(a) * mod_q(b)) % Q
+    # lowest limbs: a*b mod 2^(64*4) from the low limbs alone
+    lo = (oracle.limbs_to_int(a[:4]) * oracle.limbs_to_int(b[:4])) % (1 << 256)
+    assert oracle.limbs_to_int(got[:4]) == lo
+    assert got[-1] != 0 or got[-2] != 0
+
+
+# ---------------------------------------------------------------- batch / serving
+
+def test_batch_equals_per_call(hgx, oracle):
+    rng = np.random.default_rng(0x48475803)
+    pairs = []
+    for i in range(40):
+        na = int(2 ** rng.uniform(0, 14))
+        nb = int(2 ** rng.uniform(0, 14))
+        a = oracle.splitmix64(3 * i + 1, max(na, 1))
+        b = a if i % 7 == 0 else oracle.splitmix64(3 * i + 2, max(nb, 1))
+        pairs.append((a, b))
+    outs = hgx.mul_batch(pairs)
+    for (a, b), o in zip(pairs, outs):
+        assert np.array_equal(o, hgx.mul_limbs(a, b))
+        assert oracle.limbs_to_int(o) == oracle.limbs_to_int(a) * oracle.limbs_to_int(b)
+
+
+def test_mpz_mirror(hgx):
+    import random
+    rnd = random.Random(4)
+    for _ in range(10):
+        a = rnd.randrange(-10**300, 10**300)
+        b = rnd.randrange(-10**250, 10**250)
+        za, zb = hgx.integer_mpz(a), hgx.integer_mpz(b)
+        assert hgx.mpz_integer(hgx.mpz_mul(za, zb)) == a * b
+        assert hgx.mpz_integer(hgx.mpz_mul(za, za)) == a * a
+        assert hgx.mpz_integer(hgx.mpz_square(za)) == a * a
+    assert hgx.mpz_integer(hgx.mpz_expt(hgx.integer_mpz(10), 9001)) == 10**9001
