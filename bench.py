@@ -74,6 +74,68 @@ def cpu_baseline(sample_bits: int = 20) -> dict:
     }
 
 
+def log_uniform_sizes(seed: int, count: int, lo_bits: float, hi_bits: float) -> list[int]:
+    """n = floor(2^(lo + (hi - lo) u)), u = SplitMix64 output / 2^64."""
+    from oracle.oracle import splitmix64_py
+    us = splitmix64_py(seed, count)
+    return [max(1, int(2 ** (lo_bits + (hi_bits - lo_bits) * (u / 2**64)))) for u in us]
+
+
+def batched_leg(torch, h, _lib, count: int = 4096, reps: int = 3) -> dict:
+    """Binary-splitting level: COUNT independent products, operand sizes
+    log-uniform in [2^8, 2^16] limbs (seed 0x48475803), operands in HBM.
+    Batched (hgx_mul_batch_device, grouped by transform length) against the
+    per-call path (hgx_mul_device per product); outputs compared."""
+    import ctypes
+    sizes = log_uniform_sizes(0x48475803, 2 * count, 8, 16)
+    na, nb = sizes[0::2], sizes[1::2]
+    tot_in = sum(na) + sum(nb)
+    tot_out = tot_in
+    dev = torch.device("cuda:0")
+    IN = torch.empty(tot_in, dtype=torch.int64, device=dev)
+    OUT1 = torch.zeros(tot_out, dtype=torch.int64, device=dev)
+    OUT2 = torch.zeros(tot_out, dtype=torch.int64, device=dev)
+    descs1 = (_lib.MulDesc * count)()
+    descs2 = (_lib.MulDesc * count)()
+    base, ob = IN.data_ptr(), 0
+    at = 0
+    for i in range(count):
+        a = base + 8 * at
+        h.fill_splitmix64(a, na[i], 0x48475803 + 2 * i + 1)
+        at += na[i]
+        b = base + 8 * at
+        h.fill_splitmix64(b, nb[i], 0x48475803 + 2 * i + 2)
+        at += nb[i]
+        descs1[i] = _lib.MulDesc(a, na[i], b, nb[i], OUT1.data_ptr() + 8 * ob)
+        descs2[i] = _lib.MulDesc(a, na[i], b, nb[i], OUT2.data_ptr() + 8 * ob)
+        ob += na[i] + nb[i]
+    L_ = _lib.lib()
+    # warm up both paths (plans, work memory)
+    _lib.check(L_.hgx_mul_batch_device(descs1, count, None))
+    for i in range(count):
+        _lib.check(L_.hgx_mul_device(descs2[i].a, descs2[i].na, descs2[i].b, descs2[i].nb, descs2[i].out, None))
+    torch.cuda.synchronize()
+    tb = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _lib.check(L_.hgx_mul_batch_device(descs1, count, None))
+        torch.cuda.synchronize()
+        tb.append(time.perf_counter() - t0)
+    tp = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for i in range(count):
+            _lib.check(L_.hgx_mul_device(descs2[i].a, descs2[i].na, descs2[i].b, descs2[i].nb, descs2[i].out, None))
+        torch.cuda.synchronize()
+        tp.append(time.perf_counter() - t0)
+    same = bool(torch.equal(OUT1, OUT2))
+    b, p = min(tb), min(tp)
+    return {"workload": f"{count} products, operand limbs log-uniform in [2^8, 2^16], seed 0x48475803",
+            "limbs": tot_out, "batched_s": round(b, 5), "per_call_s": round(p, 5),
+            "batched_limbs_per_s": tot_out / b, "per_call_limbs_per_s": tot_out / p,
+            "speedup": round(p / b, 2), "outputs_identical": same}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +144,7 @@ def main() -> int:
     ap.add_argument("--bits", type=int, default=26, help="operand size 2^bits limbs each")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-bits", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); products are independent per GPU "
@@ -165,6 +228,13 @@ def main() -> int:
     p50 = per_step_sorted[len(per_step_sorted) // 2]
     p99 = per_step_sorted[min(len(per_step_sorted) - 1, int(0.99 * len(per_step_sorted)))]
 
+    del A, B, OUT
+    torch.cuda.empty_cache()
+    batched = None
+    if not args.no_batch:
+        print("# batched vs per-call ...", file=sys.stderr, flush=True)
+        batched = batched_leg(torch, h, _lib)
+
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
@@ -190,6 +260,7 @@ def main() -> int:
         "multiply_roofline": {"alg_bytes": b_mul, "frac": round(b_mul / (ms_per_step * 1e-3) / HBM_PEAK, 4)},
         "latency_ms": {"p50": round(1e3 * p50, 3), "p99": round(1e3 * p99, 3)},
         "kernels": kernels,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -33,69 +33,15 @@
 //   k_carry_*         resolves the run-to-run carries (block scan)
 //   For L <= 4096 one k_row<..., SINGLE> does the whole transform work.
 #pragma once
-#include "modarith.hpp"
-
-#define HGX_TILE 4096
-#define HGX_NT 256
-#define HGX_EPT 16  // elements of a full tile per thread
-#define HGX_PAD(x) ((x) + ((x) >> 4))
-#define HGX_TILE_LDS (HGX_TILE + HGX_TILE / 16)
-#define HGX_RUN 16  // limbs per carry run
-
-enum { MODE_MUL = 0, MODE_SQR = 1, MODE_FWD = 2, MODE_INV = 3 };
-
-// Moduli and in-tile root tables.  tw_fwd[k][i] = w_4096^i * R mod p_k and
-// tw_inv[k][i] = w_4096^-i * R mod p_k for i < 2048, with
-// w_4096 = g^((p-1)/4096) (primitive-root-from-generator, src/moduli.lisp:182-187).
-struct PrimeSet {
-  u64 p[3];
-  u32 c[3];
-  const u64 *tw_fwd[3];
-  const u64 *tw_inv[3];
-  // Garner constants in Montgomery form: p0^-1 mod p1, p0^-1 mod p2, p1^-1 mod p2
-  u64 g01, g02, g12;
-};
-
-// Two-level root table of one column pass on blocks of length 2^b:
-// w_B^e = TH[e >> lb] * TL[e & (2^lb - 1)], both in Montgomery form.
-struct ColTw {
-  const u64 *TL[3];
-  const u64 *TH[3];
-  int lb;
-};
-
-// One multiplication as the kernels see it: operands, product and work
-// arrays in device memory.  A launch covers one job (passed by value) or a
-// table of jobs of the same transform length (a batch); blockIdx.x selects
-// job = blockIdx.x >> jt_bits and its tile = blockIdx.x & (2^jt_bits - 1).
-struct MulJob {
-  const u64 *a;
-  const u64 *b;
-  u64 na, nb;
-  u64 *out;       // product limbs
-  u64 nout;       // limbs to store (na + nb); positions at or above are zero
-  u64 *res0;      // 3 L residues of operand a (prime k at + k L); the product spectrum lands here
-  u64 *res1;      // 3 L residues of operand b
-  u64 *ovf;       // 2 words per run: what the run's sum carries past its 16 limbs
-  u32 *flags;     // per run: limbs 2..15 of the run are all ones
-  uint8_t *maps;  // per run: carry map (kill / propagate / generate)
-  uint8_t *agg;   // per block of 1024 runs: combined map
-  uint8_t *blkin; // per block of 1024 runs: carry into the block
-};
-
-struct JobRef {
-  MulJob one;
-  const MulJob *table;  // null: every block works on ONE
-};
-
-__device__ __forceinline__ MulJob load_job(const JobRef &jr, u32 j) { return jr.table ? jr.table[j] : jr.one; }
+#include "hgx_types.hpp"
 
 __device__ __forceinline__ u32 bitrev8(u32 m) { return __builtin_bitreverse32(m) >> 24; }
 
-__device__ __forceinline__ u64 twiddle2(const u64 *TH, const u64 *TL, int lb, u64 e, u64 p, u32 c) {
-  u64 hi = TH[e >> lb];
-  u64 lo = TL[e & ((1ull << lb) - 1)];
-  return mont_mul(hi, lo, p, c);
+// x * w_B^e from the two-level table: two Shoup products.
+__device__ __forceinline__ u64 apply_twiddle2(u64 x, const u64 *TH, const u64 *TL, int lb, u64 e, u64 p, u32 c) {
+  const ulonglong2 lo = *(const ulonglong2 *)(TL + 2 * (e & ((1ull << lb) - 1)));
+  const ulonglong2 hi = *(const ulonglong2 *)(TH + 2 * (e >> lb));
+  return shoup_mul(shoup_mul(x, lo.x, lo.y, p, c), hi.x, hi.y, p, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -162,8 +108,8 @@ __device__ __forceinline__ void tile_round(u64 *lds, int wbits, const u64 *__res
             x[j + half] = mod_sub(u, v, p);  // twiddle w^0 = 1
           } else {
             const int e = goff + (j & (half - 1)) * D;
-            const u64 w = tw[e * tstride];
-            x[j + half] = mont_mul(u + (p - v), w, p, c);
+            const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
+            x[j + half] = shoup_mul(u + (p - v), w.x, w.y, p, c);
           }
         }
       }
@@ -182,7 +128,8 @@ __device__ __forceinline__ void tile_round(u64 *lds, int wbits, const u64 *__res
             v = x[j + half];  // twiddle w^0 = 1
           } else {
             const int e = goff + (j & (half - 1)) * D;
-            v = mont_mul(x[j + half], tw[e * tstride], p, c);
+            const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
+            v = shoup_mul(x[j + half], w.x, w.y, p, c);
           }
           x[j] = mod_add(u, v, p);
           x[j + half] = mod_sub(u, v, p);
@@ -350,8 +297,7 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
         const u32 m = x >> 4;
         const u64 t = t0 + (x & 15);
         const u64 e = t * bitrev8(m);
-        const u64 w = twiddle2(tw.TH[k], tw.TL[k], tw.lb, e, p, c);
-        dst[gbase + (x & 15) + rs * m] = mont_mul(lds[HGX_PAD(x)], w, p, c);
+        dst[gbase + (x & 15) + rs * m] = apply_twiddle2(lds[HGX_PAD(x)], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
       }
       __syncthreads();
     }
@@ -373,8 +319,7 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
       const u32 m = x >> 4;
       const u64 t = t0 + (x & 15);
       const u64 e = t * bitrev8(m);
-      const u64 w = twiddle2(tw.TH[k], tw.TL[k], tw.lb, e, p, c);
-      arr[gbase + (x & 15) + rs * m] = mont_mul(lds[HGX_PAD(x)], w, p, c);
+      arr[gbase + (x & 15) + rs * m] = apply_twiddle2(lds[HGX_PAD(x)], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
     }
   }
 }
@@ -408,8 +353,7 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
       const u32 m = x >> 4;
       const u64 t = t0 + (x & 15);
       const u64 e = t * bitrev8(m);
-      const u64 w = twiddle2(tw.TH[k], tw.TL[k], tw.lb, e, p, c);
-      lds[HGX_PAD(x)] = mont_mul(arr[gbase + (x & 15) + rs * m], w, p, c);
+      lds[HGX_PAD(x)] = apply_twiddle2(arr[gbase + (x & 15) + rs * m], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
     }
     __syncthreads();
     tile_inverse<8, true>(lds, 4, ps.tw_inv[k], p, c);
@@ -431,8 +375,7 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
         const u32 m = x >> 4;
         const u64 t = t0 + (x & 15);
         const u64 e = t * bitrev8(m);
-        const u64 w = twiddle2(tw.TH[k], tw.TL[k], tw.lb, e, p, c);
-        lds[HGX_PAD(x)] = mont_mul(arr[gbase + (x & 15) + rs * m], w, p, c);
+        lds[HGX_PAD(x)] = apply_twiddle2(arr[gbase + (x & 15) + rs * m], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
       }
       __syncthreads();
       tile_inverse<8, true>(lds, 4, ps.tw_inv[k], p, c);
@@ -465,12 +408,6 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
 //   MODE_FWD / MODE_INV: transform-only row pass on res0 (prime kbase + y).
 //   SINGLE (L <= 4096): the limbs come in, all three primes are done in
 //     turn, and the CRT + carry sum goes out (as k_inv_col<true>).
-struct RowParams {
-  u64 L;
-  int tw_bits;
-  int kbase;
-  u64 scale[3];  // Montgomery-form factor applied to the pointwise product when SINGLE
-};
 
 template <int S_BITS, int MODE, bool SINGLE>
 __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRef jr, int jt_bits) {
@@ -582,8 +519,6 @@ __global__ __launch_bounds__(HGX_NT) void k_row_inv_scaled(PrimeSet ps, RowParam
 // The carry into a run is the value of the nearest non-propagating map before
 // it (0 if none): a last-non-P scan, done per 1024 runs and then across blocks.
 // ---------------------------------------------------------------------------
-#define HGX_CARRY_RUNS_PER_THREAD 4
-#define HGX_CARRY_RUNS_PER_BLOCK (HGX_NT * HGX_CARRY_RUNS_PER_THREAD)
 
 __device__ __forceinline__ u32 run_map(const u64 *out, u64 n_out, const u64 *ovf, const u32 *flags, u64 r,
                                        int rl) {
@@ -611,7 +546,7 @@ __device__ __forceinline__ u32 run_map(const u64 *out, u64 n_out, const u64 *ovf
 // combine(earlier, later): later unless it propagates
 __device__ __forceinline__ u32 map_combine(u32 a, u32 b) { return (b == 2u) ? a : b; }
 
-__device__ u32 block_exclusive_last(u32 v, u32 *sh) {
+__device__ inline u32 block_exclusive_last(u32 v, u32 *sh) {
   // inclusive Hillis-Steele scan of map_combine over the block, then shift
   const int tid = threadIdx.x;
   sh[tid] = v;
@@ -627,103 +562,3 @@ __device__ u32 block_exclusive_last(u32 v, u32 *sh) {
   return ex;
 }
 
-// per-run maps and each block's combined map.
-// grid = jobs << bpj_bits (bpj = blocks of 1024 runs per job)
-__global__ __launch_bounds__(HGX_NT) void k_carry_map(JobRef jr, int bpj_bits, u64 nruns, int rl) {
-  __shared__ u32 sh[HGX_NT];
-  const int tid = threadIdx.x;
-  const MulJob J = load_job(jr, blockIdx.x >> bpj_bits);
-  const u64 blk = blockIdx.x & ((1u << bpj_bits) - 1);
-  const u64 r0 = blk * HGX_CARRY_RUNS_PER_BLOCK + (u64)tid * HGX_CARRY_RUNS_PER_THREAD;
-  u32 summary = 2u;
-#pragma unroll
-  for (int i = 0; i < HGX_CARRY_RUNS_PER_THREAD; i++) {
-    const u64 r = r0 + i;
-    if (r < nruns) {
-      const u32 m = run_map(J.out, J.nout, J.ovf, J.flags, r, rl);
-      J.maps[r] = (uint8_t)m;
-      summary = map_combine(summary, m);
-    }
-  }
-  sh[tid] = summary;
-  __syncthreads();
-  for (int s = 1; s < HGX_NT; s <<= 1) {
-    if ((tid & (2 * s - 1)) == 0) sh[tid] = map_combine(sh[tid], sh[tid + s]);
-    __syncthreads();
-  }
-  if (tid == 0) J.agg[blk] = (uint8_t)sh[0];
-}
-
-// carry into every block of runs; one workgroup of 1024 threads per job
-__global__ __launch_bounds__(1024) void k_carry_blocks(JobRef jr, u64 nblocks) {
-  __shared__ u32 sh[1024];
-  const int tid = threadIdx.x;
-  const MulJob J = load_job(jr, blockIdx.x);
-  const u64 chunk = (nblocks + 1023) / 1024;
-  const u64 b0 = (u64)tid * chunk;
-  u64 b1 = b0 + chunk;
-  if (b1 > nblocks) b1 = nblocks;
-  u32 summary = 2u;
-  for (u64 b = b0; b < b1; b++) summary = map_combine(summary, J.agg[b]);
-  u32 state = block_exclusive_last(summary, sh);
-  for (u64 b = b0; b < b1; b++) {
-    J.blkin[b] = (state == 3u) ? 1 : 0;
-    state = map_combine(state, J.agg[b]);
-  }
-}
-
-// add ovf[r-1] and the carry bit into every run (grid as k_carry_map)
-__global__ __launch_bounds__(HGX_NT) void k_carry_apply(JobRef jr, int bpj_bits, u64 nruns, int rl) {
-  __shared__ u32 sh[HGX_NT];
-  const int tid = threadIdx.x;
-  const MulJob J = load_job(jr, blockIdx.x >> bpj_bits);
-  const u64 blk = blockIdx.x & ((1u << bpj_bits) - 1);
-  const u64 r0 = blk * HGX_CARRY_RUNS_PER_BLOCK + (u64)tid * HGX_CARRY_RUNS_PER_THREAD;
-  u32 summary = 2u;
-#pragma unroll
-  for (int i = 0; i < HGX_CARRY_RUNS_PER_THREAD; i++) {
-    const u64 r = r0 + i;
-    if (r < nruns) summary = map_combine(summary, J.maps[r]);
-  }
-  const u32 ex = block_exclusive_last(summary, sh);
-  u64 e = (ex == 2u) ? (u64)J.blkin[blk] : (ex == 3u ? 1ull : 0ull);
-  for (int i = 0; i < HGX_CARRY_RUNS_PER_THREAD; i++) {
-    const u64 r = r0 + i;
-    if (r >= nruns) break;
-    u64 x0 = 0, x1 = 0;
-    if (r > 0) {
-      x0 = J.ovf[2 * (r - 1)];
-      x1 = J.ovf[2 * (r - 1) + 1];
-    }
-    const u64 base = r * (u64)rl;
-    if (x0 | x1 | e) {
-      u64 carry = e;
-      for (int j = 0; j < rl; j++) {
-        const u64 pos = base + j;
-        if (pos >= J.nout) break;
-        const u64 add = (j == 0) ? x0 : (j == 1 ? x1 : 0);
-        if (j >= 2 && carry == 0) break;
-        const u64 d = J.out[pos];
-        const u64 s = d + add;
-        u64 cy = (s < d) ? 1 : 0;
-        const u64 s2 = s + carry;
-        cy += (s2 < s) ? 1 : 0;
-        J.out[pos] = s2;
-        carry = cy;
-      }
-    }
-    e = (J.maps[r] >> e) & 1u;
-  }
-}
-
-// SplitMix64 stream: out[i] = mix(seed + (i+1) * 0x9E3779B97F4A7C15)
-__global__ void k_splitmix64(u64 *out, u64 n, u64 seed, int force_top) {
-  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  u64 z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z = z ^ (z >> 31);
-  if (force_top && i == n - 1 && z == 0) z = 1;
-  out[i] = z;
-}

@@ -1,0 +1,25 @@
+// hgx_launch.hpp -- host launchers of the kernels (one per kernel family).
+// The kernels are compiled in separate translation units (hgx_k_*.hip);
+// the runtime calls these functions and holds no device code itself.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hgx_types.hpp"
+
+void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr,
+                        int jt_bits, u64 L, int bbits, int kbase);
+void hgx_launch_inv_col(bool last, dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr,
+                        int jt_bits, u64 L, int bbits, int kbase);
+// row passes; each returns false when SBITS is out of range
+bool hgx_launch_row_mul(bool single, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
+                        const JobRef &jr, int jt_bits);
+bool hgx_launch_row_sqr(bool single, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
+                        const JobRef &jr, int jt_bits);
+bool hgx_launch_row_tr(int mode, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
+                       const JobRef &jr, int jt_bits);
+bool hgx_launch_row_inv_scaled(int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
+                               const JobRef &jr);
+void hgx_launch_carry_map(dim3 grid, hipStream_t s, const JobRef &jr, int bpj_bits, u64 nruns, int rl);
+void hgx_launch_carry_blocks(dim3 grid, hipStream_t s, const JobRef &jr, u64 nblocks);
+void hgx_launch_carry_apply(dim3 grid, hipStream_t s, const JobRef &jr, int bpj_bits, u64 nruns, int rl);
+void hgx_launch_splitmix64(dim3 grid, hipStream_t s, u64 *out, u64 n, u64 seed, int force_top);

@@ -22,7 +22,7 @@
 #include <vector>
 
 #include "../../include/hgx.h"
-#include "hgx_kernels.hpp"
+#include "hgx_launch.hpp"
 
 typedef unsigned __int128 u128;
 
@@ -59,6 +59,7 @@ u64 powmod(u64 a, u64 e, u64 m) {
 }
 u64 invmod(u64 a, u64 m) { return powmod(a, m - 2, m); }  // m prime
 u64 to_mont(u64 a, u64 m) { return (u64)((((u128)(a % m)) << 64) % m); }
+u64 shoup_companion(u64 w, u64 m) { return (u64)((((u128)w) << 64) / m); }  // calc-b-inv
 
 bool is_prime_small(u64 n) {
   if (n < 2) return false;
@@ -232,14 +233,16 @@ int build_plan(int n, Plan **out) {
       const u64 wl = powmod(w, 1ull << lb, p), wil = powmod(wi, 1ull << lb, p);
       const u64 Linv = invmod(L % p, p);
       const u64 Rm = (u64)((((u128)1) << 64) % p);
-      const u64 kmul = mulmod(Rm, Linv, p);  // R / L: undoes the pointwise REDC and scales by 1/L
+      const u64 kmul = mulmod(Rm, Linv, p);  // R / L: undoes the pointwise REDC's 1/R and scales by 1/L
       const u64 ktr = Linv;                   // 1 / L
       Off &o = offs[i * 3 + k];
+      // (value, Shoup companion floor(value * 2^64 / p)) pairs
       auto push_pow = [&](u64 base, size_t cnt, u64 scale) {
         size_t at = host.size();
         u64 x = scale % p;
         for (size_t j = 0; j < cnt; j++) {
-          host.push_back(to_mont(x, p));
+          host.push_back(x);
+          host.push_back(shoup_companion(x, p));
           x = mulmod(x, base, p);
         }
         return at;
@@ -290,32 +293,20 @@ int launch_row(int sbits, dim3 grid, hipStream_t s, const RowParams &rp, const J
   static const char *names[4] = {SINGLE ? "row_single_mul" : "row_mul", SINGLE ? "row_single_sqr" : "row_sqr",
                                  "row_fwd", "row_inv"};
   HGX_TIMED(names[MODE], s);
-  switch (sbits) {
-#define HGX_ROW_CASE(S) \
-  case S:               \
-    hipLaunchKernelGGL((k_row<S, MODE, SINGLE>), grid, dim3(HGX_NT), 0, s, g_ctx.ps, rp, jr, jt_bits); break;
-    HGX_ROW_CASE(1) HGX_ROW_CASE(2) HGX_ROW_CASE(3) HGX_ROW_CASE(4) HGX_ROW_CASE(5) HGX_ROW_CASE(6)
-    HGX_ROW_CASE(7) HGX_ROW_CASE(8) HGX_ROW_CASE(9) HGX_ROW_CASE(10) HGX_ROW_CASE(11) HGX_ROW_CASE(12)
-#undef HGX_ROW_CASE
-    default:
-      return fail(HGX_EINVAL, "row pass size out of range");
-  }
-  return HGX_OK;
+  bool ok = false;
+  if (MODE == MODE_MUL)
+    ok = hgx_launch_row_mul(SINGLE, sbits, grid, s, g_ctx.ps, rp, jr, jt_bits);
+  else if (MODE == MODE_SQR)
+    ok = hgx_launch_row_sqr(SINGLE, sbits, grid, s, g_ctx.ps, rp, jr, jt_bits);
+  else
+    ok = hgx_launch_row_tr(MODE, sbits, grid, s, g_ctx.ps, rp, jr, jt_bits);
+  return ok ? HGX_OK : fail(HGX_EINVAL, "row pass size out of range");
 }
 
 int launch_row_inv_scaled(int sbits, dim3 grid, hipStream_t s, const RowParams &rp, const JobRef &jr) {
   HGX_TIMED("row_inv_scaled", s);
-  switch (sbits) {
-#define HGX_ROW_CASE(S) \
-  case S:               \
-    hipLaunchKernelGGL((k_row_inv_scaled<S>), grid, dim3(HGX_NT), 0, s, g_ctx.ps, rp, jr); break;
-    HGX_ROW_CASE(1) HGX_ROW_CASE(2) HGX_ROW_CASE(3) HGX_ROW_CASE(4) HGX_ROW_CASE(5) HGX_ROW_CASE(6)
-    HGX_ROW_CASE(7) HGX_ROW_CASE(8) HGX_ROW_CASE(9) HGX_ROW_CASE(10) HGX_ROW_CASE(11) HGX_ROW_CASE(12)
-#undef HGX_ROW_CASE
-    default:
-      return fail(HGX_EINVAL, "row pass size out of range");
-  }
-  return HGX_OK;
+  return hgx_launch_row_inv_scaled(sbits, grid, s, g_ctx.ps, rp, jr) ? HGX_OK
+                                                                       : fail(HGX_EINVAL, "row pass size out of range");
 }
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -407,13 +398,13 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     const unsigned grid = njobs << jt;
     {
       HGX_TIMED("fwd_col_first", s);
-      hipLaunchKernelGGL((k_fwd_col<true>), dim3(grid, 1, nops), dim3(HGX_NT), 0, s, g_ctx.ps,
-                         coltw(plan->col[0], 0), jr, jt, L, plan->col[0].bbits, 0);
+      hgx_launch_fwd_col(true, dim3(grid, 1, nops), s, g_ctx.ps, coltw(plan->col[0], 0), jr, jt, L,
+                         plan->col[0].bbits, 0);
     }
     for (int i = 1; i < plan->ncol; i++) {
       HGX_TIMED("fwd_col", s);
-      hipLaunchKernelGGL((k_fwd_col<false>), dim3(grid, 3, nops), dim3(HGX_NT), 0, s, g_ctx.ps,
-                         coltw(plan->col[i], 0), jr, jt, L, plan->col[i].bbits, 0);
+      hgx_launch_fwd_col(false, dim3(grid, 3, nops), s, g_ctx.ps, coltw(plan->col[i], 0), jr, jt, L,
+                         plan->col[i].bbits, 0);
     }
     RowParams rp{};
     rp.L = L;
@@ -424,26 +415,24 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     if (rc) return rc;
     for (int i = plan->ncol - 1; i >= 1; i--) {
       HGX_TIMED("inv_col", s);
-      hipLaunchKernelGGL((k_inv_col<false>), dim3(grid, 3), dim3(HGX_NT), 0, s, g_ctx.ps, coltw(plan->col[i], 1), jr,
-                         jt, L, plan->col[i].bbits, 0);
+      hgx_launch_inv_col(false, dim3(grid, 3), s, g_ctx.ps, coltw(plan->col[i], 1), jr, jt, L, plan->col[i].bbits, 0);
     }
     HGX_TIMED("inv_col_crt", s);
-    hipLaunchKernelGGL((k_inv_col<true>), dim3(grid, 1), dim3(HGX_NT), 0, s, g_ctx.ps, coltw(plan->col[0], 2), jr, jt,
-                       L, plan->col[0].bbits, 0);
+    hgx_launch_inv_col(true, dim3(grid, 1), s, g_ctx.ps, coltw(plan->col[0], 2), jr, jt, L, plan->col[0].bbits, 0);
   }
   const u64 nblk = (nruns + HGX_CARRY_RUNS_PER_BLOCK - 1) / HGX_CARRY_RUNS_PER_BLOCK;
   const int bpj = log2_ceil(nblk);
   {
     HGX_TIMED("carry_map", s);
-    hipLaunchKernelGGL(k_carry_map, dim3(njobs << bpj), dim3(HGX_NT), 0, s, jr, bpj, nruns, (int)rl);
+    hgx_launch_carry_map(dim3(njobs << bpj), s, jr, bpj, nruns, (int)rl);
   }
   {
     HGX_TIMED("carry_blocks", s);
-    hipLaunchKernelGGL(k_carry_blocks, dim3(njobs), dim3(1024), 0, s, jr, nblk);
+    hgx_launch_carry_blocks(dim3(njobs), s, jr, nblk);
   }
   {
     HGX_TIMED("carry_apply", s);
-    hipLaunchKernelGGL(k_carry_apply, dim3(njobs << bpj), dim3(HGX_NT), 0, s, jr, bpj, nruns, (int)rl);
+    hgx_launch_carry_apply(dim3(njobs << bpj), s, jr, bpj, nruns, (int)rl);
   }
   HIPCHK(hipGetLastError());
   return HGX_OK;
@@ -470,16 +459,15 @@ int enqueue_transform(u64 *d, size_t nwords, int mod_num, bool inverse, hipStrea
   if (!inverse) {
     for (int i = 0; i < plan->ncol; i++) {
       HGX_TIMED("fwd_col", s);
-      hipLaunchKernelGGL((k_fwd_col<false>), dim3(tiles, 1, 1), dim3(HGX_NT), 0, s, g_ctx.ps, coltw(plan->col[i], 0),
-                         jr, jt, L, plan->col[i].bbits, mod_num);
+      hgx_launch_fwd_col(false, dim3(tiles, 1, 1), s, g_ctx.ps, coltw(plan->col[i], 0), jr, jt, L, plan->col[i].bbits,
+                         mod_num);
     }
     rc = launch_row<MODE_FWD, false>(plan->srow, dim3(tiles, 1), s, rp, jr, jt);
     if (rc) return rc;
   } else {
     if (plan->ncol == 0) {
       const u64 p = kModuli[mod_num];
-      const u64 Rm = (u64)((((u128)1) << 64) % p);
-      rp.scale[mod_num] = to_mont(mulmod(Rm, invmod(L % p, p), p), p);  // times 1/L after REDC
+      rp.scale[mod_num] = to_mont(invmod(L % p, p), p);  // REDC(x, (1/L) R) = x / L
       rc = launch_row_inv_scaled(plan->srow, dim3(tiles), s, rp, jr);
       if (rc) return rc;
     } else {
@@ -487,8 +475,8 @@ int enqueue_transform(u64 *d, size_t nwords, int mod_num, bool inverse, hipStrea
       if (rc) return rc;
       for (int i = plan->ncol - 1; i >= 0; i--) {
         HGX_TIMED("inv_col", s);
-        hipLaunchKernelGGL((k_inv_col<false>), dim3(tiles, 1), dim3(HGX_NT), 0, s, g_ctx.ps,
-                           coltw(plan->col[i], i == 0 ? 3 : 1), jr, jt, L, plan->col[i].bbits, mod_num);
+        hgx_launch_inv_col(false, dim3(tiles, 1), s, g_ctx.ps, coltw(plan->col[i], i == 0 ? 3 : 1), jr, jt, L,
+                           plan->col[i].bbits, mod_num);
       }
     }
   }
@@ -599,11 +587,254 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
 }
 
 // ---- request server (continuous batching) ----
-struct Request {
-  hgx_mul_desc d;
-  std::atomic<int> state{0};  // 0 pending, 1 done, <0 error
+//
+// hgx_submit queues a request and returns.  A server thread admits queued
+// requests in arrival order while their memory (operands, product, work
+// arrays) fits in a device arena carved from the HBM budget, groups each
+// admission round by transform length into batched launches, and spreads
+// rounds over several streams so rounds overlap on the device.  A request
+// that arrives while others run joins the next round as soon as memory
+// frees up.  Completion is observed from events; the arena block is then
+// returned and the ticket completes.
+struct SrvReq {
+  hgx_mul_desc d{};
+  bool sqr = false;
+  int n = 0;
+  size_t bytes = 0, off = 0;
+  size_t table_off = (size_t)-1, table_bytes = 0;  // job table of its group (last request of a group owns it)
+  std::vector<MulJob> table_host;
+  hipEvent_t done_ev = nullptr;
+  int state = 0;  // 0 queued, 1 in flight, 2 done, <0 failed
   std::string err;
 };
+
+constexpr int kServeStreams = 4;
+
+struct Server {
+  bool started = false, stop = false;
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::shared_ptr<SrvReq>> queue;
+  std::vector<std::shared_ptr<SrvReq>> inflight;
+  std::unordered_map<hgx_ticket, std::shared_ptr<SrvReq>> tickets;
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
+  std::map<size_t, size_t> free_list;  // offset -> size
+  hipStream_t streams[kServeStreams]{};
+  int next_stream = 0;
+  std::vector<hipEvent_t> event_pool;
+};
+
+Server g_srv;
+
+size_t arena_alloc(size_t bytes) {
+  bytes = align_up(bytes);
+  for (auto it = g_srv.free_list.begin(); it != g_srv.free_list.end(); ++it) {
+    if (it->second >= bytes) {
+      const size_t off = it->first, sz = it->second;
+      g_srv.free_list.erase(it);
+      if (sz > bytes) g_srv.free_list[off + bytes] = sz - bytes;
+      return off;
+    }
+  }
+  return (size_t)-1;
+}
+
+void arena_free(size_t off, size_t bytes) {
+  bytes = align_up(bytes);
+  auto it = g_srv.free_list.emplace(off, bytes).first;
+  auto nx = std::next(it);
+  if (nx != g_srv.free_list.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    g_srv.free_list.erase(nx);
+  }
+  if (it != g_srv.free_list.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      g_srv.free_list.erase(it);
+    }
+  }
+}
+
+size_t request_bytes(const SrvReq &r) {
+  const size_t in = align_up(r.d.na * 8) + (r.sqr ? 0 : align_up(r.d.nb * 8));
+  return in + align_up((r.d.na + r.d.nb) * 8) + work_layout(r.n, r.sqr).total;
+}
+
+hipEvent_t take_event() {
+  if (!g_srv.event_pool.empty()) {
+    hipEvent_t e = g_srv.event_pool.back();
+    g_srv.event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  return e;
+}
+
+// Launch one admission round on stream s (called without g_srv.mu held).
+void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  std::stable_sort(batch.begin(), batch.end(), [](const std::shared_ptr<SrvReq> &x, const std::shared_ptr<SrvReq> &y) {
+    return x->n != y->n ? x->n < y->n : x->sqr < y->sqr;
+  });
+  // operands in
+  std::vector<MulJob> jobs(batch.size());
+  for (size_t i = 0; i < batch.size(); i++) {
+    SrvReq &r = *batch[i];
+    char *base = g_srv.arena + r.off;
+    u64 *da = (u64 *)base;
+    u64 *db = r.sqr ? da : (u64 *)(base + align_up(r.d.na * 8));
+    u64 *dout = (u64 *)(base + align_up(r.d.na * 8) + (r.sqr ? 0 : align_up(r.d.nb * 8)));
+    char *work = (char *)dout + align_up((r.d.na + r.d.nb) * 8);
+    (void)hipMemcpyAsync(da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s);
+    if (!r.sqr) (void)hipMemcpyAsync(db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
+    jobs[i] = make_job(da, r.d.na, db, r.d.nb, dout, r.n, r.sqr, work);
+  }
+  // one launch sequence per (length, square) group
+  size_t g = 0;
+  while (g < batch.size()) {
+    size_t h = g;
+    while (h < batch.size() && batch[h]->n == batch[g]->n && batch[h]->sqr == batch[g]->sqr) h++;
+    int rc = HGX_OK;
+    size_t toff = (size_t)-1;
+    if (h - g > 1) {
+      std::lock_guard<std::mutex> lk2(g_srv.mu);
+      toff = arena_alloc((h - g) * sizeof(MulJob));
+    }
+    if (toff != (size_t)-1) {
+      SrvReq &owner = *batch[h - 1];
+      owner.table_off = toff;
+      owner.table_bytes = (h - g) * sizeof(MulJob);
+      owner.table_host.assign(jobs.begin() + g, jobs.begin() + h);
+      MulJob *dtab = (MulJob *)(g_srv.arena + toff);
+      (void)hipMemcpyAsync(dtab, owner.table_host.data(), owner.table_bytes, hipMemcpyHostToDevice, s);
+      JobRef jr{};
+      jr.table = dtab;
+      rc = enqueue_group(jr, (unsigned)(h - g), batch[g]->n, batch[g]->sqr, s);
+    } else {
+      for (size_t i = g; i < h && rc == HGX_OK; i++) {
+        JobRef jr{};
+        jr.one = jobs[i];
+        rc = enqueue_group(jr, 1, batch[i]->n, batch[i]->sqr, s);
+      }
+    }
+    if (rc) {
+      for (size_t i = g; i < h; i++) batch[i]->err = g_err;
+    }
+    g = h;
+  }
+  // products out, completion events
+  for (size_t i = 0; i < batch.size(); i++) {
+    SrvReq &r = *batch[i];
+    (void)hipMemcpyAsync(r.d.out, jobs[i].out, (r.d.na + r.d.nb) * 8, hipMemcpyDeviceToHost, s);
+    (void)hipEventRecord(r.done_ev, s);
+  }
+}
+
+void server_loop() {
+  std::unique_lock<std::mutex> lk(g_srv.mu);
+  while (!g_srv.stop) {
+    // retire finished requests
+    for (size_t i = 0; i < g_srv.inflight.size();) {
+      auto &r = g_srv.inflight[i];
+      if (hipEventQuery(r->done_ev) == hipSuccess) {
+        arena_free(r->off, r->bytes);
+        if (r->table_off != (size_t)-1) arena_free(r->table_off, r->table_bytes);
+        g_srv.event_pool.push_back(r->done_ev);
+        r->done_ev = nullptr;
+        r->state = r->err.empty() ? 2 : HGX_EDEVICE;
+        g_srv.inflight[i] = g_srv.inflight.back();
+        g_srv.inflight.pop_back();
+        g_srv.cv.notify_all();
+      } else {
+        i++;
+      }
+    }
+    // admit in arrival order while memory lasts
+    std::vector<std::shared_ptr<SrvReq>> round;
+    while (!g_srv.queue.empty()) {
+      auto r = g_srv.queue.front();
+      if (r->bytes > g_srv.arena_bytes) {
+        r->state = HGX_ENOMEM;
+        r->err = "request needs " + std::to_string(r->bytes) + " bytes, the serving arena holds " +
+                 std::to_string(g_srv.arena_bytes);
+        g_srv.queue.pop_front();
+        g_srv.cv.notify_all();
+        continue;
+      }
+      const size_t off = arena_alloc(r->bytes);
+      if (off == (size_t)-1) break;
+      r->off = off;
+      r->state = 1;
+      r->done_ev = take_event();
+      round.push_back(r);
+      g_srv.queue.pop_front();
+    }
+    if (!round.empty()) {
+      for (auto &r : round) g_srv.inflight.push_back(r);
+      hipStream_t s = g_srv.streams[g_srv.next_stream++ % kServeStreams];
+      lk.unlock();
+      launch_round(round, s);
+      lk.lock();
+      continue;
+    }
+    if (g_srv.inflight.empty())
+      g_srv.cv.wait(lk, [] { return g_srv.stop || !g_srv.queue.empty(); });
+    else
+      g_srv.cv.wait_for(lk, std::chrono::microseconds(20));
+  }
+}
+
+void server_stop();
+void server_stop_at_exit() { server_stop(); }
+
+int server_start_locked() {
+  if (g_srv.started) return HGX_OK;
+  static bool registered = false;
+  if (!registered) {
+    std::atexit(server_stop_at_exit);  // join the server thread before static destruction
+    registered = true;
+  }
+  size_t arena = 0;
+  if (const char *e = getenv("HGX_SERVE_ARENA_BYTES")) arena = strtoull(e, nullptr, 10);
+  if (!arena) arena = (g_ctx.budget - g_ctx.allocated) / 2;
+  arena &= ~(size_t)255;
+  int rc = dev_alloc((void **)&g_srv.arena, arena);
+  if (rc) return rc;
+  g_srv.arena_bytes = arena;
+  g_srv.free_list.clear();
+  g_srv.free_list[0] = arena;
+  for (int i = 0; i < kServeStreams; i++) HIPCHK(hipStreamCreateWithFlags(&g_srv.streams[i], hipStreamNonBlocking));
+  g_srv.stop = false;
+  g_srv.th = std::thread(server_loop);
+  g_srv.started = true;
+  return HGX_OK;
+}
+
+void server_stop() {
+  if (!g_srv.started) return;
+  {
+    std::lock_guard<std::mutex> lk(g_srv.mu);
+    g_srv.stop = true;
+  }
+  g_srv.cv.notify_all();
+  g_srv.th.join();
+  for (int i = 0; i < kServeStreams; i++) {
+    (void)hipStreamSynchronize(g_srv.streams[i]);
+    (void)hipStreamDestroy(g_srv.streams[i]);
+  }
+  for (auto e : g_srv.event_pool) (void)hipEventDestroy(e);
+  g_srv.event_pool.clear();
+  dev_free(g_srv.arena, g_srv.arena_bytes);
+  g_srv.arena = nullptr;
+  g_srv.arena_bytes = 0;
+  g_srv.queue.clear();
+  g_srv.inflight.clear();
+  g_srv.started = false;
+}
 
 }  // namespace
 
@@ -631,7 +862,7 @@ int hgx_init(int device, size_t hbm_budget_bytes) {
   HIPCHK(hipEventCreateWithFlags(&g_ctx.jobs_copied, hipEventDisableTiming));
   HIPCHK(hipEventRecord(g_ctx.jobs_copied, g_ctx.stream));
   // prime set: moduli, Proth factors, generators, in-tile root tables
-  std::vector<u64> host(3 * 2 * 2048);
+  std::vector<u64> host(3 * 2 * 2048 * 2);
   PrimeSet ps{};
   for (int k = 0; k < 3; k++) {
     const u64 p = kModuli[k];
@@ -642,8 +873,10 @@ int hgx_init(int device, size_t hbm_budget_bytes) {
     const u64 wi = invmod(w, p);
     u64 x = 1, y = 1;
     for (int i = 0; i < 2048; i++) {
-      host[(k * 2 + 0) * 2048 + i] = to_mont(x, p);
-      host[(k * 2 + 1) * 2048 + i] = to_mont(y, p);
+      host[((k * 2 + 0) * 2048 + i) * 2] = x;
+      host[((k * 2 + 0) * 2048 + i) * 2 + 1] = shoup_companion(x, p);
+      host[((k * 2 + 1) * 2048 + i) * 2] = y;
+      host[((k * 2 + 1) * 2048 + i) * 2 + 1] = shoup_companion(y, p);
       x = mulmod(x, w, p);
       y = mulmod(y, wi, p);
     }
@@ -655,8 +888,8 @@ int hgx_init(int device, size_t hbm_budget_bytes) {
   if (rc) return rc;
   HIPCHK(hipMemcpy(g_ctx.tables, host.data(), host.size() * sizeof(u64), hipMemcpyHostToDevice));
   for (int k = 0; k < 3; k++) {
-    ps.tw_fwd[k] = g_ctx.tables + (k * 2 + 0) * 2048;
-    ps.tw_inv[k] = g_ctx.tables + (k * 2 + 1) * 2048;
+    ps.tw_fwd[k] = g_ctx.tables + (k * 2 + 0) * 2048 * 2;
+    ps.tw_inv[k] = g_ctx.tables + (k * 2 + 1) * 2048 * 2;
   }
   g_ctx.ps = ps;
   g_ctx.ready = true;
@@ -664,6 +897,7 @@ int hgx_init(int device, size_t hbm_budget_bytes) {
 }
 
 void hgx_shutdown(void) {
+  server_stop();
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   if (!g_ctx.ready) return;
   hipStreamSynchronize(g_ctx.stream);
@@ -681,7 +915,7 @@ void hgx_shutdown(void) {
   g_ctx.jobs_host = nullptr;
   g_ctx.jobs_host_cap = 0;
   (void)hipEventDestroy(g_ctx.jobs_copied);
-  dev_free(g_ctx.tables, 3 * 2 * 2048 * sizeof(u64));
+  dev_free(g_ctx.tables, 3 * 2 * 2048 * 2 * sizeof(u64));
   g_ctx.tables = nullptr;
   hipEventDestroy(g_ctx.ws_done);
   hipStreamDestroy(g_ctx.stream);
@@ -866,58 +1100,59 @@ int hgx_fill_splitmix64(uint64_t *d_out, size_t n, uint64_t seed, int force_top,
   if (!d_out) return fail(HGX_EINVAL, "null output");
   if (n == 0) return HGX_OK;
   hipStream_t s = stream ? (hipStream_t)stream : g_ctx.stream;
-  hipLaunchKernelGGL(k_splitmix64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_out, (u64)n, seed, force_top);
+  hgx_launch_splitmix64(dim3((unsigned)((n + 255) / 256)), s, d_out, (u64)n, seed, force_top);
   HIPCHK(hipGetLastError());
   return HGX_OK;
 }
 
-// Request server: submitted products are executed in arrival order by a
-// worker thread (replaced by the continuous-batching server).
-static std::mutex g_req_mu;
-static std::unordered_map<hgx_ticket, std::shared_ptr<Request>> g_reqs;
-static std::atomic<hgx_ticket> g_next_ticket{1};
-
 int hgx_submit(const hgx_mul_desc *desc, hgx_ticket *ticket) {
   if (!desc || !ticket) return fail(HGX_EINVAL, "null descriptor or ticket");
-  auto r = std::make_shared<Request>();
-  r->d = *desc;
-  const hgx_ticket t = g_next_ticket++;
+  const hgx_mul_desc &d = *desc;
+  if (!d.a || !d.b || !d.out || d.na == 0 || d.nb == 0) return fail(HGX_EINVAL, "null operand or zero length");
   {
-    std::lock_guard<std::mutex> lk(g_req_mu);
-    g_reqs[t] = r;
+    std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+    int rc = check_ready();
+    if (rc) return rc;
+    rc = server_start_locked();
+    if (rc) return rc;
   }
-  std::thread([r]() {
-    int rc = hgx_mul(r->d.a, r->d.na, r->d.b, r->d.nb, r->d.out);
-    if (rc) r->err = g_err;
-    r->state = rc ? rc : 1;
-  }).detach();
+  auto r = std::make_shared<SrvReq>();
+  r->d = d;
+  r->sqr = (d.a == d.b && d.na == d.nb);
+  r->n = log2_ceil((u64)hgx_product_length(d.na, d.nb));
+  if (r->n > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32");
+  r->bytes = request_bytes(*r);
+  static std::atomic<hgx_ticket> next_ticket{1};
+  const hgx_ticket t = next_ticket++;
+  {
+    std::lock_guard<std::mutex> lk(g_srv.mu);
+    g_srv.tickets[t] = r;
+    g_srv.queue.push_back(r);
+  }
+  g_srv.cv.notify_all();
   *ticket = t;
   return HGX_OK;
 }
 
 int hgx_poll(hgx_ticket ticket) {
-  std::shared_ptr<Request> r;
-  {
-    std::lock_guard<std::mutex> lk(g_req_mu);
-    auto it = g_reqs.find(ticket);
-    if (it == g_reqs.end()) return fail(HGX_EINVAL, "unknown ticket");
-    r = it->second;
-  }
-  int st = r->state.load();
-  if (st < 0) g_err = r->err;
-  return st;
+  std::lock_guard<std::mutex> lk(g_srv.mu);
+  auto it = g_srv.tickets.find(ticket);
+  if (it == g_srv.tickets.end()) return fail(HGX_EINVAL, "unknown ticket");
+  const int st = it->second->state;
+  if (st == 2) return 1;
+  if (st < 0) return fail(st, it->second->err);
+  return 0;
 }
 
 int hgx_wait(hgx_ticket ticket) {
-  for (;;) {
-    int st = hgx_poll(ticket);
-    if (st != 0) {
-      std::lock_guard<std::mutex> lk(g_req_mu);
-      g_reqs.erase(ticket);
-      return st == 1 ? HGX_OK : st;
-    }
-    std::this_thread::sleep_for(std::chrono::microseconds(20));
-  }
+  std::unique_lock<std::mutex> lk(g_srv.mu);
+  auto it = g_srv.tickets.find(ticket);
+  if (it == g_srv.tickets.end()) return fail(HGX_EINVAL, "unknown ticket");
+  auto r = it->second;
+  g_srv.cv.wait(lk, [&] { return r->state == 2 || r->state < 0; });
+  g_srv.tickets.erase(ticket);
+  if (r->state < 0) return fail(r->state, r->err);
+  return HGX_OK;
 }
 
 int hgx_timing_enable(int on) {

@@ -1,0 +1,74 @@
+// hgx_types.hpp -- constants and the structs shared by the kernels and the
+// host runtime (job descriptors, prime set, root-table views).
+#pragma once
+#include "modarith.hpp"
+
+
+#define HGX_TILE 4096
+#define HGX_NT 256
+#define HGX_EPT 16  // elements of a full tile per thread
+#define HGX_PAD(x) ((x) + ((x) >> 4))
+#define HGX_TILE_LDS (HGX_TILE + HGX_TILE / 16)
+#define HGX_RUN 16  // limbs per carry run
+
+enum { MODE_MUL = 0, MODE_SQR = 1, MODE_FWD = 2, MODE_INV = 3 };
+
+// Moduli and in-tile root tables.  tw_fwd[k] holds the pairs
+// (w_4096^i, floor(w_4096^i * 2^64 / p_k)) for i < 2048 -- the root and its
+// Shoup companion, as the reference's roots table holds (w, w~)
+// (src/moduli.lisp:321-332) -- and tw_inv[k] the same for w_4096^-i, with
+// w_4096 = g^((p-1)/4096) (primitive-root-from-generator, src/moduli.lisp:182-187).
+struct PrimeSet {
+  u64 p[3];
+  u32 c[3];
+  const u64 *tw_fwd[3];
+  const u64 *tw_inv[3];
+  // Garner constants in Montgomery form: p0^-1 mod p1, p0^-1 mod p2, p1^-1 mod p2
+  u64 g01, g02, g12;
+};
+
+// Two-level root table of one column pass on blocks of length 2^b:
+// w_B^e = TH[e >> lb] * TL[e & (2^lb - 1)]; both tables hold (value, Shoup
+// companion) pairs, so a twiddle is applied as two Shoup products.
+struct ColTw {
+  const u64 *TL[3];
+  const u64 *TH[3];
+  int lb;
+};
+
+// One multiplication as the kernels see it: operands, product and work
+// arrays in device memory.  A launch covers one job (passed by value) or a
+// table of jobs of the same transform length (a batch); blockIdx.x selects
+// job = blockIdx.x >> jt_bits and its tile = blockIdx.x & (2^jt_bits - 1).
+struct MulJob {
+  const u64 *a;
+  const u64 *b;
+  u64 na, nb;
+  u64 *out;       // product limbs
+  u64 nout;       // limbs to store (na + nb); positions at or above are zero
+  u64 *res0;      // 3 L residues of operand a (prime k at + k L); the product spectrum lands here
+  u64 *res1;      // 3 L residues of operand b
+  u64 *ovf;       // 2 words per run: what the run's sum carries past its 16 limbs
+  u32 *flags;     // per run: limbs 2..15 of the run are all ones
+  uint8_t *maps;  // per run: carry map (kill / propagate / generate)
+  uint8_t *agg;   // per block of 1024 runs: combined map
+  uint8_t *blkin; // per block of 1024 runs: carry into the block
+};
+
+struct JobRef {
+  MulJob one;
+  const MulJob *table;  // null: every block works on ONE
+};
+
+__device__ __forceinline__ MulJob load_job(const JobRef &jr, u32 j) { return jr.table ? jr.table[j] : jr.one; }
+
+// Parameters of a row pass (see k_row).
+struct RowParams {
+  u64 L;
+  int tw_bits;
+  int kbase;
+  u64 scale[3];  // Montgomery-form factor applied to the pointwise product when SINGLE
+};
+
+#define HGX_CARRY_RUNS_PER_THREAD 4
+#define HGX_CARRY_RUNS_PER_BLOCK (HGX_NT * HGX_CARRY_RUNS_PER_THREAD)

@@ -37,22 +37,64 @@ HGX_HD void mul64x64(u64 a, u64 b, u64 &lo, u64 &hi) {
   lo = (u64)(u32)t00 | (t01 << 32);
 }
 
+// low 32 bits of a 24 x 24-bit product on the full-rate multiplier
+// (v_mul_u32_u24); the compiler would otherwise pick v_mul_lo_u32.
+HGX_HD u32 mul_u24(u32 a, u32 b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  u32 r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+#else
+  return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+#endif
+}
+
 // a*b*2^-64 mod p, canonical, for a < 2^64 and b < p (so a*b < p*2^64).
 HGX_HD u64 mont_mul(u64 a, u64 b, u64 p, u32 c) {
-  u64 lo, hi;
-  mul64x64(a, b, lo, hi);
-  // m = lo * (c*2^55 - 1) mod 2^64
-  u64 m = ((u64)((u32)lo * c) << 55) - lo;
-  // m*c as a 72-bit number: mc_hi * 2^32 + mc_lo32
-  u64 mc_lo = (u64)(u32)m * c;
-  u64 mc_hi = (u64)(u32)(m >> 32) * c + (mc_lo >> 32);
-  // floor(m*c / 2^9), and (m*c mod 2^9) * 2^55 + m with its carry
-  u64 q = (mc_hi << 23) | ((u32)mc_lo >> 9);
-  u64 low = ((u64)((u32)mc_lo & 511u) << 55) + m;
-  u64 carry = (low < m) ? 1 : 0;
-  // t = (T + m*p) / 2^64;  T_lo + lo64(m*p) == 0 mod 2^64, carrying 1 iff T_lo != 0
-  u64 t = hi + q + carry + (lo != 0 ? 1 : 0);
+  u64 t00, hi;
+  {
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    t00 = (u64)a0 * b0;
+    const u64 t10 = (u64)a1 * b0 + (t00 >> 32);
+    const u64 t01 = (u64)a0 * b1 + (u32)t10;
+    hi = (u64)a1 * b1 + (t10 >> 32) + (t01 >> 32);
+    t00 = (u64)(u32)t00 | (t01 << 32);
+  }
+  const u32 lo0 = (u32)t00, lo1 = (u32)(t00 >> 32);
+  // m = lo * (c*2^55 - 1) mod 2^64 = ((lo*c) << 55) - lo: the low word is
+  // -lo0, the high word (lo0*c mod 2^9) << 23 - lo1 - borrow
+  const u32 mlo = 0u - lo0;
+  const u32 mhi = (mul_u24(lo0 & 511u, c) << 23) - lo1 - (lo0 != 0u ? 1u : 0u);
+  // m*c as a 72-bit number mc_hi*2^32 + (u32)mc_lo
+  const u64 mc_lo = (u64)mlo * c;
+  const u64 mc_hi = (u64)mhi * c + (mc_lo >> 32);
+  // lo64(m*p) = ((m*c mod 2^9) << 55) + m; its carry shows in the high word
+  const u32 x = mhi + (((u32)mc_lo & 511u) << 23);
+  const u32 carry2 = (x < mhi) ? 1u : 0u;
+  // T_lo + lo64(m*p) == 0 mod 2^64, carrying 1 iff T_lo != 0
+  const u32 nz = ((lo0 | lo1) != 0u) ? 1u : 0u;
+  // t = hi + floor(m*c / 2^9) + carry2 + nz; the small terms join the low
+  // 23 bits of floor(m*c / 2^9) without a carry out of 32 bits
+  const u32 qlo = ((u32)mc_lo >> 9) + carry2 + nz;
+  const u64 t = hi + (mc_hi << 23) + (u64)qlo;
   return (t >= p) ? t - p : t;
+}
+
+// a*w mod p, canonical, by Shoup's method with the precomputed companion
+// wq = floor(w * 2^64 / p) (calc-b-inv and m*/fast2,
+// src/modular-arithmetic.lisp:206-231): q = hi64(a*wq) is exact, so
+// r = a*w - q*p lies in [0, 2p) for any a < 2^64 and one subtraction makes it
+// canonical.  q*p mod 2^64 = q + ((q*c mod 2^9) << 55) for p = c*2^55 + 1.
+HGX_HD u64 shoup_mul(u64 a, u64 w, u64 wq, u64 p, u32 c) {
+  u64 qlo_unused, q;
+  mul64x64(a, wq, qlo_unused, q);
+  (void)qlo_unused;
+  const u32 a0 = (u32)a, a1 = (u32)(a >> 32), w0 = (u32)w, w1 = (u32)(w >> 32);
+  const u64 aw = (u64)a0 * w0 + ((u64)(a0 * w1 + a1 * w0) << 32);
+  const u32 q0 = (u32)q;
+  const u64 qp = q + ((u64)(mul_u24(q0 & 511u, c) << 23) << 32);
+  const u64 r = aw - qp;
+  return (r >= p) ? r - p : r;
 }
 
 HGX_HD u64 mod_add(u64 a, u64 b, u64 p) {

@@ -117,6 +117,39 @@ def mul_batch(pairs) -> list[np.ndarray]:
     return outs
 
 
+class Ticket:
+    """A queued product on the library's request server (hgx_submit).
+    Holds the operand and output arrays until the product completes."""
+
+    def __init__(self, a: np.ndarray, b: np.ndarray, sqr: bool = False):
+        self.a = _u64(a)
+        self.b = self.a if sqr else _u64(b)
+        self.out = np.empty(self.a.size + self.b.size, dtype=np.uint64)
+        d = MulDesc(_ptr(self.a), self.a.size, _ptr(self.b), self.b.size, _ptr(self.out))
+        t = ctypes.c_uint64(0)
+        check(_lib.lib().hgx_submit(ctypes.byref(d), ctypes.byref(t)))
+        self.ticket = t.value
+        self._done = False
+
+    def done(self) -> bool:
+        if self._done:
+            return True
+        return check(_lib.lib().hgx_poll(self.ticket)) == 1
+
+    def wait(self) -> np.ndarray:
+        if not self._done:
+            check(_lib.lib().hgx_wait(self.ticket))
+            self._done = True
+        return self.out
+
+
+def submit(a, b=None) -> Ticket:
+    """Queue a*b (or a^2 when b is None) on the request server."""
+    if b is None:
+        return Ticket(a, a, sqr=True)
+    return Ticket(a, b)
+
+
 # ------------------------------------------------------------------ MPZ
 
 def mpz_mul_ntt(x: MPZ, y: MPZ) -> MPZ:
@@ -205,5 +238,5 @@ __all__ = [
     "MPZ", "HgxError", "NTT_MULTIPLY_THRESHOLD", "integer_mpz", "mpz_integer", "mpz_size",
     "mpz_mul", "mpz_mul_ntt", "mpz_square", "mpz_expt", "mul_limbs", "sqr_limbs", "mul_batch",
     "ntt_forward", "ntt_reverse", "moduli", "product_length", "mul_device", "fill_splitmix64",
-    "trim",
+    "trim", "submit", "Ticket",
 ]

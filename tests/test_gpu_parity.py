@@ -165,3 +165,43 @@ def test_mpz_mirror(hgx):
         assert hgx.mpz_integer(hgx.mpz_mul(za, za)) == a * a
         assert hgx.mpz_integer(hgx.mpz_square(za)) == a * a
     assert hgx.mpz_integer(hgx.mpz_expt(hgx.integer_mpz(10), 9001)) == 10**9001
+
+
+def test_batch_device_grouping_many_lengths(hgx, oracle):
+    # 300 products spanning every pass structure, batched in one call
+    rng = np.random.default_rng(0x48475804)
+    pairs = []
+    for i in range(300):
+        na = int(2 ** rng.uniform(0, 15.5))
+        nb = int(2 ** rng.uniform(0, 15.5))
+        a = oracle.splitmix64(7 * i + 1, max(na, 1))
+        pairs.append((a, a if i % 5 == 0 else oracle.splitmix64(7 * i + 2, max(nb, 1))))
+    outs = hgx.mul_batch(pairs)
+    for (a, b), o in zip(pairs, outs):
+        assert oracle.limbs_to_int(o) == oracle.limbs_to_int(a) * oracle.limbs_to_int(b)
+
+
+def test_server_results_alone_and_in_the_mix(hgx, oracle):
+    rng = np.random.default_rng(0x48475805)
+    reqs = []
+    for i in range(60):
+        na = int(2 ** rng.uniform(4, 17))
+        nb = int(2 ** rng.uniform(4, 17))
+        a = oracle.splitmix64(11 * i + 1, na)
+        b = None if i % 6 == 0 else oracle.splitmix64(11 * i + 2, nb)
+        reqs.append((a, b))
+    # alone: one request in flight at a time
+    alone = [hgx.submit(a, b).wait().copy() for a, b in reqs]
+    # in the mix: all submitted before any is waited on
+    tickets = [hgx.submit(a, b) for a, b in reqs]
+    mixed = [t.wait() for t in tickets]
+    for (a, b), x, y in zip(reqs, alone, mixed):
+        assert np.array_equal(x, y)
+        bb = a if b is None else b
+        assert oracle.limbs_to_int(x) == oracle.limbs_to_int(a) * oracle.limbs_to_int(bb)
+
+
+def test_server_rejects_empty_operand(hgx):
+    from hypergeometrica_amd import HgxError
+    with pytest.raises(HgxError):
+        hgx.submit(np.zeros(0, dtype=np.uint64), np.ones(3, dtype=np.uint64))
