@@ -1,0 +1,202 @@
+"""Hypergeometric series by binary splitting, one tree level per batch.
+
+Mirrors src/binary-splitting.lisp: a SERIES is four integer functions
+a, b, p, q (:19-34); a PARTIAL over [lower, upper) holds P, Q, B, R (:68-89);
+the base case (:100-110) and COMBINE (:112-121) are the reference's.  The
+reference recurses depth first (binary-split, :123-133); here the same tree
+(split at floor((lower + upper) / 2)) is evaluated bottom-up, one height at a
+time, so that every product of a height goes to the GPU in one batched call
+(hgx_mul_batch, grouped by transform length).  All arithmetic is exact, so
+every partial equals the reference's.
+
+Each COMBINE needs seven products in two dependent rounds:
+  round 1: P_l P_r, Q_l Q_r, B_l B_r, Q_r R_l, P_l R_r
+  round 2: B_r (Q_r R_l), B_l (P_l R_r)
+then R = B_r Q_r R_l + B_l P_l R_r (an addition, done on the host).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from math import isqrt
+from typing import Callable
+
+import numpy as np
+
+from . import mpz as _mpz
+
+
+@dataclass(frozen=True)
+class Series:
+    """sum_k a(k)/b(k) * p(0)...p(k) / (q(0)...q(k)) (src/binary-splitting.lisp:19-34)."""
+    a: Callable[[int], int] = lambda n: 1
+    b: Callable[[int], int] = lambda n: 1
+    p: Callable[[int], int] = lambda n: 1
+    q: Callable[[int], int] = lambda n: 1
+
+
+@dataclass
+class Partial:
+    """PARTIAL (src/binary-splitting.lisp:68-89) over [lower, upper)."""
+    lower: int
+    upper: int
+    p: int
+    q: int
+    b: int
+    r: int
+
+    def numerator(self) -> int:
+        return self.r
+
+    def denominator(self, mul=None) -> int:
+        """partial-denominator: B * Q (src/binary-splitting.lisp:87-89)."""
+        return (mul or _mul_one)(self.b, self.q)
+
+
+def _limbs(x: int) -> np.ndarray:
+    m = abs(x)
+    n = max(1, (m.bit_length() + 63) // 64)
+    return np.frombuffer(m.to_bytes(8 * n, "little"), dtype="<u8").astype(np.uint64)
+
+
+def _int(limbs: np.ndarray) -> int:
+    return int.from_bytes(np.ascontiguousarray(limbs).astype("<u8").tobytes(), "little")
+
+
+def mul_many(pairs: list[tuple[int, int]]) -> list[int]:
+    """Exact products of Python integers, all in one GPU batch."""
+    out = [0] * len(pairs)
+    idx, batch = [], []
+    for i, (x, y) in enumerate(pairs):
+        if x == 0 or y == 0:
+            continue
+        if x is y or x == y:
+            a = _limbs(x)
+            batch.append((a, a))
+        else:
+            batch.append((_limbs(x), _limbs(y)))
+        idx.append(i)
+    if batch:
+        prods = _mpz.mul_batch(batch)
+        for i, pr in zip(idx, prods):
+            x, y = pairs[i]
+            s = -1 if (x < 0) != (y < 0) else 1
+            out[i] = s * _int(pr)
+    return out
+
+
+def _mul_one(x: int, y: int) -> int:
+    return mul_many([(x, y)])[0]
+
+
+def base_case(series: Series, lower: int) -> Partial:
+    """binary-split-base-case=1 (src/binary-splitting.lisp:100-110); its one
+    product p * a(lower) is a small integer product."""
+    p = series.p(lower)
+    return Partial(lower, lower + 1, p, series.q(lower), series.b(lower), p * series.a(lower))
+
+
+def _tree(lower: int, upper: int, depth: int, nodes: list):
+    """The reference's split tree; returns the node's height."""
+    if upper - lower == 1:
+        return 0
+    m = (lower + upper) // 2
+    h = 1 + max(_tree(lower, m, depth + 1, nodes), _tree(m, upper, depth + 1, nodes))
+    nodes.append((h, lower, m, upper))
+    return h
+
+
+def binary_split(series: Series, lower: int, upper: int, mul_batch=None) -> Partial:
+    """binary-split (src/binary-splitting.lisp:123-133), evaluated one tree
+    height at a time with batched products."""
+    assert upper > lower
+    mul = mul_batch or mul_many
+    parts: dict[tuple[int, int], Partial] = {(k, k + 1): base_case(series, k) for k in range(lower, upper)}
+    nodes: list = []
+    _tree(lower, upper, 0, nodes)
+    heights: dict[int, list] = {}
+    for h, lo, m, up in nodes:
+        heights.setdefault(h, []).append((lo, m, up))
+    for h in sorted(heights):
+        level = heights[h]
+        r1 = []
+        for lo, m, up in level:
+            L, R = parts[(lo, m)], parts[(m, up)]
+            r1 += [(L.p, R.p), (L.q, R.q), (L.b, R.b), (R.q, L.r), (L.p, R.r)]
+        o1 = mul(r1)
+        r2 = []
+        for i, (lo, m, up) in enumerate(level):
+            L, R = parts[(lo, m)], parts[(m, up)]
+            r2 += [(R.b, o1[5 * i + 3]), (L.b, o1[5 * i + 4])]
+        o2 = mul(r2)
+        for i, (lo, m, up) in enumerate(level):
+            parts[(lo, up)] = Partial(lo, up, o1[5 * i], o1[5 * i + 1], o1[5 * i + 2], o2[2 * i] + o2[2 * i + 1])
+            del parts[(lo, m)], parts[(m, up)]
+    return parts[(lower, upper)]
+
+
+# ------------------------------------------------------------------ series
+
+def e_series() -> Series:
+    """make-e-series (src/binary-splitting.lisp:150-154)."""
+    return Series(q=lambda n: 1 if n == 0 else n)
+
+
+def exp_series(num: int, den: int) -> Series:
+    """make-exp-series for x = num/den (src/binary-splitting.lisp:140-148)."""
+    return Series(p=lambda n: 1 if n == 0 else num, q=lambda n: 1 if n == 0 else n * den)
+
+
+RAMA_A, RAMA_B, RAMA_C = 1103, 26390, 396
+
+
+def ramanujan_series() -> Series:
+    """make-ramanujan-series (src/binary-splitting.lisp:167-184)."""
+    return Series(a=lambda n: RAMA_A + n * RAMA_B,
+                  p=lambda n: 1 if n == 0 else -3 + n * (22 + n * (-48 + n * 32)),
+                  q=lambda n: 1 if n == 0 else n ** 3 * (RAMA_C ** 4 // 8))
+
+
+CHUD_A, CHUD_B, CHUD_C = 13591409, 545140134, 640320
+CHUD_PREFACTOR = 10  # computes pi/10 (src/pi.lisp:14-20)
+
+
+def chudnovsky_series() -> Series:
+    """make-chudnovsky-series (src/pi.lisp:24-43)."""
+    return Series(a=lambda n: CHUD_A * CHUD_PREFACTOR + n * CHUD_B * CHUD_PREFACTOR,
+                  b=lambda n: (8 * CHUD_C) // 12,
+                  p=lambda n: 1 if n == 0 else 5 + n * (-46 + n * (108 + n * -72)),
+                  q=lambda n: 1 if n == 0 else n ** 3 * (CHUD_C ** 3 // 24))
+
+
+def catalan_series() -> Series:
+    """make-catalan-series (src/binary-splitting.lisp:203-213)."""
+    return Series(b=lambda n: 1 + 2 * n, p=lambda n: 1 if n == 0 else n, q=lambda n: 1 if n == 0 else 2 + 4 * n)
+
+
+def apery_series() -> Series:
+    """make-apery-series (src/binary-splitting.lisp:218-231)."""
+    return Series(a=lambda n: 77 + n * (250 + n * 205), b=lambda n: 2,
+                  p=lambda n: 1 if n == 0 else -(n ** 5), q=lambda n: 32 * (1 + 2 * n) ** 5)
+
+
+CHUD_DECIMALS_PER_TERM = 14.181647462725477  # log10(151931373056000), src/pi.lisp:10
+
+
+def compute_pi_chudnovsky(prec: int) -> int:
+    """compute-pi/chudnovsky (src/pi.lisp:45-51): floor(10^prec * pi / 10)
+    with the series summed by batched binary splitting and the final
+    division on the host."""
+    num_terms = int(2 + prec / CHUD_DECIMALS_PER_TERM)
+    sqrt_c = isqrt(10005 * 100 ** prec)
+    comp = binary_split(chudnovsky_series(), 0, num_terms)
+    den = comp.denominator()
+    return (sqrt_c * den) // comp.numerator()
+
+
+def partial_digits(x: Partial, digits: int, mul=None) -> int:
+    """partial-digits (src/binary-splitting.lisp:92-96)."""
+    num, den = 10 ** digits * x.numerator(), x.denominator(mul)
+    q, r = divmod(num, den)
+    if 2 * r > den or (2 * r == den and q % 2 == 1):  # CL ROUND: nearest, ties to even
+        q += 1
+    return q
