@@ -47,7 +47,7 @@ def kernel_bytes(name: str, L: int, na: int, nb: int, nops: int) -> float | None
         return 8 * K * L * 2
     if name == "inv_col":
         return 16 * K * L
-    if name == "inv_col_crt":
+    if name == "crt":
         return 8 * K * L + 8 * (na + nb) + 20 * runs
     return None
 
@@ -145,6 +145,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-bits", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first run one plain device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); products are independent per GPU "
@@ -166,6 +168,14 @@ def main() -> int:
     h.fill_splitmix64(A.data_ptr(), na, 0x48475801)
     h.fill_splitmix64(B.data_ptr(), nb, 0x48475802)
     torch.cuda.synchronize()
+    if args.calib_copy:
+        words = args.calib_copy << 17
+        src = torch.ones(words, dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# calibration copy: {words * 8} bytes read, {words * 8} bytes written", file=sys.stderr, flush=True)
+        del src, dst
 
     def step():
         _lib.check(L_.hgx_mul_device(A.data_ptr(), na, B.data_ptr(), nb, OUT.data_ptr(), None))

@@ -2,6 +2,97 @@
 #include "hgx_kernels.hpp"
 #include "hgx_launch.hpp"
 
+// Column pass, inverse.  Twiddles w_B^-(t q) (times the final scale on the
+// outermost pass) are applied at load, then the 256-point DIT, in place on
+// the residue array of prime kbase + y.
+// grid = (jobs << jt_bits, nprimes)
+__global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
+                                                      int kbase) {
+  __shared__ u64 lds[HGX_TILE_LDS];
+  const int tid = threadIdx.x;
+  const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
+  const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
+  const int tpb_bits = bbits - 12;
+  const u64 blk = tile >> tpb_bits;
+  const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
+  const u64 rs = 1ull << (bbits - 8);
+  const u64 gbase = (blk << bbits) + t0;
+  const int k = kbase + blockIdx.y;
+  const u64 p = ps.p[k];
+  const u32 c = ps.c[k];
+  u64 *arr = J.res0 + (u64)blockIdx.y * L;
+#pragma unroll
+  for (int j = 0; j < HGX_EPT; j++) {
+    const int x = tid + HGX_NT * j;
+    const u32 m = x >> 4;
+    const u64 t = t0 + (x & 15);
+    const u64 e = t * bitrev8(m);
+    lds[HGX_PAD(x)] = apply_twiddle2(arr[gbase + (x & 15) + rs * m], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
+  }
+  __syncthreads();
+  tile_inverse<8, true>(lds, 4, ps.tw_inv[k], p, c);
+#pragma unroll
+  for (int j = 0; j < HGX_EPT; j++) {
+    const int x = tid + HGX_NT * j;
+    arr[gbase + (x & 15) + rs * (u64)(x >> 4)] = lds[HGX_PAD(x)];
+  }
+}
+
+// CRT and carry sums.  A workgroup takes min(L, 2048) consecutive
+// coefficients: every thread reconstructs coefficients from the three
+// residue arrays (coalesced loads, Garner form), the 192-bit values go to
+// LDS, then one thread per run of 16 adds its run exactly (add-big-digit,
+// src/ntt-multiply.lisp:10-51, adds the same values at the same positions),
+// recording the digits, the 2-word overflow past the run and whether limbs
+// 2..15 are all ones.  grid = jobs << bpj_bits
+#define HGX_CRT_BLOCK 2048
+__global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_bits, u64 L) {
+  __shared__ u64 w0[HGX_CRT_BLOCK + HGX_CRT_BLOCK / 16];
+  __shared__ u64 w1[HGX_CRT_BLOCK + HGX_CRT_BLOCK / 16];
+  __shared__ u64 w2[HGX_CRT_BLOCK + HGX_CRT_BLOCK / 16];
+  const int tid = threadIdx.x;
+  const MulJob J = load_job(jr, blockIdx.x >> bpj_bits);
+  const u64 blk = blockIdx.x & ((1u << bpj_bits) - 1);
+  const int P = L < HGX_CRT_BLOCK ? (int)L : HGX_CRT_BLOCK;
+  const int rl = P < HGX_RUN ? P : HGX_RUN;
+  const u64 base = blk * HGX_CRT_BLOCK;
+  for (int x = tid; x < P; x += HGX_NT) {
+    const u64 pos = base + x;
+    const U192 cf = garner3(J.res0[pos], J.res0[L + pos], J.res0[2 * L + pos], ps);
+    w0[HGX_PAD(x)] = cf.w0;
+    w1[HGX_PAD(x)] = cf.w1;
+    w2[HGX_PAD(x)] = cf.w2;
+  }
+  __syncthreads();
+  const int nruns = P / rl;
+  if (tid < nruns) {
+    u64 a0 = 0, a1 = 0, andhi = ~0ull;
+    for (int j = 0; j < rl; j++) {
+      const int x = tid * rl + j;
+      const u64 c0 = w0[HGX_PAD(x)], c1 = w1[HGX_PAD(x)], c2 = w2[HGX_PAD(x)];
+      const u64 s0 = a0 + c0;
+      const u64 cy = (s0 < a0) ? 1 : 0;
+      const u64 s1 = a1 + c1;
+      u64 cy1 = (s1 < a1) ? 1 : 0;
+      const u64 s1b = s1 + cy;
+      cy1 += (s1b < s1) ? 1 : 0;
+      w0[HGX_PAD(x)] = s0;
+      if (j >= 2) andhi &= s0;
+      a0 = s1b;
+      a1 = c2 + cy1;
+    }
+    const u64 run_g = base / rl + tid;
+    J.ovf[2 * run_g] = a0;
+    J.ovf[2 * run_g + 1] = a1;
+    J.flags[run_g] = (andhi == ~0ull) ? 1u : 0u;
+  }
+  __syncthreads();
+  for (int x = tid; x < P; x += HGX_NT) {
+    const u64 pos = base + x;
+    if (pos < J.nout) J.out[pos] = w0[HGX_PAD(x)];
+  }
+}
+
 // per-run maps and each block's combined map.
 // grid = jobs << bpj_bits (bpj = blocks of 1024 runs per job)
 __global__ __launch_bounds__(HGX_NT) void k_carry_map(JobRef jr, int bpj_bits, u64 nruns, int rl) {
@@ -111,12 +202,13 @@ void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps
     hipLaunchKernelGGL((k_fwd_col<false>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
 }
 
-void hgx_launch_inv_col(bool last, dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr,
-                        int jt_bits, u64 L, int bbits, int kbase) {
-  if (last)
-    hipLaunchKernelGGL((k_inv_col<true>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
-  else
-    hipLaunchKernelGGL((k_inv_col<false>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+void hgx_launch_inv_col(dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr, int jt_bits,
+                        u64 L, int bbits, int kbase) {
+  hipLaunchKernelGGL(k_inv_col, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+}
+
+void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &jr, int bpj_bits, u64 L) {
+  hipLaunchKernelGGL(k_crt, grid, dim3(HGX_NT), 0, s, ps, jr, bpj_bits, L);
 }
 
 void hgx_launch_carry_map(dim3 grid, hipStream_t s, const JobRef &jr, int bpj_bits, u64 nruns, int rl) {

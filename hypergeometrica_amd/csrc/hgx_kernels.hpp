@@ -26,12 +26,12 @@
 //   k_fwd_col         further column passes (in place)
 //   k_row<MUL|SQR>    last forward pass of both operands, pointwise product,
 //                     first inverse pass, per prime
-//   k_inv_col         inverse column passes (in place)
-//   k_inv_col<LAST>   outermost inverse pass for all three primes, CRT
-//                     (Garner form) of every coefficient and the carry sum
-//                     of every 16-limb run, with each run's overflow saved
+//   k_inv_col         inverse column passes (in place); the outermost one
+//                     folds in the 1/L scale
+//   k_crt             CRT (Garner form) of every coefficient and the exact
+//                     sum of every 16-limb run, with each run's overflow
 //   k_carry_*         resolves the run-to-run carries (block scan)
-//   For L <= 4096 one k_row<..., SINGLE> does the whole transform work.
+//   For L <= 4096 one k_row<..., SINGLE> does all the transform work.
 #pragma once
 #include "hgx_types.hpp"
 
@@ -215,37 +215,6 @@ __device__ __forceinline__ U192 garner3(u64 r0, u64 r1, u64 r2, const PrimeSet &
 }
 
 // ---------------------------------------------------------------------------
-// Carry sum of one run of up to 16 coefficients given their three residues:
-// writes the run's digits into LDS (run-major) and returns the overflow past
-// the run.  add-big-digit, src/ntt-multiply.lisp:10-51, adds the same
-// coefficients at the same positions.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void run_crt_sum(const u64 (&r)[3][HGX_RUN], int rl, const PrimeSet &ps, u64 *lds,
-                                            int run, u64 &ov0, u64 &ov1, u32 &allones) {
-  u64 a0 = 0, a1 = 0;
-  u64 andhi = ~0ull;
-#pragma unroll
-  for (int j = 0; j < HGX_RUN; j++) {
-    if (j < rl) {
-      const U192 cf = garner3(r[0][j], r[1][j], r[2][j], ps);
-      const u64 s0 = a0 + cf.w0;
-      const u64 cy = (s0 < a0) ? 1 : 0;
-      const u64 s1 = a1 + cf.w1;
-      u64 cy1 = (s1 < a1) ? 1 : 0;
-      const u64 s1b = s1 + cy;
-      cy1 += (s1b < s1) ? 1 : 0;
-      lds[HGX_PAD(run * rl + j)] = s0;
-      if (j >= 2) andhi &= s0;
-      a0 = s1b;
-      a1 = cf.w2 + cy1;
-    }
-  }
-  ov0 = a0;
-  ov1 = a1;
-  allones = (andhi == ~0ull) ? 1u : 0u;
-}
-
-// ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
 
@@ -324,90 +293,14 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
   }
 }
 
-// Column pass, inverse.  Twiddles w_B^-(t q) (times the final scale on the
-// outermost pass) are applied at load, then the 256-point DIT.  LAST: all
-// three primes are transformed in turn, each 16-column row of the tile is a
-// run of 16 consecutive coefficients, and the CRT + carry sum is fused in.
-// grid = (jobs << jt_bits, LAST ? 1 : nprimes)
-template <bool LAST>
-__global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
-                                                      int kbase) {
-  __shared__ u64 lds[HGX_TILE_LDS];
-  const int tid = threadIdx.x;
-  const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
-  const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
-  const int tpb_bits = bbits - 12;
-  const u64 blk = tile >> tpb_bits;
-  const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
-  const u64 rs = 1ull << (bbits - 8);
-  const u64 gbase = (blk << bbits) + t0;
-
-  if (!LAST) {
-    const int k = kbase + blockIdx.y;
-    const u64 p = ps.p[k];
-    const u32 c = ps.c[k];
-    u64 *arr = J.res0 + (u64)blockIdx.y * L;
-#pragma unroll
-    for (int j = 0; j < HGX_EPT; j++) {
-      const int x = tid + HGX_NT * j;
-      const u32 m = x >> 4;
-      const u64 t = t0 + (x & 15);
-      const u64 e = t * bitrev8(m);
-      lds[HGX_PAD(x)] = apply_twiddle2(arr[gbase + (x & 15) + rs * m], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
-    }
-    __syncthreads();
-    tile_inverse<8, true>(lds, 4, ps.tw_inv[k], p, c);
-#pragma unroll
-    for (int j = 0; j < HGX_EPT; j++) {
-      const int x = tid + HGX_NT * j;
-      arr[gbase + (x & 15) + rs * (u64)(x >> 4)] = lds[HGX_PAD(x)];
-    }
-  } else {
-    u64 r[3][HGX_RUN];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const u64 p = ps.p[k];
-      const u32 c = ps.c[k];
-      const u64 *arr = J.res0 + (u64)k * L;
-#pragma unroll
-      for (int j = 0; j < HGX_EPT; j++) {
-        const int x = tid + HGX_NT * j;
-        const u32 m = x >> 4;
-        const u64 t = t0 + (x & 15);
-        const u64 e = t * bitrev8(m);
-        lds[HGX_PAD(x)] = apply_twiddle2(arr[gbase + (x & 15) + rs * m], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
-      }
-      __syncthreads();
-      tile_inverse<8, true>(lds, 4, ps.tw_inv[k], p, c);
-      // run tid = tile row tid = 16 consecutive coefficients
-#pragma unroll
-      for (int j = 0; j < HGX_RUN; j++) r[k][j] = lds[HGX_PAD(tid * 16 + j)];
-      __syncthreads();
-    }
-    u64 ov0, ov1;
-    u32 allones;
-    run_crt_sum(r, 16, ps, lds, tid, ov0, ov1, allones);
-    const u64 run_g = (gbase + rs * (u64)tid) >> 4;
-    J.ovf[2 * run_g] = ov0;
-    J.ovf[2 * run_g + 1] = ov1;
-    J.flags[run_g] = allones;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < HGX_EPT; j++) {
-      const int x = tid + HGX_NT * j;
-      const u64 pos = gbase + (x & 15) + rs * (u64)(x >> 4);
-      if (pos < J.nout) J.out[pos] = lds[HGX_PAD(x)];
-    }
-  }
-}
-
 // Row pass: contiguous blocks of 2^S_BITS words; a tile is
 // min(L, 4096) consecutive words (2^tw_bits).
 //   MODE_MUL / MODE_SQR, !SINGLE: last forward pass of both operands (or the
 //     one operand), pointwise product (a*b/R by REDC), first inverse pass.
 //   MODE_FWD / MODE_INV: transform-only row pass on res0 (prime kbase + y).
 //   SINGLE (L <= 4096): the limbs come in, all three primes are done in
-//     turn, and the CRT + carry sum goes out (as k_inv_col<true>).
+//     turn, and the three residue arrays of the product go out (k_crt
+//     finishes).
 
 template <int S_BITS, int MODE, bool SINGLE>
 __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRef jr, int jt_bits) {
@@ -447,10 +340,9 @@ __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRe
     if (MODE != MODE_FWD) tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], p, c);
     for (int x = tid; x < TW; x += HGX_NT) A[x] = ldsA[HGX_PAD(x)];
   } else {
-    const int rl = TW < HGX_RUN ? TW : HGX_RUN;
-    const int nruns = TW / rl;
-    u64 r[3][HGX_RUN];
-#pragma unroll
+    // whole transform in one tile (L <= 4096): limbs in, all three primes in
+    // turn, residues of the product out (k_crt finishes)
+#pragma unroll 1
     for (int k = 0; k < 3; k++) {
       const u64 p = ps.p[k];
       const u32 c = ps.c[k];
@@ -469,26 +361,9 @@ __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRe
       }
       __syncthreads();
       tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], p, c);
-      if (tid < nruns) {
-#pragma unroll
-        for (int j = 0; j < HGX_RUN; j++)
-          if (j < rl) r[k][j] = ldsA[HGX_PAD(tid * rl + j)];
-      }
+      u64 *dst = J.res0 + (u64)k * rp.L + gbase;
+      for (int x = tid; x < TW; x += HGX_NT) dst[x] = ldsA[HGX_PAD(x)];
       __syncthreads();
-    }
-    u64 ov0 = 0, ov1 = 0;
-    u32 allones = 0;
-    if (tid < nruns) {
-      run_crt_sum(r, rl, ps, ldsA, tid, ov0, ov1, allones);
-      const u64 run_g = (gbase / rl) + tid;
-      J.ovf[2 * run_g] = ov0;
-      J.ovf[2 * run_g + 1] = ov1;
-      J.flags[run_g] = allones;
-    }
-    __syncthreads();
-    for (int x = tid; x < TW; x += HGX_NT) {
-      const u64 pos = gbase + x;
-      if (pos < J.nout) J.out[pos] = ldsA[HGX_PAD(x)];
     }
   }
 }

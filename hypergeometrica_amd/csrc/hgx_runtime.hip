@@ -330,7 +330,7 @@ WorkLayout work_layout(int n, bool sqr) {
   size_t at = 0;
   const bool single = n <= kRowMaxBits;
   w.res0 = at;
-  at += single ? 0 : align_up(3 * L * sizeof(u64));
+  at += align_up(3 * L * sizeof(u64));
   w.res1 = at;
   at += (single || sqr) ? 0 : align_up(3 * L * sizeof(u64));
   w.ovf = at;
@@ -413,12 +413,16 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     rc = sqr ? launch_row<MODE_SQR, false>(plan->srow, dim3(grid, 3), s, rp, jr, jt)
              : launch_row<MODE_MUL, false>(plan->srow, dim3(grid, 3), s, rp, jr, jt);
     if (rc) return rc;
-    for (int i = plan->ncol - 1; i >= 1; i--) {
+    for (int i = plan->ncol - 1; i >= 0; i--) {
       HGX_TIMED("inv_col", s);
-      hgx_launch_inv_col(false, dim3(grid, 3), s, g_ctx.ps, coltw(plan->col[i], 1), jr, jt, L, plan->col[i].bbits, 0);
+      hgx_launch_inv_col(dim3(grid, 3), s, g_ctx.ps, coltw(plan->col[i], i == 0 ? 2 : 1), jr, jt, L, plan->col[i].bbits,
+                         0);
     }
-    HGX_TIMED("inv_col_crt", s);
-    hgx_launch_inv_col(true, dim3(grid, 1), s, g_ctx.ps, coltw(plan->col[0], 2), jr, jt, L, plan->col[0].bbits, 0);
+  }
+  {
+    const int cb = n > 11 ? n - 11 : 0;  // CRT blocks of 2048 coefficients per job
+    HGX_TIMED("crt", s);
+    hgx_launch_crt(dim3(njobs << cb), s, g_ctx.ps, jr, cb, L);
   }
   const u64 nblk = (nruns + HGX_CARRY_RUNS_PER_BLOCK - 1) / HGX_CARRY_RUNS_PER_BLOCK;
   const int bpj = log2_ceil(nblk);
@@ -475,7 +479,7 @@ int enqueue_transform(u64 *d, size_t nwords, int mod_num, bool inverse, hipStrea
       if (rc) return rc;
       for (int i = plan->ncol - 1; i >= 0; i--) {
         HGX_TIMED("inv_col", s);
-        hgx_launch_inv_col(false, dim3(tiles, 1), s, g_ctx.ps, coltw(plan->col[i], i == 0 ? 3 : 1), jr, jt, L,
+        hgx_launch_inv_col(dim3(tiles, 1), s, g_ctx.ps, coltw(plan->col[i], i == 0 ? 3 : 1), jr, jt, L,
                            plan->col[i].bbits, mod_num);
       }
     }
