@@ -136,6 +136,75 @@ def batched_leg(torch, h, _lib, count: int = 4096, reps: int = 3) -> dict:
             "speedup": round(p / b, 2), "outputs_identical": same}
 
 
+def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 0.5, slots: int = 16) -> dict:
+    """Request server under load: Poisson arrivals at RATE/s for SECONDS,
+    operand sizes log-uniform in [2^10, 2^24] limbs (seed 0x48475804), operands
+    and products in pinned host memory (operands are slices of two random
+    2^24-limb buffers; SLOTS output buffers bound the requests in flight).
+    Latency = completion seen by the client - scheduled arrival."""
+    import math
+
+    import numpy as np
+    from oracle.oracle import splitmix64_py
+    L_ = _lib.lib()
+    count = max(1, int(rate * seconds))
+    sizes = log_uniform_sizes(0x48475804, 2 * count, 10, 24)
+    us = splitmix64_py(0x48475805, count)
+    gaps = [-math.log(1.0 - (u + 0.5) / 2**64) / rate for u in us]
+    nmax = 1 << 24
+    rng = np.random.default_rng(0x48475806)
+    A = torch.empty(nmax, dtype=torch.int64, pin_memory=True)
+    B = torch.empty(nmax, dtype=torch.int64, pin_memory=True)
+    A.numpy()[:] = rng.integers(-2**63, 2**63, size=nmax, dtype=np.int64)
+    B.numpy()[:] = rng.integers(-2**63, 2**63, size=nmax, dtype=np.int64)
+    outs = [torch.empty(2 * nmax, dtype=torch.int64, pin_memory=True) for _ in range(slots)]
+    free = list(range(slots))
+    inflight = {}
+    lat, limbs = [], 0
+    import ctypes
+    t0 = time.perf_counter()
+    sched = t0
+
+    def reap(block: bool):
+        nonlocal limbs
+        while True:
+            done = [t for t in inflight if L_.hgx_poll(ctypes.c_uint64(t)) == 1]
+            now = time.perf_counter()
+            for t in done:
+                arr, slot, n = inflight.pop(t)
+                _lib.check(L_.hgx_wait(ctypes.c_uint64(t)))
+                lat.append(now - arr)
+                limbs += n
+                free.append(slot)
+            if done or not block or not inflight:
+                return
+
+    for i in range(count):
+        sched += gaps[i]
+        na, nb = sizes[2 * i], sizes[2 * i + 1]
+        while time.perf_counter() < sched:
+            reap(False)
+        while not free:
+            reap(True)
+        slot = free.pop()
+        oa = (us[i] >> 8) % (nmax - na + 1)
+        ob = (us[i] >> 32) % (nmax - nb + 1)
+        d = _lib.MulDesc(A.data_ptr() + 8 * oa, na, B.data_ptr() + 8 * ob, nb, outs[slot].data_ptr())
+        t = ctypes.c_uint64(0)
+        _lib.check(L_.hgx_submit(ctypes.byref(d), ctypes.byref(t)))
+        inflight[t.value] = (sched, slot, na + nb)
+    while inflight:
+        reap(True)
+    wall = time.perf_counter() - t0
+    lat.sort()
+    return {"workload": f"Poisson arrivals {rate:.0f}/s for {seconds} s ({count} requests), operand limbs log-uniform "
+                        f"in [2^10, 2^24], pinned host buffers, {slots} output slots",
+            "requests": count, "limbs": limbs, "wall_s": round(wall, 4), "limbs_per_s": limbs / wall,
+            "p50_ms": round(1e3 * lat[len(lat) // 2], 3),
+            "p99_ms": round(1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
+            "mean_request_limbs": limbs / count}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +214,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-bits", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-server leg")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -245,6 +315,11 @@ def main() -> int:
         print("# batched vs per-call ...", file=sys.stderr, flush=True)
         batched = batched_leg(torch, h, _lib)
 
+    serving = None
+    if not args.no_serve:
+        print("# serving ...", file=sys.stderr, flush=True)
+        serving = serving_leg(torch, h, _lib)
+
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
@@ -271,6 +346,7 @@ def main() -> int:
         "latency_ms": {"p50": round(1e3 * p50, 3), "p99": round(1e3 * p99, 3)},
         "kernels": kernels,
         "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

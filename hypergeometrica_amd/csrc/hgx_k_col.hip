@@ -26,8 +26,7 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
     const int x = tid + HGX_NT * j;
     const u32 m = x >> 4;
     const u64 t = t0 + (x & 15);
-    const u64 e = t * bitrev8(m);
-    lds[HGX_PAD(x)] = apply_twiddle2(arr[gbase + (x & 15) + rs * m], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
+    lds[HGX_PAD(x)] = apply_twiddle(arr[gbase + (x & 15) + rs * m], tw, k, t, bitrev8(m), p, c);
   }
   __syncthreads();
   tile_inverse<8, true>(lds, 4, ps.tw_inv[k], p, c);
@@ -38,24 +37,31 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
   }
 }
 
-// CRT and carry sums.  A workgroup takes min(L, 2048) consecutive
+// CRT and carry sums.  A workgroup takes a block of min(L, 2048) consecutive
 // coefficients: every thread reconstructs coefficients from the three
-// residue arrays (coalesced loads, Garner form), the 192-bit values go to
-// LDS, then one thread per run of 16 adds its run exactly (add-big-digit,
-// src/ntt-multiply.lisp:10-51, adds the same values at the same positions),
-// recording the digits, the 2-word overflow past the run and whether limbs
-// 2..15 are all ones.  grid = jobs << bpj_bits
-#define HGX_CRT_BLOCK 2048
+// residue arrays (coalesced loads, Garner form) into LDS; one thread per run
+// of 16 sums its run exactly (digits + a 2-word overflow); the overflows
+// are then carried from run to run inside the block (kill / propagate /
+// generate scan over the 128 runs), so the block leaves exact digits plus
+// one 2-word overflow and an all-ones flag for limbs 2.. of the block.
+// add-big-digit (src/ntt-multiply.lisp:10-51) adds the same values at the
+// same positions.  grid = jobs << bpj_bits
 __global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_bits, u64 L) {
-  __shared__ u64 w0[HGX_CRT_BLOCK + HGX_CRT_BLOCK / 16];
-  __shared__ u64 w1[HGX_CRT_BLOCK + HGX_CRT_BLOCK / 16];
-  __shared__ u64 w2[HGX_CRT_BLOCK + HGX_CRT_BLOCK / 16];
+  constexpr int PADN = HGX_CRT_BLOCK + HGX_CRT_BLOCK / 16;
+  __shared__ u64 w0[PADN];
+  __shared__ u64 w1[PADN];
+  __shared__ u64 w2[PADN];
+  __shared__ u64 ovs[2 * (HGX_CRT_BLOCK / HGX_RUN)];
+  __shared__ u32 scan[HGX_NT];
+  __shared__ u32 allones_blk;
   const int tid = threadIdx.x;
   const MulJob J = load_job(jr, blockIdx.x >> bpj_bits);
   const u64 blk = blockIdx.x & ((1u << bpj_bits) - 1);
   const int P = L < HGX_CRT_BLOCK ? (int)L : HGX_CRT_BLOCK;
   const int rl = P < HGX_RUN ? P : HGX_RUN;
+  const int nruns = P / rl;
   const u64 base = blk * HGX_CRT_BLOCK;
+  if (tid == 0) allones_blk = 1u;
   for (int x = tid; x < P; x += HGX_NT) {
     const u64 pos = base + x;
     const U192 cf = garner3(J.res0[pos], J.res0[L + pos], J.res0[2 * L + pos], ps);
@@ -64,7 +70,8 @@ __global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_
     w2[HGX_PAD(x)] = cf.w2;
   }
   __syncthreads();
-  const int nruns = P / rl;
+  // exact sum of each run
+  u32 hi_ones = 0u;
   if (tid < nruns) {
     u64 a0 = 0, a1 = 0, andhi = ~0ull;
     for (int j = 0; j < rl; j++) {
@@ -81,16 +88,69 @@ __global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_
       a0 = s1b;
       a1 = c2 + cy1;
     }
-    const u64 run_g = base / rl + tid;
-    J.ovf[2 * run_g] = a0;
-    J.ovf[2 * run_g + 1] = a1;
-    J.flags[run_g] = (andhi == ~0ull) ? 1u : 0u;
+    ovs[2 * tid] = a0;
+    ovs[2 * tid + 1] = a1;
+    hi_ones = (andhi == ~0ull) ? 1u : 0u;
   }
   __syncthreads();
+  // carry maps of the runs for the overflow of the previous run
+  u32 m = 2u;
+  u64 x0 = 0, x1 = 0;
+  if (tid < nruns && tid > 0) {
+    x0 = ovs[2 * (tid - 1)];
+    x1 = ovs[2 * (tid - 1) + 1];
+    const u64 d0 = w0[HGX_PAD(tid * rl)];
+    const u64 d1 = rl > 1 ? w0[HGX_PAD(tid * rl + 1)] : 0;
+    const u64 s0 = d0 + x0;
+    const u64 cy = (s0 < d0) ? 1 : 0;
+    const u64 s1 = d1 + x1;
+    u64 c1 = (s1 < d1) ? 1 : 0;
+    const u64 s1b = s1 + cy;
+    c1 += (s1b < s1) ? 1 : 0;
+    const u32 f0 = c1 ? 1u : 0u;
+    const u32 f1 = (c1 || (s0 == ~0ull && s1b == ~0ull)) ? 1u : 0u;
+    m = hi_ones ? (f0 | (f1 << 1)) : 0u;
+  } else if (tid == 0 && nruns > 0) {
+    m = 0u;  // the block's first run: incoming carries are added by k_carry_apply
+  }
+  const u32 ex = block_exclusive_last(m, scan);
+  u32 e = (ex == 3u) ? 1u : 0u;  // carry bit into this run from the run before
+  if (tid < nruns && tid > 0 && (x0 | x1 | e)) {
+    u64 carry = e;
+    for (int j = 0; j < rl; j++) {
+      const int x = tid * rl + j;
+      const u64 add = (j == 0) ? x0 : (j == 1 ? x1 : 0);
+      if (j >= 2 && carry == 0) break;
+      const u64 d = w0[HGX_PAD(x)];
+      const u64 s = d + add;
+      u64 cy = (s < d) ? 1 : 0;
+      const u64 s2 = s + carry;
+      cy += (s2 < s) ? 1 : 0;
+      w0[HGX_PAD(x)] = s2;
+      carry = cy;
+    }
+  }
+  // the block's overflow: the last run's overflow plus the carry out of it
+  if (tid == nruns - 1) {
+    const u32 eo = (m >> e) & 1u;
+    u64 o0 = ovs[2 * tid], o1 = ovs[2 * tid + 1];
+    const u64 n0 = o0 + eo;
+    o1 += (n0 < o0) ? 1 : 0;
+    J.ovf[2 * blk] = n0;
+    J.ovf[2 * blk + 1] = o1;
+  }
+  __syncthreads();
+  // digits out; limbs 2.. of the block all ones?
+  u64 andall = ~0ull;
   for (int x = tid; x < P; x += HGX_NT) {
     const u64 pos = base + x;
-    if (pos < J.nout) J.out[pos] = w0[HGX_PAD(x)];
+    const u64 d = w0[HGX_PAD(x)];
+    if (x >= 2) andall &= d;
+    if (pos < J.nout) J.out[pos] = d;
   }
+  if (andall != ~0ull) allones_blk = 0u;
+  __syncthreads();
+  if (tid == 0) J.flags[blk] = allones_blk;
 }
 
 // per-run maps and each block's combined map.

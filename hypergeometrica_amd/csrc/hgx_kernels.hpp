@@ -37,10 +37,16 @@
 
 __device__ __forceinline__ u32 bitrev8(u32 m) { return __builtin_bitreverse32(m) >> 24; }
 
-// x * w_B^e from the two-level table: two Shoup products.
-__device__ __forceinline__ u64 apply_twiddle2(u64 x, const u64 *TH, const u64 *TL, int lb, u64 e, u64 p, u32 c) {
-  const ulonglong2 lo = *(const ulonglong2 *)(TL + 2 * (e & ((1ull << lb) - 1)));
-  const ulonglong2 hi = *(const ulonglong2 *)(TH + 2 * (e >> lb));
+// x * w_B^(t q) for column t, frequency q: one Shoup product from the direct
+// table, or two from the two-level table.
+__device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u64 t, u32 q, u64 p, u32 c) {
+  if (tw.D[k]) {
+    const ulonglong2 w = *(const ulonglong2 *)(tw.D[k] + 2 * (((u64)q << tw.tbits) + t));
+    return shoup_mul(x, w.x, w.y, p, c);
+  }
+  const u64 e = t * q;
+  const ulonglong2 lo = *(const ulonglong2 *)(tw.TL[k] + 2 * (e & ((1ull << tw.lb) - 1)));
+  const ulonglong2 hi = *(const ulonglong2 *)(tw.TH[k] + 2 * (e >> tw.lb));
   return shoup_mul(shoup_mul(x, lo.x, lo.y, p, c), hi.x, hi.y, p, c);
 }
 
@@ -265,8 +271,7 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
         const int x = tid + HGX_NT * j;
         const u32 m = x >> 4;
         const u64 t = t0 + (x & 15);
-        const u64 e = t * bitrev8(m);
-        dst[gbase + (x & 15) + rs * m] = apply_twiddle2(lds[HGX_PAD(x)], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
+        dst[gbase + (x & 15) + rs * m] = apply_twiddle(lds[HGX_PAD(x)], tw, k, t, bitrev8(m), p, c);
       }
       __syncthreads();
     }
@@ -287,8 +292,7 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
       const int x = tid + HGX_NT * j;
       const u32 m = x >> 4;
       const u64 t = t0 + (x & 15);
-      const u64 e = t * bitrev8(m);
-      arr[gbase + (x & 15) + rs * m] = apply_twiddle2(lds[HGX_PAD(x)], tw.TH[k], tw.TL[k], tw.lb, e, p, c);
+      arr[gbase + (x & 15) + rs * m] = apply_twiddle(lds[HGX_PAD(x)], tw, k, t, bitrev8(m), p, c);
     }
   }
 }

@@ -98,7 +98,11 @@ const u64 kModuli[3] = {0x6280000000000001ull, 0x5700000000000001ull, 0x41800000
 struct ColPassTables {
   int bbits = 0, lb = 0;
   u64 *TLf[3], *THf[3], *TLi[3], *THi[3], *THi_mul[3], *THi_tr[3];
+  bool direct = false;  // B <= 2^kDirectMaxBits: direct (q, t) tables below
+  u64 *Df[3], *Di[3], *Di_mul[3], *Di_tr[3];
 };
+
+constexpr int kDirectMaxBits = 20;
 
 struct Plan {
   int n = 0, ncol = 0, srow = 0;
@@ -219,13 +223,15 @@ int build_plan(int n, Plan **out) {
   plan->srow = rem;
   const u64 L = 1ull << n;
   std::vector<u64> host;
-  struct Off { size_t TLf, THf, TLi, THi, THm, THt; };
+  struct Off { size_t TLf, THf, TLi, THi, THm, THt, Df, Di, Dm, Dt; };
   std::vector<Off> offs(plan->ncol * 3);
   for (int i = 0; i < plan->ncol; i++) {
     const int b = n - 8 * i;
     const int lb = b / 2, hb = b - lb;
+    const bool direct = b <= kDirectMaxBits;
     plan->col[i].bbits = b;
     plan->col[i].lb = lb;
+    plan->col[i].direct = direct;
     for (int k = 0; k < 3; k++) {
       const u64 p = kModuli[k];
       const u64 w = powmod(g_ctx.gen[k], (p - 1) >> b, p);
@@ -236,6 +242,7 @@ int build_plan(int n, Plan **out) {
       const u64 kmul = mulmod(Rm, Linv, p);  // R / L: undoes the pointwise REDC's 1/R and scales by 1/L
       const u64 ktr = Linv;                   // 1 / L
       Off &o = offs[i * 3 + k];
+      o = Off{};
       // (value, Shoup companion floor(value * 2^64 / p)) pairs
       auto push_pow = [&](u64 base, size_t cnt, u64 scale) {
         size_t at = host.size();
@@ -247,12 +254,38 @@ int build_plan(int n, Plan **out) {
         }
         return at;
       };
-      o.TLf = push_pow(w, 1ull << lb, 1);
-      o.THf = push_pow(wl, 1ull << hb, 1);
-      o.TLi = push_pow(wi, 1ull << lb, 1);
-      o.THi = push_pow(wil, 1ull << hb, 1);
-      o.THm = (i == 0) ? push_pow(wil, 1ull << hb, kmul) : 0;
-      o.THt = (i == 0) ? push_pow(wil, 1ull << hb, ktr) : 0;
+      // direct table: row q holds (root^q)^t for t < T, times SCALE
+      auto push_direct = [&](u64 root, u64 scale) {
+        size_t at = host.size();
+        const size_t T = (size_t)1 << (b - 8);
+        u64 rq = 1;
+        for (int q = 0; q < 256; q++) {
+          u64 x = scale % p;
+          for (size_t t = 0; t < T; t++) {
+            host.push_back(x);
+            host.push_back(shoup_companion(x, p));
+            x = mulmod(x, rq, p);
+          }
+          rq = mulmod(rq, root, p);
+        }
+        return at;
+      };
+      if (direct) {
+        o.Df = push_direct(w, 1);
+        if (i == 0) {
+          o.Dm = push_direct(wi, kmul);
+          o.Dt = push_direct(wi, ktr);
+        } else {
+          o.Di = push_direct(wi, 1);
+        }
+      } else {
+        o.TLf = push_pow(w, 1ull << lb, 1);
+        o.THf = push_pow(wl, 1ull << hb, 1);
+        o.TLi = push_pow(wi, 1ull << lb, 1);
+        o.THi = push_pow(wil, 1ull << hb, 1);
+        o.THm = (i == 0) ? push_pow(wil, 1ull << hb, kmul) : 0;
+        o.THt = (i == 0) ? push_pow(wil, 1ull << hb, ktr) : 0;
+      }
     }
   }
   if (!host.empty()) {
@@ -263,12 +296,20 @@ int build_plan(int n, Plan **out) {
     for (int i = 0; i < plan->ncol; i++)
       for (int k = 0; k < 3; k++) {
         const Off &o = offs[i * 3 + k];
-        plan->col[i].TLf[k] = plan->dev + o.TLf;
-        plan->col[i].THf[k] = plan->dev + o.THf;
-        plan->col[i].TLi[k] = plan->dev + o.TLi;
-        plan->col[i].THi[k] = plan->dev + o.THi;
-        plan->col[i].THi_mul[k] = plan->dev + o.THm;
-        plan->col[i].THi_tr[k] = plan->dev + o.THt;
+        ColPassTables &c = plan->col[i];
+        if (c.direct) {
+          c.Df[k] = plan->dev + o.Df;
+          c.Di[k] = (i == 0) ? nullptr : plan->dev + o.Di;
+          c.Di_mul[k] = (i == 0) ? plan->dev + o.Dm : nullptr;
+          c.Di_tr[k] = (i == 0) ? plan->dev + o.Dt : nullptr;
+        } else {
+          c.TLf[k] = plan->dev + o.TLf;
+          c.THf[k] = plan->dev + o.THf;
+          c.TLi[k] = plan->dev + o.TLi;
+          c.THi[k] = plan->dev + o.THi;
+          c.THi_mul[k] = plan->dev + o.THm;
+          c.THi_tr[k] = plan->dev + o.THt;
+        }
       }
   }
   *out = plan.get();
@@ -280,9 +321,15 @@ ColTw coltw(const ColPassTables &c, int which) {
   // which: 0 forward, 1 inverse, 2 inverse with multiply scale, 3 inverse with 1/L
   ColTw t{};
   t.lb = c.lb;
+  t.tbits = c.bbits - 8;
   for (int k = 0; k < 3; k++) {
-    t.TL[k] = (which == 0) ? c.TLf[k] : c.TLi[k];
-    t.TH[k] = (which == 0) ? c.THf[k] : (which == 1 ? c.THi[k] : (which == 2 ? c.THi_mul[k] : c.THi_tr[k]));
+    if (c.direct) {
+      t.D[k] = (which == 0) ? c.Df[k] : (which == 1 ? c.Di[k] : (which == 2 ? c.Di_mul[k] : c.Di_tr[k]));
+    } else {
+      t.D[k] = nullptr;
+      t.TL[k] = (which == 0) ? c.TLf[k] : c.TLi[k];
+      t.TH[k] = (which == 0) ? c.THf[k] : (which == 1 ? c.THi[k] : (which == 2 ? c.THi_mul[k] : c.THi_tr[k]));
+    }
   }
   return t;
 }
@@ -323,7 +370,7 @@ struct WorkLayout {
 };
 WorkLayout work_layout(int n, bool sqr) {
   const u64 L = 1ull << n;
-  const u64 rl = std::min<u64>(L, HGX_RUN);
+  const u64 rl = std::min<u64>(L, HGX_CRT_BLOCK);
   const u64 nruns = L / rl;
   const u64 nblk = (nruns + HGX_CARRY_RUNS_PER_BLOCK - 1) / HGX_CARRY_RUNS_PER_BLOCK;
   WorkLayout w{};
@@ -375,7 +422,7 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
   int rc = build_plan(n, &plan);
   if (rc) return rc;
   const u64 L = 1ull << n;
-  const u64 rl = std::min<u64>(L, HGX_RUN);
+  const u64 rl = std::min<u64>(L, HGX_CRT_BLOCK);  // carry runs are whole CRT blocks
   const u64 nruns = L / rl;
   const int nops = sqr ? 1 : 2;
 
@@ -424,19 +471,22 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     HGX_TIMED("crt", s);
     hgx_launch_crt(dim3(njobs << cb), s, g_ctx.ps, jr, cb, L);
   }
-  const u64 nblk = (nruns + HGX_CARRY_RUNS_PER_BLOCK - 1) / HGX_CARRY_RUNS_PER_BLOCK;
-  const int bpj = log2_ceil(nblk);
-  {
-    HGX_TIMED("carry_map", s);
-    hgx_launch_carry_map(dim3(njobs << bpj), s, jr, bpj, nruns, (int)rl);
-  }
-  {
-    HGX_TIMED("carry_blocks", s);
-    hgx_launch_carry_blocks(dim3(njobs), s, jr, nblk);
-  }
-  {
-    HGX_TIMED("carry_apply", s);
-    hgx_launch_carry_apply(dim3(njobs << bpj), s, jr, bpj, nruns, (int)rl);
+  // run-to-run carries between the CRT blocks of 2048 limbs
+  if (nruns > 1) {
+    const u64 nblk = (nruns + HGX_CARRY_RUNS_PER_BLOCK - 1) / HGX_CARRY_RUNS_PER_BLOCK;
+    const int bpj = log2_ceil(nblk);
+    {
+      HGX_TIMED("carry_map", s);
+      hgx_launch_carry_map(dim3(njobs << bpj), s, jr, bpj, nruns, (int)rl);
+    }
+    {
+      HGX_TIMED("carry_blocks", s);
+      hgx_launch_carry_blocks(dim3(njobs), s, jr, nblk);
+    }
+    {
+      HGX_TIMED("carry_apply", s);
+      hgx_launch_carry_apply(dim3(njobs << bpj), s, jr, bpj, nruns, (int)rl);
+    }
   }
   HIPCHK(hipGetLastError());
   return HGX_OK;

@@ -9,7 +9,9 @@
 #define HGX_EPT 16  // elements of a full tile per thread
 #define HGX_PAD(x) ((x) + ((x) >> 4))
 #define HGX_TILE_LDS (HGX_TILE + HGX_TILE / 16)
-#define HGX_RUN 16  // limbs per carry run
+#define HGX_RUN 16  // limbs per carry run inside a CRT block
+#define HGX_CRT_BLOCK 2048  // limbs per CRT workgroup (the unit of the global carry scan)
+
 
 enum { MODE_MUL = 0, MODE_SQR = 1, MODE_FWD = 2, MODE_INV = 3 };
 
@@ -27,13 +29,17 @@ struct PrimeSet {
   u64 g01, g02, g12;
 };
 
-// Two-level root table of one column pass on blocks of length 2^b:
-// w_B^e = TH[e >> lb] * TL[e & (2^lb - 1)]; both tables hold (value, Shoup
-// companion) pairs, so a twiddle is applied as two Shoup products.
+// Roots of one column pass on blocks of length B = 2^b (T = B/256 columns).
+// Direct form (B <= 2^20): D[k] holds (w_B^(t q), Shoup companion) pairs at
+// index q*T + t, read coalesced along t, one Shoup product per element.
+// Two-level form: w_B^e = TH[e >> lb] * TL[e & (2^lb - 1)], both tables of
+// (value, companion) pairs, two Shoup products per element.
 struct ColTw {
   const u64 *TL[3];
   const u64 *TH[3];
+  const u64 *D[3];
   int lb;
+  int tbits;  // log2 T
 };
 
 // One multiplication as the kernels see it: operands, product and work

@@ -49,6 +49,16 @@ HGX_HD u32 mul_u24(u32 a, u32 b) {
 #endif
 }
 
+// r mod p for r in [0, 2p): t = r - p lies in (-p, p), and p < 2^63, so the
+// sign of t (as a 64-bit two's-complement value) says whether to add p back.
+// Written with a mask instead of a compare so that no VALU carry or compare
+// result has to travel through an SGPR.
+HGX_HD u64 csub(u64 r, u64 p) {
+  const u64 t = r - p;
+  const u32 m = (u32)((int32_t)(u32)(t >> 32) >> 31);
+  return t + ((((u64)m << 32) | m) & p);
+}
+
 // a*b*2^-64 mod p, canonical, for a < 2^64 and b < p (so a*b < p*2^64).
 HGX_HD u64 mont_mul(u64 a, u64 b, u64 p, u32 c) {
   u64 t00, hi;
@@ -77,7 +87,7 @@ HGX_HD u64 mont_mul(u64 a, u64 b, u64 p, u32 c) {
   // 23 bits of floor(m*c / 2^9) without a carry out of 32 bits
   const u32 qlo = ((u32)mc_lo >> 9) + carry2 + nz;
   const u64 t = hi + (mc_hi << 23) + (u64)qlo;
-  return (t >= p) ? t - p : t;
+  return csub(t, p);
 }
 
 // a*w mod p, canonical, by Shoup's method with the precomputed companion
@@ -90,21 +100,21 @@ HGX_HD u64 shoup_mul(u64 a, u64 w, u64 wq, u64 p, u32 c) {
   mul64x64(a, wq, qlo_unused, q);
   (void)qlo_unused;
   const u32 a0 = (u32)a, a1 = (u32)(a >> 32), w0 = (u32)w, w1 = (u32)(w >> 32);
-  const u64 aw = (u64)a0 * w0 + ((u64)(a0 * w1 + a1 * w0) << 32);
-  const u32 q0 = (u32)q;
-  const u64 qp = q + ((u64)(mul_u24(q0 & 511u, c) << 23) << 32);
-  const u64 r = aw - qp;
-  return (r >= p) ? r - p : r;
+  // r = a*w - q*p mod 2^64: the terms that only reach the high word are
+  // gathered in 32 bits
+  const u32 hiterm = a0 * w1 + a1 * w0 - (mul_u24((u32)q & 511u, c) << 23);
+  const u64 r = ((u64)a0 * w0 - q) + ((u64)hiterm << 32);
+  return csub(r, p);
 }
 
-HGX_HD u64 mod_add(u64 a, u64 b, u64 p) {
-  u64 s = a + b;
-  return (s >= p) ? s - p : s;
-}
+HGX_HD u64 mod_add(u64 a, u64 b, u64 p) { return csub(a + b, p); }
 
+// a - b mod p for canonical a, b: a - b lies in (-p, p); add p back when
+// it is negative.
 HGX_HD u64 mod_sub(u64 a, u64 b, u64 p) {
-  u64 d = a - b;
-  return (a < b) ? d + p : d;
+  const u64 d = a - b;
+  const u32 m = (u32)((int32_t)(u32)(d >> 32) >> 31);
+  return d + ((((u64)m << 32) | m) & p);
 }
 
 // x mod p for any 64-bit x (x < 4p because p > 2^62).

@@ -23,10 +23,11 @@ from ._lib import HgxError, MulDesc, check
 
 NTT_MULTIPLY_THRESHOLD = 100
 """*ntt-multiply-threshold* (src/multiply.lisp:9).  The reference multiplies
-operands of at most this many limbs on the host by schoolbook; that routing
-is the Lisp dispatcher's and stays there.  This module sends every non-zero
-product to the GPU NTT, which is exact at every size (as the reference's own
-test forces with the threshold set to 0, tests/mpz.lisp:44-47)."""
+operands of at most this many limbs on the host by schoolbook; the callers
+in series.py and mpd.py keep that routing.  The mpz_* functions of this
+module send every non-zero product to the GPU NTT, which is exact at every
+size (as the reference's own test forces with the threshold set to 0,
+tests/mpz.lisp:44-47)."""
 
 
 def _u64(a) -> np.ndarray:

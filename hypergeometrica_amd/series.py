@@ -62,12 +62,20 @@ def _int(limbs: np.ndarray) -> int:
     return int.from_bytes(np.ascontiguousarray(limbs).astype("<u8").tobytes(), "little")
 
 
-def mul_many(pairs: list[tuple[int, int]]) -> list[int]:
-    """Exact products of Python integers, all in one GPU batch."""
+def mul_many(pairs: list[tuple[int, int]], threshold: int | None = None) -> list[int]:
+    """Exact products of Python integers.  As the MPZ-* dispatcher does
+    (src/multiply.lisp:21-42), a product whose smaller operand has at most
+    THRESHOLD limbs (*ntt-multiply-threshold*, 100) stays on the host; all
+    others go to the GPU in one batched call."""
+    if threshold is None:
+        threshold = _mpz.NTT_MULTIPLY_THRESHOLD
     out = [0] * len(pairs)
     idx, batch = [], []
     for i, (x, y) in enumerate(pairs):
         if x == 0 or y == 0:
+            continue
+        if min(abs(x).bit_length(), abs(y).bit_length()) <= 64 * threshold:
+            out[i] = x * y
             continue
         if x is y or x == y:
             a = _limbs(x)

@@ -205,3 +205,45 @@ def test_server_rejects_empty_operand(hgx):
     from hypergeometrica_amd import HgxError
     with pytest.raises(HgxError):
         hgx.submit(np.zeros(0, dtype=np.uint64), np.ones(3, dtype=np.uint64))
+
+
+# ---------------------------------------------------------------- callers
+
+def test_binary_splitting_gpu_equals_depth_first(hgx):
+    from hypergeometrica_amd import series as S
+    from oracle import series_oracle as SO
+    s = S.chudnovsky_series()
+    got = S.binary_split(s, 0, 300)
+    assert (got.lower, got.upper, got.p, got.q, got.b, got.r) == SO.binary_split(s, 0, 300)
+
+
+@pytest.mark.parametrize("bits,truncate", [(1000, False), (10000, False), (1 << 16, True)])
+def test_mpd_pi_gpu_bit_identical_to_cpu_routing(hgx, bits, truncate):
+    from hypergeometrica_amd import mpd as M
+    gpu = M.Context(truncate=truncate).mpd_pi(bits)
+    cpu = M.Context(mul=lambda x, y: x * y, truncate=truncate).mpd_pi(bits)
+    assert gpu == cpu
+
+
+@pytest.mark.parametrize("k", [4096, 6000, 1 << 17, 1 << 21])
+def test_product_that_is_all_ones(hgx, k):
+    # (2^(32k) - 1)(2^(32k) + 1) = 2^(64k) - 1: every limb of the product is
+    # all ones, so carries cross every run and every 2048-limb block
+    half = k // 2
+    a = np.full(half, 2**64 - 1, dtype=np.uint64)
+    b = np.zeros(half + 1, dtype=np.uint64)
+    b[0] = 1
+    b[half] = 1
+    got = hgx.mul_limbs(a, b)
+    assert got.size == k + 1
+    assert np.all(got[:k] == np.uint64(2**64 - 1)) and got[k] == 0
+
+
+@pytest.mark.parametrize("na,nb", [(3000, 5000), (2047, 2049), (1 << 19, (1 << 19) + 3)])
+def test_all_ones_uneven(hgx, oracle, na, nb):
+    a = np.full(na, 2**64 - 1, dtype=np.uint64)
+    b = np.full(nb, 2**64 - 1, dtype=np.uint64)
+    got = hgx.mul_limbs(a, b)
+    # (2^A - 1)(2^B - 1) = 2^(A+B) - 2^A - 2^B + 1
+    A, B = 64 * na, 64 * nb
+    assert oracle.limbs_to_int(got) == (1 << (A + B)) - (1 << A) - (1 << B) + 1
