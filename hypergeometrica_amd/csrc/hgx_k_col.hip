@@ -6,6 +6,7 @@
 // outermost pass) are applied at load, then the 256-point DIT, in place on
 // the residue array of prime kbase + y.
 // grid = (jobs << jt_bits, nprimes)
+template <bool DIRECT>
 __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
                                                       int kbase) {
   __shared__ u64 lds[HGX_TILE_LDS];
@@ -18,18 +19,17 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
   const u64 rs = 1ull << (bbits - 8);
   const u64 gbase = (blk << bbits) + t0;
   const int k = kbase + blockIdx.y;
-  const u64 p = ps.p[k];
-  const u32 c = ps.c[k];
+  const Mod M = mod_of(ps, k);
   u64 *arr = J.res0 + (u64)blockIdx.y * L;
 #pragma unroll
   for (int j = 0; j < HGX_EPT; j++) {
     const int x = tid + HGX_NT * j;
     const u32 m = x >> 4;
     const u64 t = t0 + (x & 15);
-    lds[HGX_PAD(x)] = apply_twiddle(arr[gbase + (x & 15) + rs * m], tw, k, t, bitrev8(m), p, c);
+    lds[HGX_PAD(x)] = apply_twiddle<DIRECT>(arr[gbase + (x & 15) + rs * m], tw, k, t, bitrev8(m), M);
   }
   __syncthreads();
-  tile_inverse<8, true>(lds, 4, ps.tw_inv[k], p, c);
+  tile_inverse<8, true>(lds, 4, ps.tw_inv[k], M);
 #pragma unroll
   for (int j = 0; j < HGX_EPT; j++) {
     const int x = tid + HGX_NT * j;
@@ -37,31 +37,31 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
   }
 }
 
-// CRT and carry sums.  A workgroup takes a block of min(L, 2048) consecutive
-// coefficients: every thread reconstructs coefficients from the three
-// residue arrays (coalesced loads, Garner form) into LDS; one thread per run
-// of 16 sums its run exactly (digits + a 2-word overflow); the overflows
-// are then carried from run to run inside the block (kill / propagate /
-// generate scan over the 128 runs), so the block leaves exact digits plus
-// one 2-word overflow and an all-ones flag for limbs 2.. of the block.
+// CRT and carry sums.  A workgroup takes a block of min(L, 1024) consecutive
+// coefficients: the threads reconstruct them from the three residue arrays
+// (coalesced loads, Garner form) into LDS; then thread t sums its run of 4
+// coefficients exactly in registers (4 digits + a 2-word overflow); the
+// overflows are carried from run to run inside the block (kill / propagate /
+// generate scan over the 256 runs), so the block leaves exact digits plus one
+// 2-word overflow and an all-ones flag for limbs 2.. of the block.
 // add-big-digit (src/ntt-multiply.lisp:10-51) adds the same values at the
 // same positions.  grid = jobs << bpj_bits
+static_assert(HGX_CRT_BLOCK == HGX_NT * HGX_RUN, "one run per thread");
 __global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_bits, u64 L) {
   constexpr int PADN = HGX_CRT_BLOCK + HGX_CRT_BLOCK / 16;
   __shared__ u64 w0[PADN];
   __shared__ u64 w1[PADN];
   __shared__ u64 w2[PADN];
-  __shared__ u64 ovs[2 * (HGX_CRT_BLOCK / HGX_RUN)];
+  __shared__ u64 ovs[2 * HGX_NT];
   __shared__ u32 scan[HGX_NT];
-  __shared__ u32 allones_blk;
   const int tid = threadIdx.x;
   const MulJob J = load_job(jr, blockIdx.x >> bpj_bits);
   const u64 blk = blockIdx.x & ((1u << bpj_bits) - 1);
   const int P = L < HGX_CRT_BLOCK ? (int)L : HGX_CRT_BLOCK;
   const int rl = P < HGX_RUN ? P : HGX_RUN;
   const int nruns = P / rl;
+  const bool act = tid < nruns;
   const u64 base = blk * HGX_CRT_BLOCK;
-  if (tid == 0) allones_blk = 1u;
   for (int x = tid; x < P; x += HGX_NT) {
     const u64 pos = base + x;
     const U192 cf = garner3(J.res0[pos], J.res0[L + pos], J.res0[2 * L + pos], ps);
@@ -70,11 +70,13 @@ __global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_
     w2[HGX_PAD(x)] = cf.w2;
   }
   __syncthreads();
-  // exact sum of each run
-  u32 hi_ones = 0u;
-  if (tid < nruns) {
-    u64 a0 = 0, a1 = 0, andhi = ~0ull;
-    for (int j = 0; j < rl; j++) {
+  // exact sum of this thread's run: digits d[], overflow (a0, a1)
+  u64 d[HGX_RUN];
+  u64 a0 = 0, a1 = 0, andhi = ~0ull;
+#pragma unroll
+  for (int j = 0; j < HGX_RUN; j++) {
+    d[j] = 0;
+    if (act && j < rl) {
       const int x = tid * rl + j;
       const u64 c0 = w0[HGX_PAD(x)], c1 = w1[HGX_PAD(x)], c2 = w2[HGX_PAD(x)];
       const u64 s0 = a0 + c0;
@@ -83,74 +85,75 @@ __global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_
       u64 cy1 = (s1 < a1) ? 1 : 0;
       const u64 s1b = s1 + cy;
       cy1 += (s1b < s1) ? 1 : 0;
-      w0[HGX_PAD(x)] = s0;
+      d[j] = s0;
       if (j >= 2) andhi &= s0;
       a0 = s1b;
       a1 = c2 + cy1;
     }
+  }
+  if (act) {
     ovs[2 * tid] = a0;
     ovs[2 * tid + 1] = a1;
-    hi_ones = (andhi == ~0ull) ? 1u : 0u;
   }
+  const bool hi_ones = andhi == ~0ull;  // limbs 2.. of the run (vacuous for runs of 2)
   __syncthreads();
-  // carry maps of the runs for the overflow of the previous run
+  // carry map of the run for the overflow of the previous run
   u32 m = 2u;
   u64 x0 = 0, x1 = 0;
-  if (tid < nruns && tid > 0) {
+  if (act && tid > 0) {
     x0 = ovs[2 * (tid - 1)];
     x1 = ovs[2 * (tid - 1) + 1];
-    const u64 d0 = w0[HGX_PAD(tid * rl)];
-    const u64 d1 = rl > 1 ? w0[HGX_PAD(tid * rl + 1)] : 0;
-    const u64 s0 = d0 + x0;
-    const u64 cy = (s0 < d0) ? 1 : 0;
-    const u64 s1 = d1 + x1;
-    u64 c1 = (s1 < d1) ? 1 : 0;
+    const u64 s0 = d[0] + x0;
+    const u64 cy = (s0 < d[0]) ? 1 : 0;
+    const u64 s1 = d[1] + x1;
+    u64 c1 = (s1 < d[1]) ? 1 : 0;
     const u64 s1b = s1 + cy;
     c1 += (s1b < s1) ? 1 : 0;
     const u32 f0 = c1 ? 1u : 0u;
     const u32 f1 = (c1 || (s0 == ~0ull && s1b == ~0ull)) ? 1u : 0u;
     m = hi_ones ? (f0 | (f1 << 1)) : 0u;
-  } else if (tid == 0 && nruns > 0) {
+  } else if (tid == 0) {
     m = 0u;  // the block's first run: incoming carries are added by k_carry_apply
   }
   const u32 ex = block_exclusive_last(m, scan);
-  u32 e = (ex == 3u) ? 1u : 0u;  // carry bit into this run from the run before
-  if (tid < nruns && tid > 0 && (x0 | x1 | e)) {
+  const u64 e = (ex == 3u) ? 1u : 0u;  // carry bit into this run from the run before
+  if (act && tid > 0) {
     u64 carry = e;
-    for (int j = 0; j < rl; j++) {
-      const int x = tid * rl + j;
-      const u64 add = (j == 0) ? x0 : (j == 1 ? x1 : 0);
-      if (j >= 2 && carry == 0) break;
-      const u64 d = w0[HGX_PAD(x)];
-      const u64 s = d + add;
-      u64 cy = (s < d) ? 1 : 0;
-      const u64 s2 = s + carry;
-      cy += (s2 < s) ? 1 : 0;
-      w0[HGX_PAD(x)] = s2;
-      carry = cy;
+#pragma unroll
+    for (int j = 0; j < HGX_RUN; j++) {
+      if (j < rl) {
+        const u64 add = (j == 0) ? x0 : (j == 1 ? x1 : 0);
+        const u64 s = d[j] + add;
+        u64 cy = (s < d[j]) ? 1 : 0;
+        const u64 s2 = s + carry;
+        cy += (s2 < s) ? 1 : 0;
+        d[j] = s2;
+        carry = cy;
+      }
     }
   }
   // the block's overflow: the last run's overflow plus the carry out of it
   if (tid == nruns - 1) {
-    const u32 eo = (m >> e) & 1u;
-    u64 o0 = ovs[2 * tid], o1 = ovs[2 * tid + 1];
-    const u64 n0 = o0 + eo;
-    o1 += (n0 < o0) ? 1 : 0;
+    const u64 eo = (m >> e) & 1u;
+    const u64 n0 = a0 + eo;
     J.ovf[2 * blk] = n0;
-    J.ovf[2 * blk + 1] = o1;
+    J.ovf[2 * blk + 1] = a1 + ((n0 < a0) ? 1 : 0);
   }
-  __syncthreads();
-  // digits out; limbs 2.. of the block all ones?
+  // limbs 2.. of the block all ones?
   u64 andall = ~0ull;
+#pragma unroll
+  for (int j = 0; j < HGX_RUN; j++)
+    if (act && j < rl && tid * rl + j >= 2) andall &= d[j];
+  // digits out through LDS, coalesced
+#pragma unroll
+  for (int j = 0; j < HGX_RUN; j++)
+    if (act && j < rl) w0[HGX_PAD(tid * rl + j)] = d[j];
+  const int allones = __syncthreads_and(andall == ~0ull);
   for (int x = tid; x < P; x += HGX_NT) {
     const u64 pos = base + x;
-    const u64 d = w0[HGX_PAD(x)];
-    if (x >= 2) andall &= d;
-    if (pos < J.nout) J.out[pos] = d;
+    if (pos < J.nout) J.out[pos] = w0[HGX_PAD(x)];
   }
-  if (andall != ~0ull) allones_blk = 0u;
-  __syncthreads();
-  if (tid == 0) J.flags[blk] = allones_blk;
+  if (tid == 0) J.flags[blk] = allones ? 1u : 0u;
 }
 
 // per-run maps and each block's combined map.
@@ -256,15 +259,23 @@ __global__ void k_splitmix64(u64 *out, u64 n, u64 seed, int force_top) {
 
 void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr,
                         int jt_bits, u64 L, int bbits, int kbase) {
-  if (first)
-    hipLaunchKernelGGL((k_fwd_col<true>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+  const bool direct = tw.D[0] != nullptr;
+  if (first && direct)
+    hipLaunchKernelGGL((k_fwd_col<true, true>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+  else if (first)
+    hipLaunchKernelGGL((k_fwd_col<true, false>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+  else if (direct)
+    hipLaunchKernelGGL((k_fwd_col<false, true>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
   else
-    hipLaunchKernelGGL((k_fwd_col<false>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+    hipLaunchKernelGGL((k_fwd_col<false, false>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
 }
 
 void hgx_launch_inv_col(dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr, int jt_bits,
                         u64 L, int bbits, int kbase) {
-  hipLaunchKernelGGL(k_inv_col, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+  if (tw.D[0] != nullptr)
+    hipLaunchKernelGGL(k_inv_col<true>, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+  else
+    hipLaunchKernelGGL(k_inv_col<false>, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
 }
 
 void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &jr, int bpj_bits, u64 L) {

@@ -22,7 +22,7 @@
 //   consecutive words.
 //
 // Multiply pipeline (one L, three primes):
-//   k_fwd_col<FIRST>  limbs -> residues mod p_k for all k, first column pass
+//   k_fwd_col<true>   limbs -> residues mod p_k for all k, first column pass
 //   k_fwd_col         further column passes (in place)
 //   k_row<MUL|SQR>    last forward pass of both operands, pointwise product,
 //                     first inverse pass, per prime
@@ -38,16 +38,18 @@
 __device__ __forceinline__ u32 bitrev8(u32 m) { return __builtin_bitreverse32(m) >> 24; }
 
 // x * w_B^(t q) for column t, frequency q: one Shoup product from the direct
-// table, or two from the two-level table.
-__device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u64 t, u32 q, u64 p, u32 c) {
-  if (tw.D[k]) {
+// table (DIRECT), or two from the two-level table.
+template <bool DIRECT>
+__device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u64 t, u32 q, const Mod &M) {
+  if constexpr (DIRECT) {
     const ulonglong2 w = *(const ulonglong2 *)(tw.D[k] + 2 * (((u64)q << tw.tbits) + t));
-    return shoup_mul(x, w.x, w.y, p, c);
+    return shoup_mul(x, w.x, w.y, M);
+  } else {
+    const u64 e = t * q;
+    const ulonglong2 lo = *(const ulonglong2 *)(tw.TL[k] + 2 * (e & ((1ull << tw.lb) - 1)));
+    const ulonglong2 hi = *(const ulonglong2 *)(tw.TH[k] + 2 * (e >> tw.lb));
+    return shoup_mul(shoup_mul(x, lo.x, lo.y, M), hi.x, hi.y, M);
   }
-  const u64 e = t * q;
-  const ulonglong2 lo = *(const ulonglong2 *)(tw.TL[k] + 2 * (e & ((1ull << tw.lb) - 1)));
-  const ulonglong2 hi = *(const ulonglong2 *)(tw.TH[k] + 2 * (e >> tw.lb));
-  return shoup_mul(shoup_mul(x, lo.x, lo.y, p, c), hi.x, hi.y, p, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -74,7 +76,7 @@ struct Log2<1> { static constexpr int value = 0; };
 // COLFAST: column tile (16 columns, LDS index m*16 + t, lanes run along t).
 // Otherwise: row tile (LDS index t*S + m, lanes run along m).
 template <int S_BITS, int A, int R, bool INV, bool COLFAST>
-__device__ __forceinline__ void tile_round(u64 *lds, int wbits, const u64 *__restrict__ tw, u64 p, u32 c) {
+__device__ __forceinline__ void tile_round(u64 *lds, int wbits, const u64 *__restrict__ tw, const Mod &M) {
   constexpr int S = 1 << S_BITS;
   constexpr int D = S >> (A + R);
   constexpr int LD = Log2<D>::value;
@@ -109,13 +111,13 @@ __device__ __forceinline__ void tile_round(u64 *lds, int wbits, const u64 *__res
         for (int j = 0; j < RR; j++) {
           if (j & half) continue;
           const u64 u = x[j], v = x[j + half];
-          x[j] = mod_add(u, v, p);
+          x[j] = mod_add(u, v, M);
           if (D == 1 && (j & (half - 1)) == 0) {
-            x[j + half] = mod_sub(u, v, p);  // twiddle w^0 = 1
+            x[j + half] = mod_sub(u, v, M);  // twiddle w^0 = 1
           } else {
             const int e = goff + (j & (half - 1)) * D;
             const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
-            x[j + half] = shoup_mul(u + (p - v), w.x, w.y, p, c);
+            x[j + half] = shoup_mul(u + (M.p - v), w.x, w.y, M);
           }
         }
       }
@@ -135,10 +137,10 @@ __device__ __forceinline__ void tile_round(u64 *lds, int wbits, const u64 *__res
           } else {
             const int e = goff + (j & (half - 1)) * D;
             const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
-            v = shoup_mul(x[j + half], w.x, w.y, p, c);
+            v = shoup_mul(x[j + half], w.x, w.y, M);
           }
-          x[j] = mod_add(u, v, p);
-          x[j + half] = mod_sub(u, v, p);
+          x[j] = mod_add(u, v, M);
+          x[j + half] = mod_sub(u, v, M);
         }
       }
     }
@@ -153,33 +155,33 @@ __device__ __forceinline__ void tile_round(u64 *lds, int wbits, const u64 *__res
 
 template <int S_BITS, int I, bool COLFAST>
 struct FwdRounds {
-  static __device__ __forceinline__ void run(u64 *lds, int wbits, const u64 *tw, u64 p, u32 c) {
+  static __device__ __forceinline__ void run(u64 *lds, int wbits, const u64 *tw, const Mod &M) {
     if constexpr (I < RoundPlan<S_BITS>::NR) {
-      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), false, COLFAST>(lds, wbits, tw, p, c);
+      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), false, COLFAST>(lds, wbits, tw, M);
       __syncthreads();
-      FwdRounds<S_BITS, I + 1, COLFAST>::run(lds, wbits, tw, p, c);
+      FwdRounds<S_BITS, I + 1, COLFAST>::run(lds, wbits, tw, M);
     }
   }
 };
 
 template <int S_BITS, int I, bool COLFAST>
 struct InvRounds {  // I counts down from NR-1
-  static __device__ __forceinline__ void run(u64 *lds, int wbits, const u64 *tw, u64 p, u32 c) {
+  static __device__ __forceinline__ void run(u64 *lds, int wbits, const u64 *tw, const Mod &M) {
     if constexpr (I >= 0) {
-      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), true, COLFAST>(lds, wbits, tw, p, c);
+      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), true, COLFAST>(lds, wbits, tw, M);
       __syncthreads();
-      InvRounds<S_BITS, I - 1, COLFAST>::run(lds, wbits, tw, p, c);
+      InvRounds<S_BITS, I - 1, COLFAST>::run(lds, wbits, tw, M);
     }
   }
 };
 
 template <int S_BITS, bool COLFAST>
-__device__ __forceinline__ void tile_forward(u64 *lds, int wbits, const u64 *tw, u64 p, u32 c) {
-  FwdRounds<S_BITS, 0, COLFAST>::run(lds, wbits, tw, p, c);
+__device__ __forceinline__ void tile_forward(u64 *lds, int wbits, const u64 *tw, const Mod &M) {
+  FwdRounds<S_BITS, 0, COLFAST>::run(lds, wbits, tw, M);
 }
 template <int S_BITS, bool COLFAST>
-__device__ __forceinline__ void tile_inverse(u64 *lds, int wbits, const u64 *tw, u64 p, u32 c) {
-  InvRounds<S_BITS, RoundPlan<S_BITS>::NR - 1, COLFAST>::run(lds, wbits, tw, p, c);
+__device__ __forceinline__ void tile_inverse(u64 *lds, int wbits, const u64 *tw, const Mod &M) {
+  InvRounds<S_BITS, RoundPlan<S_BITS>::NR - 1, COLFAST>::run(lds, wbits, tw, M);
 }
 
 // ---------------------------------------------------------------------------
@@ -192,14 +194,15 @@ struct U192 { u64 w0, w1, w2; };
 
 __device__ __forceinline__ U192 garner3(u64 r0, u64 r1, u64 r2, const PrimeSet &ps) {
   const u64 p0 = ps.p[0], p1 = ps.p[1], p2 = ps.p[2];
+  const Mod M1 = mod_of(ps, 1), M2 = mod_of(ps, 2);
   // v1 = (r1 - r0) p0^-1 mod p1
   const u64 r0m1 = (r0 >= p1) ? r0 - p1 : r0;
-  const u64 v1 = mont_mul(mod_sub(r1, r0m1, p1), ps.g01, p1, ps.c[1]);
+  const u64 v1 = mont_mul(mod_sub(r1, r0m1, M1), ps.g01, M1);
   // v2 = ((r2 - r0) p0^-1 - v1) p1^-1 mod p2
   const u64 r0m2 = (r0 >= p2) ? r0 - p2 : r0;
   const u64 v1m2 = (v1 >= p2) ? v1 - p2 : v1;
-  u64 t = mont_mul(mod_sub(r2, r0m2, p2), ps.g02, p2, ps.c[2]);
-  const u64 v2 = mont_mul(mod_sub(t, v1m2, p2), ps.g12, p2, ps.c[2]);
+  u64 t = mont_mul(mod_sub(r2, r0m2, M2), ps.g02, M2);
+  const u64 v2 = mont_mul(mod_sub(t, v1m2, M2), ps.g12, M2);
   // u = v1 + p1 * v2  (< p1 p2 < 2^126)
   u64 ulo, uhi;
   mul64x64(p1, v2, ulo, uhi);
@@ -230,7 +233,7 @@ __device__ __forceinline__ U192 garner3(u64 r0, u64 r1, u64 r2, const PrimeSet &
 // src/number-theoretic-transform.lisp:7-22) and writes all three residue
 // arrays.  Otherwise: in place on the residue array of prime kbase + y.
 // grid = (jobs << jt_bits, FIRST ? 1 : nprimes, noperands)
-template <bool FIRST>
+template <bool FIRST, bool DIRECT>
 __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
                                                       int kbase) {
   __shared__ u64 lds[HGX_TILE_LDS];
@@ -256,29 +259,27 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
       v[j] = (pos < n) ? limbs[pos] : 0;
     }
     for (int k = 0; k < 3; k++) {
-      const u64 p = ps.p[k];
-      const u32 c = ps.c[k];
+      const Mod M = mod_of(ps, k);
 #pragma unroll
       for (int j = 0; j < HGX_EPT; j++) {
         const int x = tid + HGX_NT * j;
-        lds[HGX_PAD(x)] = mod_reduce64(v[j], p);
+        lds[HGX_PAD(x)] = mod_reduce64(v[j], M);
       }
       __syncthreads();
-      tile_forward<8, true>(lds, 4, ps.tw_fwd[k], p, c);
+      tile_forward<8, true>(lds, 4, ps.tw_fwd[k], M);
       u64 *dst = res + (u64)k * L;
 #pragma unroll
       for (int j = 0; j < HGX_EPT; j++) {
         const int x = tid + HGX_NT * j;
         const u32 m = x >> 4;
         const u64 t = t0 + (x & 15);
-        dst[gbase + (x & 15) + rs * m] = apply_twiddle(lds[HGX_PAD(x)], tw, k, t, bitrev8(m), p, c);
+        dst[gbase + (x & 15) + rs * m] = apply_twiddle<DIRECT>(lds[HGX_PAD(x)], tw, k, t, bitrev8(m), M);
       }
       __syncthreads();
     }
   } else {
     const int k = kbase + blockIdx.y;
-    const u64 p = ps.p[k];
-    const u32 c = ps.c[k];
+    const Mod M = mod_of(ps, k);
     u64 *arr = res + (u64)blockIdx.y * L;
 #pragma unroll
     for (int j = 0; j < HGX_EPT; j++) {
@@ -286,13 +287,13 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
       lds[HGX_PAD(x)] = arr[gbase + (x & 15) + rs * (u64)(x >> 4)];
     }
     __syncthreads();
-    tile_forward<8, true>(lds, 4, ps.tw_fwd[k], p, c);
+    tile_forward<8, true>(lds, 4, ps.tw_fwd[k], M);
 #pragma unroll
     for (int j = 0; j < HGX_EPT; j++) {
       const int x = tid + HGX_NT * j;
       const u32 m = x >> 4;
       const u64 t = t0 + (x & 15);
-      arr[gbase + (x & 15) + rs * m] = apply_twiddle(lds[HGX_PAD(x)], tw, k, t, bitrev8(m), p, c);
+      arr[gbase + (x & 15) + rs * m] = apply_twiddle<DIRECT>(lds[HGX_PAD(x)], tw, k, t, bitrev8(m), M);
     }
   }
 }
@@ -320,8 +321,7 @@ __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRe
 
   if (!SINGLE) {
     const int k = rp.kbase + blockIdx.y;
-    const u64 p = ps.p[k];
-    const u32 c = ps.c[k];
+    const Mod M = mod_of(ps, k);
     u64 *A = J.res0 + (u64)blockIdx.y * rp.L + gbase;
     u64 *Bp = TWO ? (J.res1 + (u64)blockIdx.y * rp.L + gbase) : nullptr;
     for (int x = tid; x < TW; x += HGX_NT) {
@@ -330,41 +330,40 @@ __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRe
     }
     __syncthreads();
     if (MODE == MODE_MUL || MODE == MODE_SQR || MODE == MODE_FWD) {
-      tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], p, c);
-      if (TWO) tile_forward<S_BITS, false>(ldsB, wbits, ps.tw_fwd[k], p, c);
+      tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], M);
+      if (TWO) tile_forward<S_BITS, false>(ldsB, wbits, ps.tw_fwd[k], M);
     }
     if (MODE == MODE_MUL || MODE == MODE_SQR) {
       for (int x = tid; x < TW; x += HGX_NT) {
         const u64 a = ldsA[HGX_PAD(x)];
         const u64 b = TWO ? ldsB[HGX_PAD(x)] : a;
-        ldsA[HGX_PAD(x)] = mont_mul(a, b, p, c);
+        ldsA[HGX_PAD(x)] = mont_mul(a, b, M);
       }
       __syncthreads();
     }
-    if (MODE != MODE_FWD) tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], p, c);
+    if (MODE != MODE_FWD) tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], M);
     for (int x = tid; x < TW; x += HGX_NT) A[x] = ldsA[HGX_PAD(x)];
   } else {
     // whole transform in one tile (L <= 4096): limbs in, all three primes in
     // turn, residues of the product out (k_crt finishes)
 #pragma unroll 1
     for (int k = 0; k < 3; k++) {
-      const u64 p = ps.p[k];
-      const u32 c = ps.c[k];
+      const Mod M = mod_of(ps, k);
       for (int x = tid; x < TW; x += HGX_NT) {
         const u64 pos = gbase + x;
-        ldsA[HGX_PAD(x)] = (pos < J.na) ? mod_reduce64(J.a[pos], p) : 0;
-        if (TWO) ldsB[HGX_PAD(x)] = (pos < J.nb) ? mod_reduce64(J.b[pos], p) : 0;
+        ldsA[HGX_PAD(x)] = (pos < J.na) ? mod_reduce64(J.a[pos], M) : 0;
+        if (TWO) ldsB[HGX_PAD(x)] = (pos < J.nb) ? mod_reduce64(J.b[pos], M) : 0;
       }
       __syncthreads();
-      tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], p, c);
-      if (TWO) tile_forward<S_BITS, false>(ldsB, wbits, ps.tw_fwd[k], p, c);
+      tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], M);
+      if (TWO) tile_forward<S_BITS, false>(ldsB, wbits, ps.tw_fwd[k], M);
       for (int x = tid; x < TW; x += HGX_NT) {
         const u64 a = ldsA[HGX_PAD(x)];
         const u64 b = TWO ? ldsB[HGX_PAD(x)] : a;
-        ldsA[HGX_PAD(x)] = mont_mul(mont_mul(a, b, p, c), rp.scale[k], p, c);
+        ldsA[HGX_PAD(x)] = mont_mul(mont_mul(a, b, M), rp.scale[k], M);
       }
       __syncthreads();
-      tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], p, c);
+      tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], M);
       u64 *dst = J.res0 + (u64)k * rp.L + gbase;
       for (int x = tid; x < TW; x += HGX_NT) dst[x] = ldsA[HGX_PAD(x)];
       __syncthreads();
@@ -381,13 +380,12 @@ __global__ __launch_bounds__(HGX_NT) void k_row_inv_scaled(PrimeSet ps, RowParam
   const int TW = 1 << rp.tw_bits;
   const int wbits = rp.tw_bits - S_BITS;
   const int k = rp.kbase;
-  const u64 p = ps.p[k];
-  const u32 c = ps.c[k];
+  const Mod M = mod_of(ps, k);
   u64 *A = jr.one.res0 + ((u64)blockIdx.x << rp.tw_bits);
   for (int x = tid; x < TW; x += HGX_NT) ldsA[HGX_PAD(x)] = A[x];
   __syncthreads();
-  tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], p, c);
-  for (int x = tid; x < TW; x += HGX_NT) A[x] = mont_mul(ldsA[HGX_PAD(x)], rp.scale[k], p, c);
+  tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], M);
+  for (int x = tid; x < TW; x += HGX_NT) A[x] = mont_mul(ldsA[HGX_PAD(x)], rp.scale[k], M);
 }
 
 // ---------------------------------------------------------------------------

@@ -467,7 +467,7 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     }
   }
   {
-    const int cb = n > 11 ? n - 11 : 0;  // CRT blocks of 2048 coefficients per job
+    const int cb = n > HGX_CRT_BLOCK_BITS ? n - HGX_CRT_BLOCK_BITS : 0;  // CRT blocks per job (log2)
     HGX_TIMED("crt", s);
     hgx_launch_crt(dim3(njobs << cb), s, g_ctx.ps, jr, cb, L);
   }
@@ -921,6 +921,7 @@ int hgx_init(int device, size_t hbm_budget_bytes) {
   for (int k = 0; k < 3; k++) {
     const u64 p = kModuli[k];
     ps.p[k] = p;
+    ps.np[k] = 0 - p;
     ps.c[k] = (u32)((p - 1) >> 55);
     g_ctx.gen[k] = find_generator(p, ps.c[k]);
     const u64 w = powmod(g_ctx.gen[k], (p - 1) >> 12, p);

@@ -9,8 +9,9 @@
 #define HGX_EPT 16  // elements of a full tile per thread
 #define HGX_PAD(x) ((x) + ((x) >> 4))
 #define HGX_TILE_LDS (HGX_TILE + HGX_TILE / 16)
-#define HGX_RUN 16  // limbs per carry run inside a CRT block
-#define HGX_CRT_BLOCK 2048  // limbs per CRT workgroup (the unit of the global carry scan)
+#define HGX_RUN 4  // limbs per carry run: one run per thread of a CRT workgroup
+#define HGX_CRT_BLOCK_BITS 10
+#define HGX_CRT_BLOCK (1 << HGX_CRT_BLOCK_BITS)  // limbs per CRT workgroup (the unit of the global carry scan)
 
 
 enum { MODE_MUL = 0, MODE_SQR = 1, MODE_FWD = 2, MODE_INV = 3 };
@@ -22,6 +23,7 @@ enum { MODE_MUL = 0, MODE_SQR = 1, MODE_FWD = 2, MODE_INV = 3 };
 // w_4096 = g^((p-1)/4096) (primitive-root-from-generator, src/moduli.lisp:182-187).
 struct PrimeSet {
   u64 p[3];
+  u64 np[3];  // 2^64 - p
   u32 c[3];
   const u64 *tw_fwd[3];
   const u64 *tw_inv[3];
@@ -41,6 +43,8 @@ struct ColTw {
   int lb;
   int tbits;  // log2 T
 };
+
+__host__ __device__ inline Mod mod_of(const PrimeSet &ps, int k) { return Mod{ps.p[k], ps.np[k], ps.c[k]}; }
 
 // One multiplication as the kernels see it: operands, product and work
 // arrays in device memory.  A launch covers one job (passed by value) or a

@@ -49,18 +49,28 @@ HGX_HD u32 mul_u24(u32 a, u32 b) {
 #endif
 }
 
+// One modulus as the kernels use it: p, its negation 2^64 - p (kept as its
+// own value so that r - p compiles to a single 64-bit add, with no carry
+// through an SGPR), and the Proth factor c of p = c*2^55 + 1.
+struct Mod {
+  u64 p;
+  u64 np;
+  u32 c;
+};
+
 // r mod p for r in [0, 2p): t = r - p lies in (-p, p), and p < 2^63, so the
-// sign of t (as a 64-bit two's-complement value) says whether to add p back.
-// Written with a mask instead of a compare so that no VALU carry or compare
-// result has to travel through an SGPR.
-HGX_HD u64 csub(u64 r, u64 p) {
-  const u64 t = r - p;
+// sign of t (as a 64-bit two's-complement value) says whether r was already
+// reduced; a bitwise select (v_bfi_b32) picks r or t.
+HGX_HD u64 csub(u64 r, const Mod &M) {
+  const u64 t = r + M.np;
   const u32 m = (u32)((int32_t)(u32)(t >> 32) >> 31);
-  return t + ((((u64)m << 32) | m) & p);
+  const u64 mm = ((u64)m << 32) | m;
+  return (r & mm) | (t & ~mm);
 }
 
 // a*b*2^-64 mod p, canonical, for a < 2^64 and b < p (so a*b < p*2^64).
-HGX_HD u64 mont_mul(u64 a, u64 b, u64 p, u32 c) {
+HGX_HD u64 mont_mul(u64 a, u64 b, const Mod &M) {
+  const u32 c = M.c;
   u64 t00, hi;
   {
     const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
@@ -87,7 +97,7 @@ HGX_HD u64 mont_mul(u64 a, u64 b, u64 p, u32 c) {
   // 23 bits of floor(m*c / 2^9) without a carry out of 32 bits
   const u32 qlo = ((u32)mc_lo >> 9) + carry2 + nz;
   const u64 t = hi + (mc_hi << 23) + (u64)qlo;
-  return csub(t, p);
+  return csub(t, M);
 }
 
 // a*w mod p, canonical, by Shoup's method with the precomputed companion
@@ -95,31 +105,31 @@ HGX_HD u64 mont_mul(u64 a, u64 b, u64 p, u32 c) {
 // src/modular-arithmetic.lisp:206-231): q = hi64(a*wq) is exact, so
 // r = a*w - q*p lies in [0, 2p) for any a < 2^64 and one subtraction makes it
 // canonical.  q*p mod 2^64 = q + ((q*c mod 2^9) << 55) for p = c*2^55 + 1.
-HGX_HD u64 shoup_mul(u64 a, u64 w, u64 wq, u64 p, u32 c) {
+HGX_HD u64 shoup_mul(u64 a, u64 w, u64 wq, const Mod &M) {
   u64 qlo_unused, q;
   mul64x64(a, wq, qlo_unused, q);
   (void)qlo_unused;
   const u32 a0 = (u32)a, a1 = (u32)(a >> 32), w0 = (u32)w, w1 = (u32)(w >> 32);
   // r = a*w - q*p mod 2^64: the terms that only reach the high word are
   // gathered in 32 bits
-  const u32 hiterm = a0 * w1 + a1 * w0 - (mul_u24((u32)q & 511u, c) << 23);
+  const u32 hiterm = a0 * w1 + a1 * w0 - (mul_u24((u32)q & 511u, M.c) << 23);
   const u64 r = ((u64)a0 * w0 - q) + ((u64)hiterm << 32);
-  return csub(r, p);
+  return csub(r, M);
 }
 
-HGX_HD u64 mod_add(u64 a, u64 b, u64 p) { return csub(a + b, p); }
+HGX_HD u64 mod_add(u64 a, u64 b, const Mod &M) { return csub(a + b, M); }
 
 // a - b mod p for canonical a, b: a - b lies in (-p, p); add p back when
 // it is negative.
-HGX_HD u64 mod_sub(u64 a, u64 b, u64 p) {
+HGX_HD u64 mod_sub(u64 a, u64 b, const Mod &M) {
   const u64 d = a - b;
   const u32 m = (u32)((int32_t)(u32)(d >> 32) >> 31);
-  return d + ((((u64)m << 32) | m) & p);
+  return d + ((((u64)m << 32) | m) & M.p);
 }
 
 // x mod p for any 64-bit x (x < 4p because p > 2^62).
-HGX_HD u64 mod_reduce64(u64 x, u64 p) {
-  u64 p2 = p << 1;
+HGX_HD u64 mod_reduce64(u64 x, const Mod &M) {
+  const u64 p2 = M.p << 1;
   x = (x >= p2) ? x - p2 : x;
-  return (x >= p) ? x - p : x;
+  return (x >= M.p) ? x - M.p : x;
 }
