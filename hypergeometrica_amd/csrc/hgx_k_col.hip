@@ -3,38 +3,44 @@
 #include "hgx_launch.hpp"
 
 // Column pass, inverse.  Twiddles w_B^-(t q) (times the final scale on the
-// outermost pass) are applied at load, then the 256-point DIT, in place on
-// the residue array of prime kbase + y.
+// outermost pass) are applied as rows 16 g + j of column t come in from
+// HBM; the 256-point DIT then runs as two rounds of 16 (stages 7-4 in
+// registers, one LDS exchange, stages 3-0) and rows g + 16 j go straight
+// back, in place on the residue array of prime kbase + y.
 // grid = (jobs << jt_bits, nprimes)
 template <bool DIRECT>
 __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
                                                       int kbase) {
   __shared__ u64 lds[HGX_TILE_LDS];
   const int tid = threadIdx.x;
+  const int t = tid & 15, g = tid >> 4;
   const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
   const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
   const int tpb_bits = bbits - 12;
   const u64 blk = tile >> tpb_bits;
   const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
   const u64 rs = 1ull << (bbits - 8);
-  const u64 gbase = (blk << bbits) + t0;
+  const u64 gbase = (blk << bbits) + t0 + t;
+  // LDS slots (padded, HGX_PAD) of rows g + 16 j and 16 g + j of column t
+  const int l1 = 17 * g + t, l2 = 272 * g + t;
   const int k = kbase + blockIdx.y;
   const Mod M = mod_of(ps, k);
   u64 *arr = J.res0 + (u64)blockIdx.y * L;
+  u64 x[16];
 #pragma unroll
-  for (int j = 0; j < HGX_EPT; j++) {
-    const int x = tid + HGX_NT * j;
-    const u32 m = x >> 4;
-    const u64 t = t0 + (x & 15);
-    lds[HGX_PAD(x)] = apply_twiddle<DIRECT>(arr[gbase + (x & 15) + rs * m], tw, k, t, bitrev8(m), M);
+  for (int j = 0; j < 16; j++) {
+    const u32 m = 16 * g + j;
+    x[j] = apply_twiddle<DIRECT>(arr[gbase + rs * m], tw, k, t0 + t, bitrev8(m), M);
   }
+  round_regs<8, 4, 4, true>(x, 0, ps.tw_inv[k], M);
+#pragma unroll
+  for (int j = 0; j < 16; j++) lds[l2 + 17 * j] = x[j];
   __syncthreads();
-  tile_inverse<8, true>(lds, 4, ps.tw_inv[k], M);
 #pragma unroll
-  for (int j = 0; j < HGX_EPT; j++) {
-    const int x = tid + HGX_NT * j;
-    arr[gbase + (x & 15) + rs * (u64)(x >> 4)] = lds[HGX_PAD(x)];
-  }
+  for (int j = 0; j < 16; j++) x[j] = lds[l1 + 272 * j];
+  round_regs<8, 0, 4, true>(x, g, ps.tw_inv[k], M);
+#pragma unroll
+  for (int j = 0; j < 16; j++) arr[gbase + rs * (u64)(g + 16 * j)] = x[j];
 }
 
 // CRT and carry sums.  A workgroup takes a block of min(L, 1024) consecutive

@@ -73,6 +73,69 @@ struct Log2 { static constexpr int value = 1 + Log2<V / 2>::value; };
 template <>
 struct Log2<1> { static constexpr int value = 0; };
 
+// A scheduling fence after every radix-2 stage keeps the twiddle loads of
+// one stage (at most 8 pairs) in registers at a time.
+#if defined(HGX_NO_STAGE_FENCE)
+#define HGX_STAGE_FENCE() ((void)0)
+#else
+#define HGX_STAGE_FENCE() __builtin_amdgcn_sched_barrier(0)
+#endif
+
+// Butterflies of one round on the 2^R values of one group, in registers:
+// x[j] is element base + j*D of the group, and goff = base mod D selects the
+// twiddles (w_2h^e from the w_4096 table, e = goff + i*D).
+template <int S_BITS, int A, int R, bool INV>
+__device__ __forceinline__ void round_regs(u64 *x, int goff, const u64 *__restrict__ tw, const Mod &M) {
+  constexpr int S = 1 << S_BITS;
+  constexpr int D = S >> (A + R);
+  constexpr int RR = 1 << R;
+  if (!INV) {
+#pragma unroll
+    for (int l = 0; l < R; l++) {
+      const int half = 1 << (R - 1 - l);
+      const int h = half * D;
+      const int tstride = 2048 / h;
+#pragma unroll
+      for (int j = 0; j < RR; j++) {
+        if (j & half) continue;
+        const u64 u = x[j], v = x[j + half];
+        x[j] = mod_add(u, v, M);
+        if (D == 1 && (j & (half - 1)) == 0) {
+          x[j + half] = mod_sub(u, v, M);  // twiddle w^0 = 1
+        } else {
+          const int e = goff + (j & (half - 1)) * D;
+          const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
+          x[j + half] = shoup_mul(u + (M.p - v), w.x, w.y, M);
+        }
+      }
+      HGX_STAGE_FENCE();
+    }
+  } else {
+#pragma unroll
+    for (int l = R - 1; l >= 0; l--) {
+      const int half = 1 << (R - 1 - l);
+      const int h = half * D;
+      const int tstride = 2048 / h;
+#pragma unroll
+      for (int j = 0; j < RR; j++) {
+        if (j & half) continue;
+        const u64 u = x[j];
+        u64 v;
+        if (D == 1 && (j & (half - 1)) == 0) {
+          v = x[j + half];  // twiddle w^0 = 1
+        } else {
+          const int e = goff + (j & (half - 1)) * D;
+          const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
+          v = shoup_mul(x[j + half], w.x, w.y, M);
+        }
+        x[j] = mod_add(u, v, M);
+        x[j + half] = mod_sub(u, v, M);
+      }
+      HGX_STAGE_FENCE();
+    }
+  }
+}
+
 // COLFAST: column tile (16 columns, LDS index m*16 + t, lanes run along t).
 // Otherwise: row tile (LDS index t*S + m, lanes run along m).
 template <int S_BITS, int A, int R, bool INV, bool COLFAST>
@@ -101,49 +164,7 @@ __device__ __forceinline__ void tile_round(u64 *lds, int wbits, const u64 *__res
       const int li = COLFAST ? (m * 16 + t) : (t * S + m);
       x[j] = lds[HGX_PAD(li)];
     }
-    if (!INV) {
-#pragma unroll
-      for (int l = 0; l < R; l++) {
-        const int half = 1 << (R - 1 - l);
-        const int h = half * D;
-        const int tstride = 2048 / h;
-#pragma unroll
-        for (int j = 0; j < RR; j++) {
-          if (j & half) continue;
-          const u64 u = x[j], v = x[j + half];
-          x[j] = mod_add(u, v, M);
-          if (D == 1 && (j & (half - 1)) == 0) {
-            x[j + half] = mod_sub(u, v, M);  // twiddle w^0 = 1
-          } else {
-            const int e = goff + (j & (half - 1)) * D;
-            const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
-            x[j + half] = shoup_mul(u + (M.p - v), w.x, w.y, M);
-          }
-        }
-      }
-    } else {
-#pragma unroll
-      for (int l = R - 1; l >= 0; l--) {
-        const int half = 1 << (R - 1 - l);
-        const int h = half * D;
-        const int tstride = 2048 / h;
-#pragma unroll
-        for (int j = 0; j < RR; j++) {
-          if (j & half) continue;
-          const u64 u = x[j];
-          u64 v;
-          if (D == 1 && (j & (half - 1)) == 0) {
-            v = x[j + half];  // twiddle w^0 = 1
-          } else {
-            const int e = goff + (j & (half - 1)) * D;
-            const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
-            v = shoup_mul(x[j + half], w.x, w.y, M);
-          }
-          x[j] = mod_add(u, v, M);
-          x[j + half] = mod_sub(u, v, M);
-        }
-      }
-    }
+    round_regs<S_BITS, A, R, INV>(x, goff, tw, M);
 #pragma unroll
     for (int j = 0; j < RR; j++) {
       const int m = base + j * D;
@@ -228,72 +249,193 @@ __device__ __forceinline__ U192 garner3(u64 r0, u64 r1, u64 r2, const PrimeSet &
 // ---------------------------------------------------------------------------
 
 // Column pass, forward.  Tile = 16 adjacent columns of one block of length
-// 2^bbits.  FIRST: the input is the operand's limbs (zero above its length);
-// each workgroup reduces them mod every prime (as %make-ntt-work,
-// src/number-theoretic-transform.lisp:7-22) and writes all three residue
-// arrays.  Otherwise: in place on the residue array of prime kbase + y.
-// grid = (jobs << jt_bits, FIRST ? 1 : nprimes, noperands)
+// 2^bbits.  FIRST: the input is the operand's limbs (zero above its length),
+// reduced mod prime y (as %make-ntt-work, src/number-theoretic-transform.lisp:7-22)
+// into residue array y.  Otherwise: in place on the residue array of prime
+// kbase + y.
+// grid = (jobs << jt_bits, nprimes, noperands)
 template <bool FIRST, bool DIRECT>
 __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
                                                       int kbase) {
+  // The 256-point column DFT is two rounds of 16: round 1 (stages 0-3) takes
+  // rows g + 16 j of column t straight from HBM, round 2 (stages 4-7) rows
+  // 16 g + j, which go straight back to HBM with their twiddles; one LDS
+  // exchange between the rounds.  Every HBM access is a 128-byte row segment.
   __shared__ u64 lds[HGX_TILE_LDS];
   const int tid = threadIdx.x;
+  const int t = tid & 15, g = tid >> 4;
   const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
   const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
   const int tpb_bits = bbits - 12;  // tiles per block
   const u64 blk = tile >> tpb_bits;
   const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
   const u64 rs = 1ull << (bbits - 8);
-  const u64 gbase = (blk << bbits) + t0;
+  const u64 gbase = (blk << bbits) + t0 + t;
+  // LDS slots (padded, HGX_PAD) of rows g + 16 j and 16 g + j of column t
+  const int l1 = 17 * g + t, l2 = 272 * g + t;
   const int op = blockIdx.z;
   u64 *res = op ? J.res1 : J.res0;
+  u64 x[16];
 
+  const int k = kbase + blockIdx.y;
+  const Mod M = mod_of(ps, k);
+  u64 *arr = res + (u64)blockIdx.y * L;
   if (FIRST) {
+    // limbs -> residues mod p_k (the limbs are read once per prime)
     const u64 *limbs = op ? J.b : J.a;
     const u64 n = op ? J.nb : J.na;
-    u64 v[HGX_EPT];
 #pragma unroll
-    for (int j = 0; j < HGX_EPT; j++) {
-      const int x = tid + HGX_NT * j;
-      const u64 pos = gbase + (x & 15) + rs * (u64)(x >> 4);
-      v[j] = (pos < n) ? limbs[pos] : 0;
-    }
-    for (int k = 0; k < 3; k++) {
-      const Mod M = mod_of(ps, k);
-#pragma unroll
-      for (int j = 0; j < HGX_EPT; j++) {
-        const int x = tid + HGX_NT * j;
-        lds[HGX_PAD(x)] = mod_reduce64(v[j], M);
-      }
-      __syncthreads();
-      tile_forward<8, true>(lds, 4, ps.tw_fwd[k], M);
-      u64 *dst = res + (u64)k * L;
-#pragma unroll
-      for (int j = 0; j < HGX_EPT; j++) {
-        const int x = tid + HGX_NT * j;
-        const u32 m = x >> 4;
-        const u64 t = t0 + (x & 15);
-        dst[gbase + (x & 15) + rs * m] = apply_twiddle<DIRECT>(lds[HGX_PAD(x)], tw, k, t, bitrev8(m), M);
-      }
-      __syncthreads();
+    for (int j = 0; j < 16; j++) {
+      const u64 pos = gbase + rs * (u64)(g + 16 * j);
+      x[j] = mod_reduce64((pos < n) ? limbs[pos] : 0, M);
     }
   } else {
-    const int k = kbase + blockIdx.y;
-    const Mod M = mod_of(ps, k);
-    u64 *arr = res + (u64)blockIdx.y * L;
 #pragma unroll
-    for (int j = 0; j < HGX_EPT; j++) {
-      const int x = tid + HGX_NT * j;
-      lds[HGX_PAD(x)] = arr[gbase + (x & 15) + rs * (u64)(x >> 4)];
+    for (int j = 0; j < 16; j++) x[j] = arr[gbase + rs * (u64)(g + 16 * j)];
+  }
+  round_regs<8, 0, 4, false>(x, g, ps.tw_fwd[k], M);
+#pragma unroll
+  for (int j = 0; j < 16; j++) lds[l1 + 272 * j] = x[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 16; j++) x[j] = lds[l2 + 17 * j];
+  round_regs<8, 4, 4, false>(x, 0, ps.tw_fwd[k], M);
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const u32 m = 16 * g + j;
+    arr[gbase + rs * m] = apply_twiddle<DIRECT>(x[j], tw, k, t0 + t, bitrev8(m), M);
+  }
+}
+
+// Round I of a row transform (rows of 2^S_BITS words, row t of the tile at
+// index t << S_BITS): group gid is row t, elements base + j*D.
+template <int S_BITS, int I>
+struct RowRound {
+  static constexpr int A = RoundPlan<S_BITS>::start(I);
+  static constexpr int R = RoundPlan<S_BITS>::bits(I);
+  static constexpr int RR = 1 << R;
+  static constexpr int D = (1 << S_BITS) >> (A + R);
+  static constexpr int LD = Log2<D>::value;
+  static constexpr int GPC_BITS = S_BITS - R;
+  static __device__ __forceinline__ void where(int gid, int &row, int &goff, int &base) {
+    const int g = gid & ((1 << GPC_BITS) - 1);
+    row = (gid >> GPC_BITS) << S_BITS;
+    goff = g & (D - 1);
+    base = ((g >> LD) << (LD + R)) + goff;
+  }
+};
+
+// Forward rounds I .. I1 in LDS on one or two tiles, a barrier after each.
+template <int S_BITS, int I, int I1, bool TWO>
+struct FwdRange {
+  static __device__ __forceinline__ void run(u64 *ldsA, u64 *ldsB, int wbits, const u64 *tw, const Mod &M) {
+    if constexpr (I <= I1) {
+      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), false, false>(ldsA, wbits, tw, M);
+      if (TWO) tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), false, false>(ldsB, wbits, tw, M);
+      __syncthreads();
+      FwdRange<S_BITS, I + 1, I1, TWO>::run(ldsA, ldsB, wbits, tw, M);
+    }
+  }
+};
+// Inverse rounds I down to I0 in LDS, a barrier after each.
+template <int S_BITS, int I, int I0>
+struct InvRange {
+  static __device__ __forceinline__ void run(u64 *lds, int wbits, const u64 *tw, const Mod &M) {
+    if constexpr (I >= I0) {
+      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), true, false>(lds, wbits, tw, M);
+      __syncthreads();
+      InvRange<S_BITS, I - 1, I0>::run(lds, wbits, tw, M);
+    }
+  }
+};
+
+// Last forward pass of both operands (or the one operand of a square), the
+// pointwise product a*b/R (REDC) and the first inverse pass, on one tile of
+// rows.  Round 0 reads HBM and the last inverse round writes it; the last
+// forward round, the product and the first inverse round run per group in
+// registers, so the operand tiles meet in LDS only between rounds.
+template <int S_BITS, bool TWO>
+__device__ __forceinline__ void row_mul_tile(u64 *A, const u64 *Bp, u64 *ldsA, u64 *ldsB, int wbits, const u64 *twf,
+                                             const u64 *twi, const Mod &M) {
+  constexpr int NR = RoundPlan<S_BITS>::NR;
+  using R0 = RowRound<S_BITS, 0>;
+  using RL = RowRound<S_BITS, NR - 1>;
+  const int tid = threadIdx.x;
+  if constexpr (NR == 1) {
+#pragma unroll 1
+    for (int gid = tid; gid < (1 << (wbits + R0::GPC_BITS)); gid += HGX_NT) {
+      int row, goff, base;
+      R0::where(gid, row, goff, base);
+      u64 a[R0::RR], b[R0::RR];
+#pragma unroll
+      for (int j = 0; j < R0::RR; j++) a[j] = A[row + base + j * R0::D];
+      round_regs<S_BITS, 0, R0::R, false>(a, goff, twf, M);
+      if (TWO) {
+#pragma unroll
+        for (int j = 0; j < R0::RR; j++) b[j] = Bp[row + base + j * R0::D];
+        round_regs<S_BITS, 0, R0::R, false>(b, goff, twf, M);
+      }
+#pragma unroll
+      for (int j = 0; j < R0::RR; j++) a[j] = mont_mul(a[j], TWO ? b[j] : a[j], M);
+      round_regs<S_BITS, 0, R0::R, true>(a, goff, twi, M);
+#pragma unroll
+      for (int j = 0; j < R0::RR; j++) A[row + base + j * R0::D] = a[j];
+    }
+  } else {
+    // forward round 0 from HBM
+#pragma unroll 1
+    for (int gid = tid; gid < (1 << (wbits + R0::GPC_BITS)); gid += HGX_NT) {
+      int row, goff, base;
+      R0::where(gid, row, goff, base);
+      u64 x[R0::RR];
+#pragma unroll
+      for (int j = 0; j < R0::RR; j++) x[j] = A[row + base + j * R0::D];
+      round_regs<S_BITS, 0, R0::R, false>(x, goff, twf, M);
+#pragma unroll
+      for (int j = 0; j < R0::RR; j++) ldsA[HGX_PAD(row + base + j * R0::D)] = x[j];
+      if (TWO) {
+#pragma unroll
+        for (int j = 0; j < R0::RR; j++) x[j] = Bp[row + base + j * R0::D];
+        round_regs<S_BITS, 0, R0::R, false>(x, goff, twf, M);
+#pragma unroll
+        for (int j = 0; j < R0::RR; j++) ldsB[HGX_PAD(row + base + j * R0::D)] = x[j];
+      }
     }
     __syncthreads();
-    tile_forward<8, true>(lds, 4, ps.tw_fwd[k], M);
+    if constexpr (NR > 2) FwdRange<S_BITS, 1, NR - 2, TWO>::run(ldsA, ldsB, wbits, twf, M);
+    // last forward round, product, first inverse round
+#pragma unroll 1
+    for (int gid = tid; gid < (1 << (wbits + RL::GPC_BITS)); gid += HGX_NT) {
+      int row, goff, base;
+      RL::where(gid, row, goff, base);
+      u64 a[RL::RR], b[RL::RR];
 #pragma unroll
-    for (int j = 0; j < HGX_EPT; j++) {
-      const int x = tid + HGX_NT * j;
-      const u32 m = x >> 4;
-      const u64 t = t0 + (x & 15);
-      arr[gbase + (x & 15) + rs * m] = apply_twiddle<DIRECT>(lds[HGX_PAD(x)], tw, k, t, bitrev8(m), M);
+      for (int j = 0; j < RL::RR; j++) a[j] = ldsA[HGX_PAD(row + base + j * RL::D)];
+      round_regs<S_BITS, RL::A, RL::R, false>(a, goff, twf, M);
+      if (TWO) {
+#pragma unroll
+        for (int j = 0; j < RL::RR; j++) b[j] = ldsB[HGX_PAD(row + base + j * RL::D)];
+        round_regs<S_BITS, RL::A, RL::R, false>(b, goff, twf, M);
+      }
+#pragma unroll
+      for (int j = 0; j < RL::RR; j++) a[j] = mont_mul(a[j], TWO ? b[j] : a[j], M);
+      round_regs<S_BITS, RL::A, RL::R, true>(a, goff, twi, M);
+#pragma unroll
+      for (int j = 0; j < RL::RR; j++) ldsA[HGX_PAD(row + base + j * RL::D)] = a[j];
+    }
+    __syncthreads();
+    if constexpr (NR > 2) InvRange<S_BITS, NR - 2, 1>::run(ldsA, wbits, twi, M);
+    // inverse round 0 to HBM
+#pragma unroll 1
+    for (int gid = tid; gid < (1 << (wbits + R0::GPC_BITS)); gid += HGX_NT) {
+      int row, goff, base;
+      R0::where(gid, row, goff, base);
+      u64 x[R0::RR];
+#pragma unroll
+      for (int j = 0; j < R0::RR; j++) x[j] = ldsA[HGX_PAD(row + base + j * R0::D)];
+      round_regs<S_BITS, 0, R0::R, true>(x, goff, twi, M);
+#pragma unroll
+      for (int j = 0; j < R0::RR; j++) A[row + base + j * R0::D] = x[j];
     }
   }
 }
@@ -323,26 +465,18 @@ __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRe
     const int k = rp.kbase + blockIdx.y;
     const Mod M = mod_of(ps, k);
     u64 *A = J.res0 + (u64)blockIdx.y * rp.L + gbase;
-    u64 *Bp = TWO ? (J.res1 + (u64)blockIdx.y * rp.L + gbase) : nullptr;
-    for (int x = tid; x < TW; x += HGX_NT) {
-      ldsA[HGX_PAD(x)] = A[x];
-      if (TWO) ldsB[HGX_PAD(x)] = Bp[x];
-    }
-    __syncthreads();
-    if (MODE == MODE_MUL || MODE == MODE_SQR || MODE == MODE_FWD) {
-      tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], M);
-      if (TWO) tile_forward<S_BITS, false>(ldsB, wbits, ps.tw_fwd[k], M);
-    }
-    if (MODE == MODE_MUL || MODE == MODE_SQR) {
-      for (int x = tid; x < TW; x += HGX_NT) {
-        const u64 a = ldsA[HGX_PAD(x)];
-        const u64 b = TWO ? ldsB[HGX_PAD(x)] : a;
-        ldsA[HGX_PAD(x)] = mont_mul(a, b, M);
-      }
+    if constexpr (MODE == MODE_MUL || MODE == MODE_SQR) {
+      const u64 *Bp = TWO ? (J.res1 + (u64)blockIdx.y * rp.L + gbase) : nullptr;
+      row_mul_tile<S_BITS, TWO>(A, Bp, ldsA, ldsB, wbits, ps.tw_fwd[k], ps.tw_inv[k], M);
+    } else {
+      for (int x = tid; x < TW; x += HGX_NT) ldsA[HGX_PAD(x)] = A[x];
       __syncthreads();
+      if (MODE == MODE_FWD)
+        tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], M);
+      else
+        tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], M);
+      for (int x = tid; x < TW; x += HGX_NT) A[x] = ldsA[HGX_PAD(x)];
     }
-    if (MODE != MODE_FWD) tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], M);
-    for (int x = tid; x < TW; x += HGX_NT) A[x] = ldsA[HGX_PAD(x)];
   } else {
     // whole transform in one tile (L <= 4096): limbs in, all three primes in
     // turn, residues of the product out (k_crt finishes)
