@@ -38,17 +38,16 @@
 __device__ __forceinline__ u32 bitrev8(u32 m) { return __builtin_bitreverse32(m) >> 24; }
 
 // x * w_B^(t q) for column t, frequency q: one Shoup product from the direct
-// table (DIRECT), or two from the two-level table.
+// table (DIRECT), or a Shoup and a REDC product from the tile-factored tables.
 template <bool DIRECT>
 __device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u64 t, u32 q, const Mod &M) {
   if constexpr (DIRECT) {
     const ulonglong2 w = *(const ulonglong2 *)(tw.D[k] + 2 * (((u64)q << tw.tbits) + t));
     return shoup_mul(x, w.x, w.y, M);
   } else {
-    const u64 e = t * q;
-    const ulonglong2 lo = *(const ulonglong2 *)(tw.TL[k] + 2 * (e & ((1ull << tw.lb) - 1)));
-    const ulonglong2 hi = *(const ulonglong2 *)(tw.TH[k] + 2 * (e >> tw.lb));
-    return shoup_mul(shoup_mul(x, lo.x, lo.y, M), hi.x, hi.y, M);
+    const ulonglong2 s = *(const ulonglong2 *)(tw.TS[k] + 2 * ((q << 4) + (u32)(t & 15)));
+    const u64 f = tw.TT[k][((t >> 4) << 8) + q];
+    return mont_mul(shoup_mul(x, s.x, s.y, M), f, M);
   }
 }
 

@@ -96,13 +96,19 @@ u64 find_generator(u64 p, u64 c) {
 const u64 kModuli[3] = {0x6280000000000001ull, 0x5700000000000001ull, 0x4180000000000001ull};
 
 struct ColPassTables {
-  int bbits = 0, lb = 0;
-  u64 *TLf[3], *THf[3], *TLi[3], *THi[3], *THi_mul[3], *THi_tr[3];
-  bool direct = false;  // B <= 2^kDirectMaxBits: direct (q, t) tables below
+  int bbits = 0;
+  bool direct = false;  // B <= 2^kDirectMaxBits: direct (q, t) tables
   u64 *Df[3], *Di[3], *Di_mul[3], *Di_tr[3];
+  // otherwise tile-factored: small (q, tt) tables with the scale, tile tables
+  u64 *TSf[3], *TSi[3], *TSi_mul[3], *TSi_tr[3], *TTf[3], *TTi[3];
 };
 
-constexpr int kDirectMaxBits = 20;
+// Column passes on blocks of at most 2^kDirectMaxBits words use the direct
+// twiddle table (16 bytes per word of the block, per prime and direction).
+#ifndef HGX_DIRECT_MAX_BITS
+#define HGX_DIRECT_MAX_BITS 20
+#endif
+constexpr int kDirectMaxBits = HGX_DIRECT_MAX_BITS;
 
 struct Plan {
   int n = 0, ncol = 0, srow = 0;
@@ -223,41 +229,29 @@ int build_plan(int n, Plan **out) {
   plan->srow = rem;
   const u64 L = 1ull << n;
   std::vector<u64> host;
-  struct Off { size_t TLf, THf, TLi, THi, THm, THt, Df, Di, Dm, Dt; };
+  struct Off { size_t Df, Di, Dm, Dt, Sf, Si, Sm, St, Tf, Ti; };
   std::vector<Off> offs(plan->ncol * 3);
   for (int i = 0; i < plan->ncol; i++) {
     const int b = n - 8 * i;
-    const int lb = b / 2, hb = b - lb;
     const bool direct = b <= kDirectMaxBits;
+    const size_t T = (size_t)1 << (b - 8);
     plan->col[i].bbits = b;
-    plan->col[i].lb = lb;
     plan->col[i].direct = direct;
     for (int k = 0; k < 3; k++) {
       const u64 p = kModuli[k];
+      const Mod M{p, 0 - p, (u32)((p - 1) >> 55)};
       const u64 w = powmod(g_ctx.gen[k], (p - 1) >> b, p);
       const u64 wi = invmod(w, p);
-      const u64 wl = powmod(w, 1ull << lb, p), wil = powmod(wi, 1ull << lb, p);
       const u64 Linv = invmod(L % p, p);
       const u64 Rm = (u64)((((u128)1) << 64) % p);
       const u64 kmul = mulmod(Rm, Linv, p);  // R / L: undoes the pointwise REDC's 1/R and scales by 1/L
       const u64 ktr = Linv;                   // 1 / L
       Off &o = offs[i * 3 + k];
       o = Off{};
-      // (value, Shoup companion floor(value * 2^64 / p)) pairs
-      auto push_pow = [&](u64 base, size_t cnt, u64 scale) {
-        size_t at = host.size();
-        u64 x = scale % p;
-        for (size_t j = 0; j < cnt; j++) {
-          host.push_back(x);
-          host.push_back(shoup_companion(x, p));
-          x = mulmod(x, base, p);
-        }
-        return at;
-      };
-      // direct table: row q holds (root^q)^t for t < T, times SCALE
+      // direct table: row q holds SCALE * (root^q)^t for t < T, as (value,
+      // Shoup companion floor(value * 2^64 / p)) pairs
       auto push_direct = [&](u64 root, u64 scale) {
         size_t at = host.size();
-        const size_t T = (size_t)1 << (b - 8);
         u64 rq = 1;
         for (int q = 0; q < 256; q++) {
           u64 x = scale % p;
@@ -270,6 +264,37 @@ int build_plan(int n, Plan **out) {
         }
         return at;
       };
+      // small table: SCALE * root^(tt q) at q*16 + tt, pairs
+      auto push_small = [&](u64 root, u64 scale) {
+        size_t at = host.size();
+        u64 rq = 1;
+        for (int q = 0; q < 256; q++) {
+          u64 x = scale % p;
+          for (int tt = 0; tt < 16; tt++) {
+            host.push_back(x);
+            host.push_back(shoup_companion(x, p));
+            x = mulmod(x, rq, p);
+          }
+          rq = mulmod(rq, root, p);
+        }
+        return at;
+      };
+      // tile table: root^(16 c q) in Montgomery form at c*256 + q, c < T/16
+      // (built with host REDC products: every value stays in Montgomery form)
+      auto push_tile = [&](u64 root) {
+        size_t at = host.size();
+        const u64 r16 = to_mont(powmod(root, 16, p), p);
+        u64 base = to_mont(1, p);
+        for (size_t cidx = 0; cidx < T / 16; cidx++) {
+          u64 x = to_mont(1, p);
+          for (int q = 0; q < 256; q++) {
+            host.push_back(x);
+            x = mont_mul(x, base, M);
+          }
+          base = mont_mul(base, r16, M);
+        }
+        return at;
+      };
       if (direct) {
         o.Df = push_direct(w, 1);
         if (i == 0) {
@@ -279,12 +304,15 @@ int build_plan(int n, Plan **out) {
           o.Di = push_direct(wi, 1);
         }
       } else {
-        o.TLf = push_pow(w, 1ull << lb, 1);
-        o.THf = push_pow(wl, 1ull << hb, 1);
-        o.TLi = push_pow(wi, 1ull << lb, 1);
-        o.THi = push_pow(wil, 1ull << hb, 1);
-        o.THm = (i == 0) ? push_pow(wil, 1ull << hb, kmul) : 0;
-        o.THt = (i == 0) ? push_pow(wil, 1ull << hb, ktr) : 0;
+        o.Sf = push_small(w, 1);
+        o.Tf = push_tile(w);
+        o.Ti = push_tile(wi);
+        if (i == 0) {
+          o.Sm = push_small(wi, kmul);
+          o.St = push_small(wi, ktr);
+        } else {
+          o.Si = push_small(wi, 1);
+        }
       }
     }
   }
@@ -303,12 +331,12 @@ int build_plan(int n, Plan **out) {
           c.Di_mul[k] = (i == 0) ? plan->dev + o.Dm : nullptr;
           c.Di_tr[k] = (i == 0) ? plan->dev + o.Dt : nullptr;
         } else {
-          c.TLf[k] = plan->dev + o.TLf;
-          c.THf[k] = plan->dev + o.THf;
-          c.TLi[k] = plan->dev + o.TLi;
-          c.THi[k] = plan->dev + o.THi;
-          c.THi_mul[k] = plan->dev + o.THm;
-          c.THi_tr[k] = plan->dev + o.THt;
+          c.TSf[k] = plan->dev + o.Sf;
+          c.TSi[k] = (i == 0) ? nullptr : plan->dev + o.Si;
+          c.TSi_mul[k] = (i == 0) ? plan->dev + o.Sm : nullptr;
+          c.TSi_tr[k] = (i == 0) ? plan->dev + o.St : nullptr;
+          c.TTf[k] = plan->dev + o.Tf;
+          c.TTi[k] = plan->dev + o.Ti;
         }
       }
   }
@@ -320,15 +348,14 @@ int build_plan(int n, Plan **out) {
 ColTw coltw(const ColPassTables &c, int which) {
   // which: 0 forward, 1 inverse, 2 inverse with multiply scale, 3 inverse with 1/L
   ColTw t{};
-  t.lb = c.lb;
   t.tbits = c.bbits - 8;
   for (int k = 0; k < 3; k++) {
     if (c.direct) {
       t.D[k] = (which == 0) ? c.Df[k] : (which == 1 ? c.Di[k] : (which == 2 ? c.Di_mul[k] : c.Di_tr[k]));
     } else {
       t.D[k] = nullptr;
-      t.TL[k] = (which == 0) ? c.TLf[k] : c.TLi[k];
-      t.TH[k] = (which == 0) ? c.THf[k] : (which == 1 ? c.THi[k] : (which == 2 ? c.THi_mul[k] : c.THi_tr[k]));
+      t.TS[k] = (which == 0) ? c.TSf[k] : (which == 1 ? c.TSi[k] : (which == 2 ? c.TSi_mul[k] : c.TSi_tr[k]));
+      t.TT[k] = (which == 0) ? c.TTf[k] : c.TTi[k];
     }
   }
   return t;

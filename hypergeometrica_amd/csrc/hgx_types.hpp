@@ -34,13 +34,14 @@ struct PrimeSet {
 // Roots of one column pass on blocks of length B = 2^b (T = B/256 columns).
 // Direct form (B <= 2^20): D[k] holds (w_B^(t q), Shoup companion) pairs at
 // index q*T + t, read coalesced along t, one Shoup product per element.
-// Two-level form: w_B^e = TH[e >> lb] * TL[e & (2^lb - 1)], both tables of
-// (value, companion) pairs, two Shoup products per element.
+// Tile-factored form: w_B^(t q) = w_B^(t0 q) * w_B^(tt q) for the tile's
+// first column t0 = t - tt (tt < 16): TS[k] holds (w_B^(tt q), companion)
+// pairs at q*16 + tt (times the pass's scale), TT[k] holds w_B^(t0 q) in
+// Montgomery form at (t0/16)*256 + q; a Shoup product and a REDC product.
 struct ColTw {
-  const u64 *TL[3];
-  const u64 *TH[3];
   const u64 *D[3];
-  int lb;
+  const u64 *TS[3];
+  const u64 *TT[3];
   int tbits;  // log2 T
 };
 
