@@ -136,32 +136,117 @@ def batched_leg(torch, h, _lib, count: int = 4096, reps: int = 3) -> dict:
             "speedup": round(p / b, 2), "outputs_identical": same}
 
 
-def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 0.5, slots: int = 16) -> dict:
+def latency_leg(torch, h, _lib, n: int = 1 << 16, reps: int = 200) -> dict:
+    """One 2^16 x 2^16-limb product (L = 2^17, seeds 0x48475801/2), REPS
+    calls: host-to-host through hgx_mul (pinned host buffers: limbs in, the
+    product out) and device-resident through hgx_mul_device.  Per-call wall
+    time p50 / p99 and limbs/s."""
+    import ctypes
+
+    L_ = _lib.lib()
+    dev = torch.device("cuda:0")
+    A = torch.empty(n, dtype=torch.int64, device=dev)
+    B = torch.empty(n, dtype=torch.int64, device=dev)
+    O = torch.empty(2 * n, dtype=torch.int64, device=dev)
+    h.fill_splitmix64(A.data_ptr(), n, 0x48475801)
+    h.fill_splitmix64(B.data_ptr(), n, 0x48475802)
+    hA = A.cpu().pin_memory()
+    hB = B.cpu().pin_memory()
+    hO = torch.empty(2 * n, dtype=torch.int64).pin_memory()
+
+    def timed(fn):
+        for _ in range(10):
+            fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        return ts
+
+    def host_call():
+        _lib.check(L_.hgx_mul(ctypes.c_void_p(hA.data_ptr()), n, ctypes.c_void_p(hB.data_ptr()), n,
+                              ctypes.c_void_p(hO.data_ptr())))
+
+    def dev_call():
+        _lib.check(L_.hgx_mul_device(A.data_ptr(), n, B.data_ptr(), n, O.data_ptr(), None))
+        torch.cuda.synchronize()
+
+    th = timed(host_call)
+    td = timed(dev_call)
+    same = bool(torch.equal(hO, O.cpu()))
+
+    def q(ts, f):
+        return round(1e3 * ts[min(len(ts) - 1, int(f * len(ts)))], 4)
+
+    return {"workload": f"one {n} x {n}-limb product (L=2^{(2 * n).bit_length() - 1}), {reps} calls",
+            "host_p50_ms": q(th, 0.5), "host_p99_ms": q(th, 0.99), "host_limbs_per_s": 2 * n / th[len(th) // 2],
+            "device_p50_ms": q(td, 0.5), "device_p99_ms": q(td, 0.99), "device_limbs_per_s": 2 * n / td[len(td) // 2],
+            "outputs_identical": same}
+
+
+def pi_leg(bits: int = 1 << 20) -> dict:
+    """mpd-pi (Chudnovsky binary splitting, Newton reciprocal and inverse
+    square root) to BITS bits with the truncating Newton steps, every product
+    above the 100-limb threshold on the GPU (binary-splitting levels batched),
+    against the same computation with host products; results compared, and
+    the leading 4096 bits checked against mpmath's pi/10."""
+    import mpmath
+
+    from hypergeometrica_amd import mpd as M
+    t0 = time.perf_counter()
+    g = M.Context(truncate=True)
+    rg = g.mpd_pi(bits)
+    tg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rc = M.Context(mul=lambda x, y: x * y, truncate=True).mpd_pi(bits)
+    tc = time.perf_counter() - t0
+    mpmath.mp.prec = 4200
+    ref = int(mpmath.floor(mpmath.pi / 10 * mpmath.mpf(2) ** 4096))
+    sh = -64 * rg.exponent - 4096
+    got = (rg.mantissa >> sh) if sh >= 0 else (rg.mantissa << -sh)
+    return {"workload": f"mpd-pi to 2^{bits.bit_length() - 1} bits, truncating Newton, GPU products vs host products",
+            "gpu_s": round(tg, 3), "host_s": round(tc, 3), "speedup": round(tc / tg, 2),
+            "results_identical": rg == rc, "leading_4096_bits_match": abs(got - ref) <= 1,
+            "products": g.products, "max_product_limbs": g.max_product_limbs}
+
+
+def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 0.5, slots: int = 32, seed: int = 0x48475804,
+                warm: bool = True) -> dict:
     """Request server under load: Poisson arrivals at RATE/s for SECONDS,
-    operand sizes log-uniform in [2^10, 2^24] limbs (seed 0x48475804), operands
-    and products in pinned host memory (operands are slices of two random
+    operand sizes log-uniform in [2^10, 2^24] limbs (SEED), operands and
+    products in pinned host memory (operands are slices of two random
     2^24-limb buffers; SLOTS output buffers bound the requests in flight).
-    Latency = completion seen by the client - scheduled arrival."""
+    The root tables of every transform length in the range are built before
+    the stream starts (one request per length), as a server keeps them
+    resident.  Latency = completion seen by the client - scheduled arrival."""
+    import ctypes
     import math
 
     import numpy as np
     from oracle.oracle import splitmix64_py
     L_ = _lib.lib()
     count = max(1, int(rate * seconds))
-    sizes = log_uniform_sizes(0x48475804, 2 * count, 10, 24)
-    us = splitmix64_py(0x48475805, count)
+    sizes = log_uniform_sizes(seed, 2 * count, 10, 24)
+    us = splitmix64_py(seed + 1, count)
     gaps = [-math.log(1.0 - (u + 0.5) / 2**64) / rate for u in us]
     nmax = 1 << 24
-    rng = np.random.default_rng(0x48475806)
+    rng = np.random.default_rng(seed + 2)
     A = torch.empty(nmax, dtype=torch.int64, pin_memory=True)
     B = torch.empty(nmax, dtype=torch.int64, pin_memory=True)
     A.numpy()[:] = rng.integers(-2**63, 2**63, size=nmax, dtype=np.int64)
     B.numpy()[:] = rng.integers(-2**63, 2**63, size=nmax, dtype=np.int64)
     outs = [torch.empty(2 * nmax, dtype=torch.int64, pin_memory=True) for _ in range(slots)]
+    if warm:
+        for bits in range(10, 25):
+            d = _lib.MulDesc(A.data_ptr(), 1 << bits, B.data_ptr(), 1 << bits, outs[0].data_ptr())
+            t = ctypes.c_uint64(0)
+            _lib.check(L_.hgx_submit(ctypes.byref(d), ctypes.byref(t)))
+            _lib.check(L_.hgx_wait(t))
     free = list(range(slots))
     inflight = {}
     lat, limbs = [], 0
-    import ctypes
     t0 = time.perf_counter()
     sched = t0
 
@@ -197,9 +282,12 @@ def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 0.5, slot
         reap(True)
     wall = time.perf_counter() - t0
     lat.sort()
-    return {"workload": f"Poisson arrivals {rate:.0f}/s for {seconds} s ({count} requests), operand limbs log-uniform "
-                        f"in [2^10, 2^24], pinned host buffers, {slots} output slots",
-            "requests": count, "limbs": limbs, "wall_s": round(wall, 4), "limbs_per_s": limbs / wall,
+    offered = sum(sizes) / seconds
+    return {"workload": f"Poisson arrivals {rate:.0f}/s for {seconds} s ({count} requests, seed {seed:#x}), operand "
+                        f"limbs log-uniform in [2^10, 2^24], pinned host buffers, {slots} output slots, root tables "
+                        f"built before the stream",
+            "requests": count, "limbs": limbs, "offered_limbs_per_s": offered, "wall_s": round(wall, 4),
+            "limbs_per_s": limbs / wall,
             "p50_ms": round(1e3 * lat[len(lat) // 2], 3),
             "p99_ms": round(1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
             "mean_request_limbs": limbs / count}
@@ -215,6 +303,9 @@ def main() -> int:
     ap.add_argument("--cpu-bits", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-server leg")
+    ap.add_argument("--no-latency", action="store_true", help="skip the 2^16 x 2^16 per-call latency leg")
+    ap.add_argument("--no-pi", action="store_true", help="skip the mpd-pi leg")
+    ap.add_argument("--pi-bits", type=int, default=20, help="mpd-pi precision 2^bits")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -315,10 +406,23 @@ def main() -> int:
         print("# batched vs per-call ...", file=sys.stderr, flush=True)
         batched = batched_leg(torch, h, _lib)
 
+    latency = None
+    if not args.no_latency:
+        print("# per-call latency ...", file=sys.stderr, flush=True)
+        latency = latency_leg(torch, h, _lib)
+
     serving = None
     if not args.no_serve:
         print("# serving ...", file=sys.stderr, flush=True)
         serving = serving_leg(torch, h, _lib)
+        print("# serving below saturation ...", file=sys.stderr, flush=True)
+        serving["below_saturation"] = serving_leg(torch, h, _lib, rate=250.0, seconds=2.0, seed=0x48475814,
+                                                  warm=False)
+
+    pi = None
+    if not args.no_pi:
+        print("# mpd-pi ...", file=sys.stderr, flush=True)
+        pi = pi_leg(1 << args.pi_bits)
 
     cpu = None
     if not args.no_cpu:
@@ -346,7 +450,9 @@ def main() -> int:
         "latency_ms": {"p50": round(1e3 * p50, 3), "p99": round(1e3 * p99, 3)},
         "kernels": kernels,
         "batched": batched,
+        "latency_2e16": latency,
         "serving": serving,
+        "mpd_pi": pi,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

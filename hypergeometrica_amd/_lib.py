@@ -5,6 +5,7 @@ every call raises HgxError.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import threading
@@ -89,15 +90,31 @@ def check(rc: int) -> int:
     return rc
 
 
+_exit_hook = False
+
+
+def _shutdown_at_exit() -> None:
+    """Release the library (request server, streams, device memory) while the
+    HIP runtime is still fully up: Python's exit hooks run before the C
+    runtime's and the loaded libraries' finalizers."""
+    global _initialised
+    if _initialised and _lib is not None:
+        _initialised = False
+        _lib.hgx_shutdown()
+
+
 def lib(device: int = 0, budget: int = 0) -> ctypes.CDLL:
     """The library, initialised on DEVICE (once per process)."""
-    global _initialised
+    global _initialised, _exit_hook
     L = load()
     if not _initialised:
         with _init_lock:
             if not _initialised:
                 check(L.hgx_init(device, budget))
                 _initialised = True
+                if not _exit_hook:
+                    atexit.register(_shutdown_at_exit)
+                    _exit_hook = True
     return L
 
 

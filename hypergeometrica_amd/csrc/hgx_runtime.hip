@@ -869,8 +869,11 @@ void server_loop() {
   }
 }
 
-void server_stop();
-void server_stop_at_exit() { server_stop(); }
+void server_stop(bool at_exit = false);
+// At process exit the HIP runtime (and any tool loaded into the process) may
+// already be finalized, so only the server thread is stopped and joined; the
+// device memory, streams and events go with the process.
+void server_stop_at_exit() { server_stop(true); }
 
 int server_start_locked() {
   if (g_srv.started) return HGX_OK;
@@ -895,7 +898,7 @@ int server_start_locked() {
   return HGX_OK;
 }
 
-void server_stop() {
+void server_stop(bool at_exit) {
   if (!g_srv.started) return;
   {
     std::lock_guard<std::mutex> lk(g_srv.mu);
@@ -903,6 +906,10 @@ void server_stop() {
   }
   g_srv.cv.notify_all();
   g_srv.th.join();
+  if (at_exit) {
+    g_srv.started = false;
+    return;
+  }
   for (int i = 0; i < kServeStreams; i++) {
     (void)hipStreamSynchronize(g_srv.streams[i]);
     (void)hipStreamDestroy(g_srv.streams[i]);

@@ -228,7 +228,7 @@ def test_mpd_pi_gpu_bit_identical_to_cpu_routing(hgx, bits, truncate):
 @pytest.mark.parametrize("k", [4096, 6000, 1 << 17, 1 << 21])
 def test_product_that_is_all_ones(hgx, k):
     # (2^(32k) - 1)(2^(32k) + 1) = 2^(64k) - 1: every limb of the product is
-    # all ones, so carries cross every run and every 2048-limb block
+    # all ones, so carries cross every run and every 1024-limb CRT block
     half = k // 2
     a = np.full(half, 2**64 - 1, dtype=np.uint64)
     b = np.zeros(half + 1, dtype=np.uint64)
