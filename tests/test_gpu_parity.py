@@ -247,3 +247,42 @@ def test_all_ones_uneven(hgx, oracle, na, nb):
     # (2^A - 1)(2^B - 1) = 2^(A+B) - 2^A - 2^B + 1
     A, B = 64 * na, 64 * nb
     assert oracle.limbs_to_int(got) == (1 << (A + B)) - (1 << A) - (1 << B) + 1
+
+
+# ---------------------------------------------------------------- four passes
+
+def test_four_pass_transform_impulse(hgx, oracle):
+    # L = 2^29: column passes of 8 + 8 + 8 bits and a 5-bit row pass, both
+    # twiddle forms.  The inverse transform of the impulse at bit-reversed
+    # slot k0 is x_j = L^-1 w^(-j k0) (checked at sampled j against Python's
+    # pow); the forward transform must give the impulse back.
+    n, k = 29, 1
+    L = 1 << n
+    p = oracle.moduli()[k]
+    w = pow(oracle.generator(p), (p - 1) >> n, p)
+    winv, Linv = pow(w, -1, p), pow(L, -1, p)
+    k0 = 0x12345679 % L
+    say("impulse transform 2^29")
+    e = np.zeros(L, dtype=np.uint64)
+    e[oracle.bitrev(k0, n)] = 1
+    x = hgx.ntt_reverse(e, k)
+    rng = np.random.default_rng(0x48475829)
+    for j in [0, 1, 2, L - 1] + [int(v) for v in rng.integers(0, L, 300)]:
+        assert int(x[j]) == Linv * pow(winv, (j * k0) % (p - 1), p) % p, j
+    del e
+    y = hgx.ntt_forward(x, k)
+    assert int(y[oracle.bitrev(k0, n)]) == 1 and np.count_nonzero(y) == 1
+
+
+def test_four_pass_product_checksum(hgx, oracle):
+    # 2^28 x 2^28 limbs, L = 2^29 (four passes): residue mod 2^61-1 and the
+    # exact low limbs, as for the 2^26 product
+    n = 1 << 28
+    say("checksum 2^28 x 2^28")
+    a = oracle.splitmix64(0x48475811, n)
+    b = oracle.splitmix64(0x48475812, n)
+    got = hgx.mul_limbs(a, b)
+    assert mod_q(got) == (mod_q(a) * mod_q(b)) % Q
+    lo = (oracle.limbs_to_int(a[:4]) * oracle.limbs_to_int(b[:4])) % (1 << 256)
+    assert oracle.limbs_to_int(got[:4]) == lo
+    assert got[-1] != 0 or got[-2] != 0
