@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+VALU_PEAK = 1024 * 2.4e9 / 4  # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz, a wave64 VALU instruction per 4 cycles
 METRIC = "NTT-multiply limbs/s batched vs per-call on GPU; %HBM roofline; p99 latency"
 
 
@@ -387,6 +388,19 @@ def main() -> int:
             traffic = json.load(open(tpath)).get("bytes_per_launch")
         except Exception:
             traffic = None
+    # integer-VALU roofline of the same kernel: VALU wave-instructions per launch
+    # (SQ_INSTS_VALU, committed counter profile) over the live launch time,
+    # against one wave-instruction per 4 cycles on each of the 1024 SIMDs
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", f"valu_{dom}.json")
+    if os.path.exists(vpath):
+        try:
+            vi = json.load(open(vpath))["valu_wave_insts_per_launch"]
+            peak = VALU_PEAK
+            valu = {"kernel": dom, "wave_insts_per_launch": vi, "achieved": vi / (d["avg_ms"] * 1e-3), "peak": peak,
+                    "unit": "wave-instructions/s", "frac": round(vi / (d["avg_ms"] * 1e-3) / peak, 4)}
+        except Exception:
+            valu = None
     roofline = {"bound": "hbm", "kernel": dom,
                 "achieved": round(achieved / 1e9, 1) if achieved else None,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -446,6 +460,7 @@ def main() -> int:
                                f"L=2^{L.bit_length() - 1}, 3 primes, operands resident in HBM",
                    "limbs_per_step": limbs, "transform_length": L},
         "roofline": roofline,
+        "valu_roofline": valu,
         "multiply_roofline": {"alg_bytes": b_mul, "frac": round(b_mul / (ms_per_step * 1e-3) / HBM_PEAK, 4)},
         "latency_ms": {"p50": round(1e3 * p50, 3), "p99": round(1e3 * p99, 3)},
         "kernels": kernels,
