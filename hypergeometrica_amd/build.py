@@ -26,7 +26,8 @@ HIPCC_FLAGS = [
 ]
 
 # kernel families compile in parallel; the runtime holds no device code
-SOURCES = ["hgx_runtime.hip", "hgx_k_col.hip", "hgx_k_row_mul.hip", "hgx_k_row_sqr.hip", "hgx_k_row_tr.hip"]
+SOURCES = ["hgx_runtime.hip", "hgx_k_col.hip", "hgx_k_row_mul.hip", "hgx_k_row_sqr.hip", "hgx_k_row_tr.hip",
+           "hgx_k_arith.hip"]
 OBJDIR = os.path.join(HERE, "_build")
 
 

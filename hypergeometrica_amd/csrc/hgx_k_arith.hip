@@ -1,0 +1,120 @@
+// hgx_k_arith.hip -- magnitude addition and subtraction of device-resident
+// limb vectors, the additions between the products of a Newton iteration
+// (mpz-+ / mpz-- on magnitudes, src/mpz-ram.lisp:185-291).
+//
+// A limb's carry (or borrow) out is a map of its carry in: kill (0),
+// propagate (2) or generate (3) -- the same last-non-propagate scan as the
+// product's carry resolution.  Three launches: the composed map of every
+// block of 1024 limbs, a scan over the blocks, and the final pass that
+// recomputes each limb with its incoming carry.
+#include "hgx_kernels.hpp"
+#include "hgx_launch.hpp"
+
+namespace {
+
+constexpr int kAsPerThread = 4;
+constexpr int kAsBlock = HGX_NT * kAsPerThread;  // limbs per workgroup
+
+__device__ __forceinline__ u64 limb(const u64 *x, u64 nx, u64 i) { return i < nx ? x[i] : 0; }
+
+// raw limb result and its carry map
+template <bool SUB>
+__device__ __forceinline__ u64 limb_op(u64 x, u64 y, u32 &m) {
+  if (SUB) {
+    const u64 d = x - y;
+    m = (x < y) ? 3u : (d == 0 ? 2u : 0u);
+    return d;
+  }
+  const u64 s = x + y;
+  m = (s < x) ? 3u : (s == ~0ull ? 2u : 0u);
+  return s;
+}
+
+template <bool SUB>
+__global__ __launch_bounds__(HGX_NT) void k_as_blocks(const u64 *a, u64 na, const u64 *b, u64 nb, u64 n, uint8_t *agg) {
+  __shared__ u32 sh[HGX_NT];
+  const u64 i0 = (u64)blockIdx.x * kAsBlock + (u64)threadIdx.x * kAsPerThread;
+  u32 summary = 2u;
+#pragma unroll
+  for (int j = 0; j < kAsPerThread; j++) {
+    const u64 i = i0 + j;
+    if (i < n) {
+      u32 m;
+      (void)limb_op<SUB>(limb(a, na, i), limb(b, nb, i), m);
+      summary = map_combine(summary, m);
+    }
+  }
+  sh[threadIdx.x] = summary;
+  __syncthreads();
+  for (int s = 1; s < HGX_NT; s <<= 1) {
+    if ((threadIdx.x & (2 * s - 1)) == 0) sh[threadIdx.x] = map_combine(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) agg[blockIdx.x] = (uint8_t)sh[0];
+}
+
+// carry into every block; one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_as_scan(const uint8_t *agg, uint8_t *blkin, u64 nblocks) {
+  __shared__ u32 sh[1024];
+  const u64 chunk = (nblocks + 1023) / 1024;
+  const u64 b0 = (u64)threadIdx.x * chunk;
+  u64 b1 = b0 + chunk;
+  if (b1 > nblocks) b1 = nblocks;
+  u32 summary = 2u;
+  for (u64 b = b0; b < b1; b++) summary = map_combine(summary, agg[b]);
+  u32 state = block_exclusive_last(summary, sh);
+  for (u64 b = b0; b < b1; b++) {
+    blkin[b] = (state == 3u) ? 1 : 0;
+    state = map_combine(state, agg[b]);
+  }
+}
+
+// out[i] = a[i] +- b[i] +- carry in, i < n; an add also writes the final carry to out[n]
+template <bool SUB>
+__global__ __launch_bounds__(HGX_NT) void k_as_apply(const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out, u64 n,
+                                                      const uint8_t *blkin) {
+  __shared__ u32 sh[HGX_NT];
+  const u64 i0 = (u64)blockIdx.x * kAsBlock + (u64)threadIdx.x * kAsPerThread;
+  u64 r[kAsPerThread];
+  u32 m[kAsPerThread];
+  u32 summary = 2u;
+#pragma unroll
+  for (int j = 0; j < kAsPerThread; j++) {
+    const u64 i = i0 + j;
+    r[j] = 0;
+    m[j] = 2u;
+    if (i < n) {
+      r[j] = limb_op<SUB>(limb(a, na, i), limb(b, nb, i), m[j]);
+      summary = map_combine(summary, m[j]);
+    }
+  }
+  const u32 ex = block_exclusive_last(summary, sh);
+  u64 c = (ex == 3u) ? 1u : (ex == 2u ? (u64)blkin[blockIdx.x] : 0u);
+#pragma unroll
+  for (int j = 0; j < kAsPerThread; j++) {
+    const u64 i = i0 + j;
+    if (i < n) {
+      out[i] = SUB ? r[j] - c : r[j] + c;
+      c = (m[j] == 2u) ? c : (m[j] == 3u ? 1u : 0u);
+      if (!SUB && i == n - 1) out[n] = c;
+    }
+  }
+}
+
+}  // namespace
+
+void hgx_launch_addsub(bool sub, hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out, u64 n,
+                       uint8_t *agg, uint8_t *blkin) {
+  const unsigned nblk = (unsigned)((n + kAsBlock - 1) / kAsBlock);
+  if (sub) {
+    hipLaunchKernelGGL(k_as_blocks<true>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, n, agg);
+    hipLaunchKernelGGL(k_as_scan, dim3(1), dim3(1024), 0, s, agg, blkin, (u64)nblk);
+    hipLaunchKernelGGL(k_as_apply<true>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, out, n, blkin);
+  } else {
+    hipLaunchKernelGGL(k_as_blocks<false>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, n, agg);
+    hipLaunchKernelGGL(k_as_scan, dim3(1), dim3(1024), 0, s, agg, blkin, (u64)nblk);
+    hipLaunchKernelGGL(k_as_apply<false>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, out, n, blkin);
+  }
+}
+
+u64 hgx_addsub_blocks(u64 n) { return (n + kAsBlock - 1) / kAsBlock; }

@@ -132,10 +132,13 @@ struct Context {
   u64 gen[3]{};
   u64 *tables = nullptr;
   std::map<int, std::unique_ptr<Plan>> plans;
-  Workspace ws, stage, jobs_dev, pack;
+  Workspace ws, stage, jobs_dev, pack, scan;
   MulJob *jobs_host = nullptr;  // pinned job-table staging
   size_t jobs_host_cap = 0;
   hipEvent_t jobs_copied = nullptr;
+  // captured launch sequences of batched chunks (hipGraph), keyed by shape
+  bool graphs_on = true;
+  std::map<std::vector<u64>, hipGraphExec_t> graphs;
   std::recursive_mutex mu;
 };
 
@@ -596,6 +599,39 @@ int mul_device_locked(const u64 *d_a, size_t na, const u64 *d_b, size_t nb, u64 
   return ws_release(s);
 }
 
+// Replay the captured launch sequence for KEY on stream S, capturing it on
+// first use (PREPARE runs first, outside the capture).  The job table the
+// kernels read is copied before every replay, so a graph depends only on the
+// shape of the chunk and on the table and work-memory addresses in KEY.
+template <class Prepare, class Enqueue>
+int launch_graph(const std::vector<u64> &key, hipStream_t s, Prepare prepare, Enqueue enqueue) {
+  auto it = g_ctx.graphs.find(key);
+  if (it == g_ctx.graphs.end()) {
+    int rc = prepare();
+    if (rc) return rc;
+    if (g_ctx.graphs.size() >= 64) {  // bounded cache
+      for (auto &kv : g_ctx.graphs) (void)hipGraphExecDestroy(kv.second);
+      g_ctx.graphs.clear();
+    }
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = enqueue();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(HGX_EDEVICE, std::string("graph capture failed: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) return fail(HGX_EDEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(e2));
+    it = g_ctx.graphs.emplace(key, exec).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, s));
+  return HGX_OK;
+}
+
 // Batched products of device operands: grouped by (transform length,
 // square), one launch sequence per group over a device job table.  Groups
 // whose work memory exceeds the budget are split.
@@ -648,16 +684,42 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
     MulJob *table = (MulJob *)g_ctx.jobs_dev.ptr;
     HIPCHK(hipMemcpyAsync(table, g_ctx.jobs_host, (end - pos) * sizeof(MulJob), hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(g_ctx.jobs_copied, s));
-    size_t g = pos;
-    while (g < end) {
+    // the chunk's groups: (length, square, count) runs of the sorted items
+    std::vector<u64> key{(u64)(uintptr_t)table, (u64)(uintptr_t)g_ctx.ws.ptr};
+    for (size_t g = pos; g < end;) {
       size_t h = g;
       while (h < end && items[h].n == items[g].n && items[h].sqr == items[g].sqr) h++;
-      JobRef jr{};
-      jr.table = table + (g - pos);
-      rc = enqueue_group(jr, (unsigned)(h - g), items[g].n, items[g].sqr, s);
-      if (rc) return rc;
+      key.push_back(((u64)items[g].n << 1) | (items[g].sqr ? 1u : 0u));
+      key.push_back(h - g);
       g = h;
     }
+    auto enqueue_chunk = [&]() -> int {
+      size_t g = pos;
+      while (g < end) {
+        size_t h = g;
+        while (h < end && items[h].n == items[g].n && items[h].sqr == items[g].sqr) h++;
+        JobRef jr{};
+        jr.table = table + (g - pos);
+        int r = enqueue_group(jr, (unsigned)(h - g), items[g].n, items[g].sqr, s);
+        if (r) return r;
+        g = h;
+      }
+      return (int)HGX_OK;
+    };
+    if (g_ctx.graphs_on && !g_timing) {
+      rc = launch_graph(key, s, [&]() -> int {
+        // root tables first: building them is host work and copies
+        for (size_t i = pos; i < end; i++) {
+          Plan *pl = nullptr;
+          int r = build_plan(items[i].n, &pl);
+          if (r) return r;
+        }
+        return (int)HGX_OK;
+      }, enqueue_chunk);
+    } else {
+      rc = enqueue_chunk();
+    }
+    if (rc) return rc;
     pos = end;
     if (pos < items.size()) {
       // the next chunk reuses the work memory
@@ -992,6 +1054,8 @@ void hgx_shutdown(void) {
   hipStreamSynchronize(g_ctx.stream);
   for (auto &kv : g_ctx.plans) dev_free(kv.second->dev, kv.second->bytes);
   g_ctx.plans.clear();
+  for (auto &kv : g_ctx.graphs) (void)hipGraphExecDestroy(kv.second);
+  g_ctx.graphs.clear();
   dev_free(g_ctx.ws.ptr, g_ctx.ws.bytes);
   g_ctx.ws = Workspace{};
   dev_free(g_ctx.stage.ptr, g_ctx.stage.bytes);
@@ -1000,6 +1064,8 @@ void hgx_shutdown(void) {
   g_ctx.jobs_dev = Workspace{};
   dev_free(g_ctx.pack.ptr, g_ctx.pack.bytes);
   g_ctx.pack = Workspace{};
+  dev_free(g_ctx.scan.ptr, g_ctx.scan.bytes);
+  g_ctx.scan = Workspace{};
   if (g_ctx.jobs_host) (void)hipHostFree(g_ctx.jobs_host);
   g_ctx.jobs_host = nullptr;
   g_ctx.jobs_host_cap = 0;
@@ -1182,6 +1248,36 @@ int hgx_ntt_reverse(uint64_t *data, size_t n, int mod_num) {
   return ws_release(s);
 }
 
+// magnitude sum / difference of device limb vectors (mpz-+ / mpz-- on the
+// magnitudes, src/mpz-ram.lisp:185-291)
+static int addsub_device(bool sub, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, uint64_t *d_out,
+                         void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!d_out || (na && !d_a) || (nb && !d_b)) return fail(HGX_EINVAL, "null operand or output");
+  hipStream_t s = stream ? (hipStream_t)stream : g_ctx.stream;
+  const u64 n = std::max(na, nb);
+  if (n == 0) return fail(HGX_EINVAL, "both operands empty");
+  const u64 nblk = hgx_addsub_blocks(n);
+  rc = ws_acquire(s);
+  if (rc) return rc;
+  rc = ensure(g_ctx.scan, 2 * nblk);
+  if (rc) return rc;
+  uint8_t *agg = (uint8_t *)g_ctx.scan.ptr;
+  hgx_launch_addsub(sub, s, d_a, na, d_b, nb, d_out, n, agg, agg + nblk);
+  HIPCHK(hipGetLastError());
+  return ws_release(s);
+}
+
+int hgx_add_device(const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, uint64_t *d_out, void *stream) {
+  return addsub_device(false, d_a, na, d_b, nb, d_out, stream);
+}
+
+int hgx_sub_device(const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, uint64_t *d_out, void *stream) {
+  return addsub_device(true, d_a, na, d_b, nb, d_out, stream);
+}
+
 int hgx_fill_splitmix64(uint64_t *d_out, size_t n, uint64_t seed, int force_top, void *stream) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   int rc = check_ready();
@@ -1241,6 +1337,12 @@ int hgx_wait(hgx_ticket ticket) {
   g_srv.cv.wait(lk, [&] { return r->state == 2 || r->state < 0; });
   g_srv.tickets.erase(ticket);
   if (r->state < 0) return fail(r->state, r->err);
+  return HGX_OK;
+}
+
+int hgx_graphs_enable(int on) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  g_ctx.graphs_on = on != 0;
   return HGX_OK;
 }
 

@@ -111,11 +111,32 @@ int hgx_submit(const hgx_mul_desc *desc, hgx_ticket *ticket);
 int hgx_poll(hgx_ticket ticket);
 int hgx_wait(hgx_ticket ticket);
 
+// Magnitude sum and difference of device limb vectors, the additions
+// between the products of a Newton iteration kept in device memory
+// (mpz-+ and mpz-- on the magnitudes of mpz/ram, src/mpz-ram.lisp:185-291).
+// N = max(NA, NB) > 0; the shorter operand reads as zero-extended.
+// hgx_add_device writes N + 1 limbs (the last is the carry out);
+// hgx_sub_device writes N limbs of A - B and requires A >= B.  D_OUT may be
+// D_A or D_B.  Ordered on STREAM (null: the library's stream) with the
+// library's other calls.
+int hgx_add_device(const uint64_t *d_a, size_t na, const uint64_t *d_b,
+                   size_t nb, uint64_t *d_out, void *stream);
+int hgx_sub_device(const uint64_t *d_a, size_t na, const uint64_t *d_b,
+                   size_t nb, uint64_t *d_out, void *stream);
+
 // Device SplitMix64 fill for benchmarks and tests:
 // d_out[i] = mix(seed + (i + 1) * 0x9E3779B97F4A7C15); when FORCE_TOP is
 // non-zero the last limb is forced non-zero.
 int hgx_fill_splitmix64(uint64_t *d_out, size_t n, uint64_t seed,
                         int force_top, void *stream);
+
+// Batched calls (hgx_mul_batch, hgx_mul_batch_device) capture the launch
+// sequence of each chunk in a HIP graph, keyed by its shape (the transform
+// lengths and counts of its groups) and by the library's job-table and
+// work-memory addresses, and replay it when the same shape comes again.
+// No counterpart in the reference (batching infrastructure).  ON = 0 launches
+// every kernel directly; the outputs are the same.  Default: on.
+int hgx_graphs_enable(int on);
 
 // Per-kernel device timing of later calls, for benchmarks: HIP events are
 // recorded around every kernel on the stream it is launched on.  Plays the

@@ -23,6 +23,11 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def hgx():
+    # torch first: it ships its own HIP runtime, and a process that loads
+    # libhgx.so first binds torch to the library's runtime instead, which
+    # torch cannot use; loaded in this order both share torch's
+    import torch  # noqa: F401
+
     import hypergeometrica_amd as h
     from hypergeometrica_amd import _lib
     _lib.lib()  # initialise device 0; raises if the library or device is missing

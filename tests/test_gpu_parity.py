@@ -286,3 +286,100 @@ def test_four_pass_product_checksum(hgx, oracle):
     lo = (oracle.limbs_to_int(a[:4]) * oracle.limbs_to_int(b[:4])) % (1 << 256)
     assert oracle.limbs_to_int(got[:4]) == lo
     assert got[-1] != 0 or got[-2] != 0
+
+
+# ---------------------------------------------------------------- Newton in HBM
+
+@pytest.mark.parametrize("bits,truncate", [(1000, False), (8000, False), (20000, True), (1 << 18, True)])
+def test_device_newton_equals_host(hgx, bits, truncate):
+    # mpd-reciprocal and mpd-inv-sqrt with every iterate in device memory give
+    # the integers of the host iteration (same steps, exact arithmetic)
+    import random
+
+    from hypergeometrica_amd import mpd as M
+    from hypergeometrica_amd import mpd_device as D
+    rnd = random.Random(bits)
+    host = M.Context(mul=lambda x, y: x * y, truncate=truncate)
+    dev = D.DeviceContext(truncate=truncate)
+    for mant, exp in [(rnd.getrandbits(bits // 2) | 1, -3), (-(rnd.getrandbits(bits // 3) | 1), 2), (10005, 0)]:
+        a = M.MPD(mant, exp)
+        say("device newton", bits, truncate, a.mantissa.bit_length())
+        assert dev.mpd_reciprocal(a, bits) == host.mpd_reciprocal(a, bits)
+        if mant > 0:
+            assert dev.mpd_inv_sqrt(a, bits) == host.mpd_inv_sqrt(a, bits)
+            assert dev.mpd_sqrt(a, bits) == host.mpd_sqrt(a, bits)
+
+
+def test_device_add_sub_carry_chains(hgx):
+    # magnitude add / subtract across blocks of 1024 limbs: all-ones runs,
+    # borrows through zero runs, uneven lengths, in place
+    import torch
+    from hypergeometrica_amd import mpd_device as D
+    from hypergeometrica_amd import _lib
+    rnd = np.random.default_rng(5)
+    L = _lib.lib()
+    s = None  # the library's stream; torch.cuda.synchronize() orders the reads after it
+    for na, nb in [(1, 1), (5000, 3), (4096, 4096), (70000, 65536), (1 << 20, 1 << 20)]:
+        for kind in ["random", "ones"]:
+            if kind == "ones":
+                a = np.full(na, 2**64 - 1, dtype=np.uint64)
+                b = np.zeros(nb, dtype=np.uint64)
+                b[0] = 1
+            else:
+                a = rnd.integers(0, 2**64, size=na, dtype=np.uint64)
+                b = rnd.integers(0, 2**64, size=nb, dtype=np.uint64)
+            A = torch.from_numpy(a.view(np.int64)).cuda()
+            B = torch.from_numpy(b.view(np.int64)).cuda()
+            n = max(na, nb)
+            out = torch.empty(n + 1, dtype=torch.int64, device="cuda")
+            torch.cuda.synchronize()
+            _lib.check(L.hgx_add_device(A.data_ptr(), na, B.data_ptr(), nb, out.data_ptr(), s))
+            torch.cuda.synchronize()
+            ia, ib = D.to_host(D.DMPD(1, A, 0)).mantissa, D.to_host(D.DMPD(1, B, 0)).mantissa
+            assert D.to_host(D.DMPD(1, out, 0)).mantissa == ia + ib
+            big, small, nbig, nsmall = (A, B, na, nb) if ia >= ib else (B, A, nb, na)
+            out2 = torch.empty(n, dtype=torch.int64, device="cuda")
+            _lib.check(L.hgx_sub_device(big.data_ptr(), nbig, small.data_ptr(), nsmall, out2.data_ptr(), s))
+            torch.cuda.synchronize()
+            assert D.to_host(D.DMPD(1, out2, 0)).mantissa == abs(ia - ib)
+    # in place: out aliases the first operand (which has room for the carry limb)
+    A = torch.full((4097,), -1, dtype=torch.int64, device="cuda")
+    A[4096] = 0
+    one = torch.ones(1, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()  # torch's fills first: the library works on its own stream
+    _lib.check(L.hgx_add_device(A.data_ptr(), 4096, one.data_ptr(), 1, A.data_ptr(), s))
+    torch.cuda.synchronize()
+    assert D.to_host(D.DMPD(1, A, 0)).mantissa == 1 << (64 * 4096)
+
+
+# ---------------------------------------------------------------- graphs
+
+def test_batch_graph_replays_equal_eager(hgx, oracle):
+    # a batched level is captured once and replayed; every replay, with other
+    # eager work between, equals the eager launches
+    from hypergeometrica_amd import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(0x4847580A)
+    pairs = []
+    for i in range(48):
+        na, nb = int(2 ** rng.uniform(0, 15)), int(2 ** rng.uniform(0, 15))
+        a = oracle.splitmix64(11 * i + 1, max(na, 1))
+        b = a if i % 9 == 0 else oracle.splitmix64(11 * i + 2, max(nb, 1))
+        pairs.append((a, b))
+    _lib.check(L.hgx_graphs_enable(0))
+    eager = hgx.mul_batch(pairs)
+    _lib.check(L.hgx_graphs_enable(1))
+    try:
+        for rep in range(4):
+            say("graph replay", rep)
+            got = hgx.mul_batch(pairs)
+            assert all(np.array_equal(g, e) for g, e in zip(got, eager)), rep
+            # eager work between replays: another shape and a single product
+            other = hgx.mul_batch(pairs[: 5 + rep])
+            assert all(np.array_equal(g, e) for g, e in zip(other, eager[: 5 + rep]))
+            x, y = oracle.splitmix64(rep, 3000), oracle.splitmix64(rep + 9, 5000)
+            assert oracle.limbs_to_int(hgx.mul_limbs(x, y)) == oracle.limbs_to_int(x) * oracle.limbs_to_int(y)
+    finally:
+        _lib.check(L.hgx_graphs_enable(1))
+    for (a, b), e in zip(pairs, eager):
+        assert oracle.limbs_to_int(e) == oracle.limbs_to_int(a) * oracle.limbs_to_int(b)
