@@ -312,6 +312,7 @@ def main() -> int:
     ap.add_argument("--no-latency", action="store_true", help="skip the 2^16 x 2^16 per-call latency leg")
     ap.add_argument("--no-pi", action="store_true", help="skip the mpd-pi leg")
     ap.add_argument("--pi-bits", type=int, default=20, help="mpd-pi precision 2^bits")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default pass structures)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -332,6 +333,7 @@ def main() -> int:
     A = torch.empty(na, dtype=torch.int64, device=dev)
     B = torch.empty(nb, dtype=torch.int64, device=dev)
     OUT = torch.empty(na + nb, dtype=torch.int64, device=dev)
+    L_bits = (2 * n - 1).bit_length()
     h.fill_splitmix64(A.data_ptr(), na, 0x48475801)
     h.fill_splitmix64(B.data_ptr(), nb, 0x48475802)
     torch.cuda.synchronize()
@@ -343,6 +345,18 @@ def main() -> int:
         torch.cuda.synchronize()
         print(f"# calibration copy: {words * 8} bytes read, {words * 8} bytes written", file=sys.stderr, flush=True)
         del src, dst
+
+    # the autotuner picks the pass structure of every length the legs use
+    tuned = {}
+    if not args.no_tune:
+        import ctypes
+        t0 = time.perf_counter()
+        _lib.check(L_.hgx_autotune(13, L_bits, 2))
+        for b in range(13, L_bits + 1):
+            c, dm = ctypes.c_int(), ctypes.c_int()
+            _lib.check(L_.hgx_plan_get(b, ctypes.byref(c), ctypes.byref(dm)))
+            tuned[b] = (c.value, dm.value)
+        print(f"# autotuned 2^13..2^{L_bits} in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     def step():
         _lib.check(L_.hgx_mul_device(A.data_ptr(), na, B.data_ptr(), nb, OUT.data_ptr(), None))
@@ -463,7 +477,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"single exact multiply, 2^{args.bits} x 2^{args.bits} uniform 64-bit limbs, "
                                f"L=2^{L.bit_length() - 1}, 3 primes, operands resident in HBM",
-                   "limbs_per_step": limbs, "transform_length": L},
+                   "limbs_per_step": limbs, "transform_length": L,
+                   "pass_structure": {"column_passes": tuned.get(L_bits, (None,))[0],
+                                      "direct_table_max_bits": tuned.get(L_bits, (None, None))[1],
+                                      "autotuned": bool(tuned)}},
         "roofline": roofline,
         "valu_roofline": valu,
         "multiply_roofline": {"alg_bytes": b_mul, "frac": round(b_mul / (ms_per_step * 1e-3) / HBM_PEAK, 4)},

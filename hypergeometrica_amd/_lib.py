@@ -22,6 +22,7 @@ EXPORTS = (
     "hgx_mul_device", "hgx_sqr_device", "hgx_mul_batch_device", "hgx_ntt_forward",
     "hgx_ntt_reverse", "hgx_submit", "hgx_poll", "hgx_wait", "hgx_fill_splitmix64",
     "hgx_timing_enable", "hgx_timing_read", "hgx_add_device", "hgx_sub_device", "hgx_graphs_enable",
+    "hgx_plan_set", "hgx_plan_get", "hgx_autotune",
 )
 
 
@@ -71,6 +72,9 @@ def load() -> ctypes.CDLL:
         L.hgx_mul_device.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
         L.hgx_sqr_device.argtypes = [_vp, _sz, _vp, _vp]
         L.hgx_add_device.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
+        L.hgx_plan_set.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.hgx_plan_get.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.hgx_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.hgx_sub_device.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
         L.hgx_mul_batch_device.argtypes = [ctypes.POINTER(MulDesc), _sz, _vp]
         L.hgx_ntt_forward.argtypes = [_vp, _sz, ctypes.c_int]

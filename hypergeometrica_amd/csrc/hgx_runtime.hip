@@ -131,7 +131,8 @@ struct Context {
   PrimeSet ps{};
   u64 gen[3]{};
   u64 *tables = nullptr;
-  std::map<int, std::unique_ptr<Plan>> plans;
+  std::map<int, std::unique_ptr<Plan>> plans;  // by plan_key(n, choice)
+  std::map<int, std::pair<int, int>> choice;    // tuned or set (ncol, direct_max) per length
   Workspace ws, stage, jobs_dev, pack, scan;
   MulJob *jobs_host = nullptr;  // pinned job-table staging
   size_t jobs_host_cap = 0;
@@ -216,27 +217,44 @@ int ensure(Workspace &w, size_t bytes) {
 }
 
 // ---- plans ----
+// A length-2^n transform is NCOL column passes of 8 bits and a row pass of
+// n - 8 NCOL bits; a column pass needs blocks of at least one 4096-word tile,
+// so the row pass keeps 4..12 bits.  Column passes on blocks of at most
+// 2^DIRECT_MAX words use the direct twiddle table.  The default is the fewest
+// passes and DIRECT_MAX = kDirectMaxBits; hgx_plan_set and hgx_autotune
+// choose others per length.
+int min_ncol(int n) { return n <= kRowMaxBits ? 0 : (n - kRowMaxBits + 7) / 8; }
+int max_ncol(int n) { return n <= kRowMaxBits ? 0 : (n - 4) / 8; }
+std::pair<int, int> choice_for(int n) {
+  auto it = g_ctx.choice.find(n);
+  return it != g_ctx.choice.end() ? it->second : std::make_pair(min_ncol(n), kDirectMaxBits);
+}
+int plan_key(int n, int ncol, int dm) { return (n << 12) | (ncol << 6) | dm; }
+
+int build_plan_variant(int n, int ncol, int dm, Plan **out);
 int build_plan(int n, Plan **out) {
-  auto it = g_ctx.plans.find(n);
+  const auto c = choice_for(n);
+  return build_plan_variant(n, c.first, c.second, out);
+}
+
+int build_plan_variant(int n, int ncol, int dm, Plan **out) {
+  const int key = plan_key(n, ncol, dm);
+  auto it = g_ctx.plans.find(key);
   if (it != g_ctx.plans.end()) {
     *out = it->second.get();
     return HGX_OK;
   }
   auto plan = std::make_unique<Plan>();
   plan->n = n;
-  int rem = n;
-  while (rem > kRowMaxBits) {
-    plan->ncol++;
-    rem -= 8;
-  }
-  plan->srow = rem;
+  plan->ncol = ncol;
+  plan->srow = n - 8 * ncol;
   const u64 L = 1ull << n;
   std::vector<u64> host;
   struct Off { size_t Df, Di, Dm, Dt, Sf, Si, Sm, St, Tf, Ti; };
   std::vector<Off> offs(plan->ncol * 3);
   for (int i = 0; i < plan->ncol; i++) {
     const int b = n - 8 * i;
-    const bool direct = b <= kDirectMaxBits;
+    const bool direct = b <= dm;
     const size_t T = (size_t)1 << (b - 8);
     plan->col[i].bbits = b;
     plan->col[i].direct = direct;
@@ -344,8 +362,15 @@ int build_plan(int n, Plan **out) {
       }
   }
   *out = plan.get();
-  g_ctx.plans[n] = std::move(plan);
+  g_ctx.plans[key] = std::move(plan);
   return HGX_OK;
+}
+
+void drop_plan(int key) {
+  auto it = g_ctx.plans.find(key);
+  if (it == g_ctx.plans.end()) return;
+  dev_free(it->second->dev, it->second->bytes);
+  g_ctx.plans.erase(it);
 }
 
 ColTw coltw(const ColPassTables &c, int which) {
@@ -986,6 +1011,62 @@ void server_stop(bool at_exit) {
   g_srv.started = false;
 }
 
+// ---- pass-structure choices and the autotuner ----
+void clear_graphs() {
+  for (auto &kv : g_ctx.graphs) (void)hipGraphExecDestroy(kv.second);
+  g_ctx.graphs.clear();
+}
+
+int set_choice_locked(int bits, int ncol, int dm) {
+  if (bits < 1 || bits > kMaxBits) return fail(HGX_EINVAL, "transform bits out of range");
+  if (ncol < min_ncol(bits) || ncol > max_ncol(bits))
+    return fail(HGX_EINVAL, "2^" + std::to_string(bits) + " takes " + std::to_string(min_ncol(bits)) + ".." +
+                                std::to_string(max_ncol(bits)) + " column passes");
+  if (dm < 12 || dm > 24) return fail(HGX_EINVAL, "direct-table limit must be 12..24 bits");
+  // products of this length may still run on any stream with the old tables
+  HIPCHK(hipDeviceSynchronize());
+  const auto old = choice_for(bits);
+  g_ctx.choice[bits] = {ncol, dm};
+  if (old != std::make_pair(ncol, dm)) {
+    clear_graphs();
+    drop_plan(plan_key(bits, old.first, old.second));
+  }
+  return HGX_OK;
+}
+
+std::string device_arch() {
+  hipDeviceProp_t prop{};
+  if (hipGetDeviceProperties(&prop, g_ctx.device) != hipSuccess) return "unknown";
+  std::string a = prop.gcnArchName;
+  return a.substr(0, a.find(':'));
+}
+
+// tuned choices persist in the file named by HGX_TUNE_CACHE: "arch bits ncol dm" lines
+void tune_cache_load() {
+  const char *path = getenv("HGX_TUNE_CACHE");
+  if (!path) return;
+  FILE *f = fopen(path, "r");
+  if (!f) return;
+  char arch[64];
+  int bits, ncol, dm;
+  const std::string mine = device_arch();
+  while (fscanf(f, "%63s %d %d %d", arch, &bits, &ncol, &dm) == 4)
+    if (mine == arch && bits >= 1 && bits <= kMaxBits && ncol >= min_ncol(bits) && ncol <= max_ncol(bits) &&
+        dm >= 12 && dm <= 24)
+      g_ctx.choice[bits] = {ncol, dm};
+  fclose(f);
+}
+
+void tune_cache_save() {
+  const char *path = getenv("HGX_TUNE_CACHE");
+  if (!path) return;
+  FILE *f = fopen(path, "w");
+  if (!f) return;
+  const std::string arch = device_arch();
+  for (auto &kv : g_ctx.choice) fprintf(f, "%s %d %d %d\n", arch.c_str(), kv.first, kv.second.first, kv.second.second);
+  fclose(f);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1044,6 +1125,7 @@ int hgx_init(int device, size_t hbm_budget_bytes) {
   }
   g_ctx.ps = ps;
   g_ctx.ready = true;
+  tune_cache_load();
   return HGX_OK;
 }
 
@@ -1337,6 +1419,91 @@ int hgx_wait(hgx_ticket ticket) {
   g_srv.cv.wait(lk, [&] { return r->state == 2 || r->state < 0; });
   g_srv.tickets.erase(ticket);
   if (r->state < 0) return fail(r->state, r->err);
+  return HGX_OK;
+}
+
+// ---- pass-structure choices and the autotuner ----
+int hgx_plan_set(int bits, int ncol, int direct_max_bits) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  return set_choice_locked(bits, ncol, direct_max_bits);
+}
+
+int hgx_plan_get(int bits, int *ncol, int *direct_max_bits) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  if (bits < 1 || bits > kMaxBits || !ncol || !direct_max_bits) return fail(HGX_EINVAL, "bad arguments");
+  const auto c = choice_for(bits);
+  *ncol = c.first;
+  *direct_max_bits = c.second;
+  return HGX_OK;
+}
+
+// For every length 2^bits, min_bits <= bits <= max_bits, time a product of two
+// 2^(bits-1)-limb operands (SplitMix64 limbs, device memory) under every
+// pass structure (column-pass counts; direct-table limits 12, 16, 20, 22 where
+// they change a pass) and keep the fastest: best of REPS runs after one warm-up.
+int hgx_autotune(int min_bits, int max_bits, int reps) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (min_bits < 1 || max_bits > kMaxBits || min_bits > max_bits || reps < 1)
+    return fail(HGX_EINVAL, "bad autotune range");
+  hipStream_t s = g_ctx.stream;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int n = std::max(min_bits, kRowMaxBits + 1); n <= max_bits; n++) {
+    const u64 L = 1ull << n, half = L / 2;
+    // candidates: every column-pass count, and the direct-table limits that
+    // change which passes use the direct table
+    std::vector<std::pair<int, int>> cand;
+    for (int ncol = min_ncol(n); ncol <= max_ncol(n); ncol++) {
+      std::vector<int> seen;
+      for (int dm : {12, 16, 20, 22}) {
+        int ndirect = 0;
+        for (int i = 0; i < ncol; i++) ndirect |= (n - 8 * i <= dm) << i;
+        if (std::find(seen.begin(), seen.end(), ndirect) != seen.end()) continue;
+        seen.push_back(ndirect);
+        cand.push_back({ncol, dm});
+      }
+    }
+    if (cand.size() < 2) continue;
+    rc = ensure(g_ctx.stage, 16 * L);
+    if (rc) break;
+    u64 *a = (u64 *)g_ctx.stage.ptr, *b = a + half, *out = b + half;
+    hgx_launch_splitmix64(dim3((unsigned)((half + 255) / 256)), s, a, half, 0x48475800 + n, 1);
+    hgx_launch_splitmix64(dim3((unsigned)((half + 255) / 256)), s, b, half, 0x48475900 + n, 1);
+    float best = 1e30f;
+    std::pair<int, int> pick = choice_for(n);
+    for (auto &c : cand) {
+      rc = set_choice_locked(n, c.first, c.second);
+      if (rc) break;
+      float t = 1e30f;
+      for (int r = 0; r <= reps && !rc; r++) {
+        HIPCHK(hipEventRecord(e0, s));
+        rc = mul_device_locked(a, half, b, half, out, false, s);
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0) t = std::min(t, ms);  // run 0 builds the tables
+      }
+      if (rc) break;
+      if (t < best) {
+        best = t;
+        pick = c;
+      }
+      // setting the next candidate drops this one's tables
+    }
+    if (rc) break;
+    rc = set_choice_locked(n, pick.first, pick.second);
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  tune_cache_save();
   return HGX_OK;
 }
 

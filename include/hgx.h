@@ -138,6 +138,22 @@ int hgx_fill_splitmix64(uint64_t *d_out, size_t n, uint64_t seed,
 // every kernel directly; the outputs are the same.  Default: on.
 int hgx_graphs_enable(int on);
 
+// Pass structure per transform length 2^BITS: NCOL column passes of 8 bits
+// and a row pass of BITS - 8 NCOL bits (4..12), and the largest column-pass
+// block (2^DIRECT_MAX_BITS words, 12..24) that uses the direct twiddle table
+// instead of the tile-factored pair.  No counterpart in the reference (its
+// transforms are one radix-2 loop).  Every choice gives the same results;
+// the default is the fewest passes and DIRECT_MAX_BITS = 20.
+int hgx_plan_set(int bits, int ncol, int direct_max_bits);
+int hgx_plan_get(int bits, int *ncol, int *direct_max_bits);
+
+// Autotuner: for every length 2^bits with MIN_BITS <= bits <= MAX_BITS that
+// offers a choice, time a product of two 2^(bits-1)-limb operands under each
+// pass structure (best of REPS runs after a warm-up) and keep the fastest.
+// With HGX_TUNE_CACHE set in the environment the choices are saved to that
+// file and read back by later hgx_init calls on the same GPU architecture.
+int hgx_autotune(int min_bits, int max_bits, int reps);
+
 // Per-kernel device timing of later calls, for benchmarks: HIP events are
 // recorded around every kernel on the stream it is launched on.  Plays the
 // role of the reference's *verbose* phase timers ("Forward...", "Reverse...",

@@ -383,3 +383,70 @@ def test_batch_graph_replays_equal_eager(hgx, oracle):
         _lib.check(L.hgx_graphs_enable(1))
     for (a, b), e in zip(pairs, eager):
         assert oracle.limbs_to_int(e) == oracle.limbs_to_int(a) * oracle.limbs_to_int(b)
+
+
+# ---------------------------------------------------------------- pass structures
+
+def _plan(L, bits):
+    import ctypes
+    n, d = ctypes.c_int(), ctypes.c_int()
+    from hypergeometrica_amd import _lib
+    _lib.check(L.hgx_plan_get(bits, ctypes.byref(n), ctypes.byref(d)))
+    return n.value, d.value
+
+
+@pytest.mark.parametrize("bits", [20, 21, 28])
+def test_every_pass_structure_gives_the_same_results(hgx, oracle, bits):
+    # each column-pass count and direct-table limit: products against GMP (or
+    # the mod 2^61-1 residue at 2^28), squares, and transforms against the
+    # oracle; the default is restored after
+    from hypergeometrica_amd import _lib
+    L = _lib.lib()
+    default = _plan(L, bits)
+    lo, hi = (bits - 12 + 7) // 8, (bits - 4) // 8
+    try:
+        for ncol in range(lo, hi + 1):
+            for dm in [12, 16, 20, 22]:
+                say("plan", bits, ncol, dm)
+                _lib.check(L.hgx_plan_set(bits, ncol, dm))
+                assert _plan(L, bits) == (ncol, dm)
+                na = (1 << (bits - 1)) - 7
+                a = oracle.splitmix64(bits * 100 + ncol, na)
+                b = oracle.splitmix64(dm, (1 << (bits - 1)) - 3)
+                got = hgx.mul_limbs(a, b)
+                if bits <= 21:
+                    assert np.array_equal(got, _gmp.mul(a, b))
+                    sq = hgx.sqr_limbs(a)
+                    assert np.array_equal(sq, _gmp.mul(a, a))
+                    x = residues(oracle, 1, 1 << bits, 0x485 + dm)
+                    assert np.array_equal(hgx.ntt_forward(x, 1), oracle.ntt_forward(x, 1))
+                    assert np.array_equal(hgx.ntt_reverse(hgx.ntt_forward(x, 1), 1), x)
+                else:
+                    assert mod_q(got) == (mod_q(a) * mod_q(b)) % Q
+                if bits > 21 and dm == 12:
+                    break  # at 2^28 the other limits change only the 2^20 pass: one of them is enough
+    finally:
+        _lib.check(L.hgx_plan_set(bits, *default))
+
+
+def test_autotune_keeps_valid_choices(hgx, oracle, tmp_path, monkeypatch):
+    import ctypes
+    from hypergeometrica_amd import _lib
+    L = _lib.lib()
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("HGX_TUNE_CACHE", str(cache))
+    saved = {b: _plan(L, b) for b in range(18, 22)}
+    try:
+        _lib.check(L.hgx_autotune(18, 21, 1))
+        lines = cache.read_text().split("\n")
+        rows = [ln.split() for ln in lines if ln.strip()]
+        assert rows and all(len(r) == 4 for r in rows)
+        for b in range(18, 22):
+            ncol, dm = _plan(L, b)
+            assert (b - 12 + 7) // 8 <= ncol <= (b - 4) // 8 and 12 <= dm <= 24
+            na = (1 << (b - 1)) - 5
+            a, bb = oracle.splitmix64(b, na), oracle.splitmix64(b + 50, na + 1)
+            assert np.array_equal(hgx.mul_limbs(a, bb), _gmp.mul(a, bb))
+    finally:
+        for b, c in saved.items():
+            _lib.check(L.hgx_plan_set(b, *c))
