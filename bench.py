@@ -353,9 +353,9 @@ def main() -> int:
         t0 = time.perf_counter()
         _lib.check(L_.hgx_autotune(13, L_bits, 2))
         for b in range(13, L_bits + 1):
-            c, dm = ctypes.c_int(), ctypes.c_int()
-            _lib.check(L_.hgx_plan_get(b, ctypes.byref(c), ctypes.byref(dm)))
-            tuned[b] = (c.value, dm.value)
+            c, dm, md = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            _lib.check(L_.hgx_plan_get(b, ctypes.byref(c), ctypes.byref(dm), ctypes.byref(md)))
+            tuned[b] = (c.value, dm.value, md.value)
         print(f"# autotuned 2^13..2^{L_bits} in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     def step():
@@ -478,8 +478,9 @@ def main() -> int:
         "config": {"workload": f"single exact multiply, 2^{args.bits} x 2^{args.bits} uniform 64-bit limbs, "
                                f"L=2^{L.bit_length() - 1}, 3 primes, operands resident in HBM",
                    "limbs_per_step": limbs, "transform_length": L,
-                   "pass_structure": {"column_passes": tuned.get(L_bits, (None,))[0],
-                                      "direct_table_max_bits": tuned.get(L_bits, (None, None))[1],
+                   "pass_structure": {"column_passes": tuned.get(L_bits, (None,) * 3)[0],
+                                      "direct_table_max_bits": tuned.get(L_bits, (None,) * 3)[1],
+                                      "direct_montgomery_tables": tuned.get(L_bits, (None,) * 3)[2],
                                       "autotuned": bool(tuned)}},
         "roofline": roofline,
         "valu_roofline": valu,

@@ -53,6 +53,12 @@ __global__ __launch_bounds__(HGX_NT) void k_as_blocks(const u64 *a, u64 na, cons
   if (threadIdx.x == 0) agg[blockIdx.x] = (uint8_t)sh[0];
 }
 
+// p[i] = v, i < n (graph-safe zeroing: no memset node)
+__global__ __launch_bounds__(256) void k_fill(u64 *p, u64 n, u64 v) {
+  const u64 i = (u64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // carry into every block; one workgroup of 1024 threads
 __global__ __launch_bounds__(1024) void k_as_scan(const uint8_t *agg, uint8_t *blkin, u64 nblocks) {
   __shared__ u32 sh[1024];
@@ -69,10 +75,11 @@ __global__ __launch_bounds__(1024) void k_as_scan(const uint8_t *agg, uint8_t *b
   }
 }
 
-// out[i] = a[i] +- b[i] +- carry in, i < n; an add also writes the final carry to out[n]
+// out[i] = a[i] +- b[i] +- carry in, i < n; an add writes its final carry to
+// *carry_out when that is not null
 template <bool SUB>
 __global__ __launch_bounds__(HGX_NT) void k_as_apply(const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out, u64 n,
-                                                      const uint8_t *blkin) {
+                                                      const uint8_t *blkin, u64 *carry_out) {
   __shared__ u32 sh[HGX_NT];
   const u64 i0 = (u64)blockIdx.x * kAsBlock + (u64)threadIdx.x * kAsPerThread;
   u64 r[kAsPerThread];
@@ -96,24 +103,28 @@ __global__ __launch_bounds__(HGX_NT) void k_as_apply(const u64 *a, u64 na, const
     if (i < n) {
       out[i] = SUB ? r[j] - c : r[j] + c;
       c = (m[j] == 2u) ? c : (m[j] == 3u ? 1u : 0u);
-      if (!SUB && i == n - 1) out[n] = c;
+      if (!SUB && i == n - 1 && carry_out) *carry_out = c;
     }
   }
 }
 
 }  // namespace
 
+void hgx_launch_fill(hipStream_t s, u64 *p, u64 n, u64 v) {
+  if (n) hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
 void hgx_launch_addsub(bool sub, hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out, u64 n,
-                       uint8_t *agg, uint8_t *blkin) {
+                       uint8_t *agg, uint8_t *blkin, u64 *carry_out) {
   const unsigned nblk = (unsigned)((n + kAsBlock - 1) / kAsBlock);
   if (sub) {
     hipLaunchKernelGGL(k_as_blocks<true>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, n, agg);
     hipLaunchKernelGGL(k_as_scan, dim3(1), dim3(1024), 0, s, agg, blkin, (u64)nblk);
-    hipLaunchKernelGGL(k_as_apply<true>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, out, n, blkin);
+    hipLaunchKernelGGL(k_as_apply<true>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, out, n, blkin, carry_out);
   } else {
     hipLaunchKernelGGL(k_as_blocks<false>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, n, agg);
     hipLaunchKernelGGL(k_as_scan, dim3(1), dim3(1024), 0, s, agg, blkin, (u64)nblk);
-    hipLaunchKernelGGL(k_as_apply<false>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, out, n, blkin);
+    hipLaunchKernelGGL(k_as_apply<false>, dim3(nblk), dim3(HGX_NT), 0, s, a, na, b, nb, out, n, blkin, carry_out);
   }
 }
 

@@ -8,7 +8,7 @@
 // registers, one LDS exchange, stages 3-0) and rows g + 16 j go straight
 // back, in place on the residue array of prime kbase + y.
 // grid = (jobs << jt_bits, nprimes)
-template <bool DIRECT>
+template <int FORM>
 __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
                                                       int kbase) {
   __shared__ u64 lds[HGX_TILE_LDS];
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     const u32 m = 16 * g + j;
-    x[j] = apply_twiddle<DIRECT>(arr[gbase + rs * m], tw, k, t0 + t, bitrev8(m), M);
+    x[j] = apply_twiddle<FORM>(arr[gbase + rs * m], tw, k, t0 + t, bitrev8(m), M);
   }
   round_regs<8, 4, 4, true>(x, 0, ps.tw_inv[k], M);
 #pragma unroll
@@ -263,25 +263,64 @@ __global__ void k_splitmix64(u64 *out, u64 n, u64 seed, int force_top) {
   out[i] = z;
 }
 
+static int tw_form(const ColTw &tw) { return tw.D[0] ? TW_DIRECT : (tw.DM[0] ? TW_DIRECT_MONT : TW_FACTORED); }
+
 void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr,
                         int jt_bits, u64 L, int bbits, int kbase) {
-  const bool direct = tw.D[0] != nullptr;
-  if (first && direct)
-    hipLaunchKernelGGL((k_fwd_col<true, true>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
-  else if (first)
-    hipLaunchKernelGGL((k_fwd_col<true, false>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
-  else if (direct)
-    hipLaunchKernelGGL((k_fwd_col<false, true>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
-  else
-    hipLaunchKernelGGL((k_fwd_col<false, false>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+  switch (tw_form(tw) + (first ? 3 : 0)) {
+    case TW_FACTORED:
+      hipLaunchKernelGGL((k_fwd_col<false, TW_FACTORED>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      break;
+    case TW_DIRECT:
+      hipLaunchKernelGGL((k_fwd_col<false, TW_DIRECT>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      break;
+    case TW_DIRECT_MONT:
+      hipLaunchKernelGGL((k_fwd_col<false, TW_DIRECT_MONT>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits,
+                         kbase);
+      break;
+    case 3 + TW_FACTORED:
+      hipLaunchKernelGGL((k_fwd_col<true, TW_FACTORED>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      break;
+    case 3 + TW_DIRECT:
+      hipLaunchKernelGGL((k_fwd_col<true, TW_DIRECT>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      break;
+    default:
+      hipLaunchKernelGGL((k_fwd_col<true, TW_DIRECT_MONT>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits,
+                         kbase);
+      break;
+  }
 }
 
 void hgx_launch_inv_col(dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr, int jt_bits,
                         u64 L, int bbits, int kbase) {
-  if (tw.D[0] != nullptr)
-    hipLaunchKernelGGL(k_inv_col<true>, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
-  else
-    hipLaunchKernelGGL(k_inv_col<false>, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+  switch (tw_form(tw)) {
+    case TW_FACTORED:
+      hipLaunchKernelGGL(k_inv_col<TW_FACTORED>, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      break;
+    case TW_DIRECT:
+      hipLaunchKernelGGL(k_inv_col<TW_DIRECT>, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      break;
+    default:
+      hipLaunchKernelGGL(k_inv_col<TW_DIRECT_MONT>, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      break;
+  }
+}
+
+// direct Montgomery table of one pass from its tile-factored pair:
+// DM[q*T + t] = TT[t/16][q] * TS[q][t mod 16] (Montgomery form times plain
+// value: the product stays in Montgomery form)
+__global__ __launch_bounds__(256) void k_direct_mont(u64 *DM, const u64 *TS, const u64 *TT, int tbits, Mod M) {
+  const u64 total = 256ull << tbits;
+  for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < total; i += (u64)gridDim.x * 256) {
+    const u32 q = (u32)(i >> tbits);
+    const u64 t = i & ((1ull << tbits) - 1);
+    const ulonglong2 w = *(const ulonglong2 *)(TS + 2 * ((q << 4) + (u32)(t & 15)));
+    DM[i] = shoup_mul(TT[((t >> 4) << 8) + q], w.x, w.y, M);
+  }
+}
+
+void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M) {
+  hipLaunchKernelGGL(k_direct_mont, dim3(4096), dim3(256), 0, s, DM, TS, TT, tbits, M);
 }
 
 void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &jr, int bpj_bits, u64 L) {

@@ -38,12 +38,15 @@
 __device__ __forceinline__ u32 bitrev8(u32 m) { return __builtin_bitreverse32(m) >> 24; }
 
 // x * w_B^(t q) for column t, frequency q: one Shoup product from the direct
-// table (DIRECT), or a Shoup and a REDC product from the tile-factored tables.
-template <bool DIRECT>
+// table, one REDC product from the direct Montgomery table, or a Shoup and a
+// REDC product from the tile-factored tables.
+template <int FORM>
 __device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u64 t, u32 q, const Mod &M) {
-  if constexpr (DIRECT) {
+  if constexpr (FORM == TW_DIRECT) {
     const ulonglong2 w = *(const ulonglong2 *)(tw.D[k] + 2 * (((u64)q << tw.tbits) + t));
     return shoup_mul(x, w.x, w.y, M);
+  } else if constexpr (FORM == TW_DIRECT_MONT) {
+    return mont_mul(x, tw.DM[k][((u64)q << tw.tbits) + t], M);
   } else {
     const ulonglong2 s = *(const ulonglong2 *)(tw.TS[k] + 2 * ((q << 4) + (u32)(t & 15)));
     const u64 f = tw.TT[k][((t >> 4) << 8) + q];
@@ -253,7 +256,7 @@ __device__ __forceinline__ U192 garner3(u64 r0, u64 r1, u64 r2, const PrimeSet &
 // into residue array y.  Otherwise: in place on the residue array of prime
 // kbase + y.
 // grid = (jobs << jt_bits, nprimes, noperands)
-template <bool FIRST, bool DIRECT>
+template <bool FIRST, int FORM>
 __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
                                                       int kbase) {
   // The 256-point column DFT is two rounds of 16: round 1 (stages 0-3) takes
@@ -302,7 +305,7 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     const u32 m = 16 * g + j;
-    arr[gbase + rs * m] = apply_twiddle<DIRECT>(x[j], tw, k, t0 + t, bitrev8(m), M);
+    arr[gbase + rs * m] = apply_twiddle<FORM>(x[j], tw, k, t0 + t, bitrev8(m), M);
   }
 }
 

@@ -25,5 +25,7 @@ void hgx_launch_carry_blocks(dim3 grid, hipStream_t s, const JobRef &jr, u64 nbl
 void hgx_launch_carry_apply(dim3 grid, hipStream_t s, const JobRef &jr, int bpj_bits, u64 nruns, int rl);
 void hgx_launch_splitmix64(dim3 grid, hipStream_t s, u64 *out, u64 n, u64 seed, int force_top);
 void hgx_launch_addsub(bool sub, hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out, u64 n,
-                       uint8_t *agg, uint8_t *blkin);
+                       uint8_t *agg, uint8_t *blkin, u64 *carry_out);
+void hgx_launch_fill(hipStream_t s, u64 *p, u64 n, u64 v);
 u64 hgx_addsub_blocks(u64 n);
+void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M);

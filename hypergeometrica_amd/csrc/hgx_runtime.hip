@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -97,10 +98,13 @@ const u64 kModuli[3] = {0x6280000000000001ull, 0x5700000000000001ull, 0x41800000
 
 struct ColPassTables {
   int bbits = 0;
-  bool direct = false;  // B <= 2^kDirectMaxBits: direct (q, t) tables
+  bool direct = false;  // B <= 2^direct_max: direct (q, t) tables
   u64 *Df[3], *Di[3], *Di_mul[3], *Di_tr[3];
   // otherwise tile-factored: small (q, tt) tables with the scale, tile tables
   u64 *TSf[3], *TSi[3], *TSi_mul[3], *TSi_tr[3], *TTf[3], *TTi[3];
+  // and, when chosen, direct Montgomery tables of the forward and of the
+  // product's inverse (the transform-only inverse keeps the factored pair)
+  u64 *DMf[3] = {}, *DMi[3] = {};
 };
 
 // Column passes on blocks of at most 2^kDirectMaxBits words use the direct
@@ -109,13 +113,27 @@ struct ColPassTables {
 #define HGX_DIRECT_MAX_BITS 20
 #endif
 constexpr int kDirectMaxBits = HGX_DIRECT_MAX_BITS;
+// whether column passes on blocks above 2^kDirectMaxBits (and at most
+// 2^28 words) use direct Montgomery tables by default
+constexpr int kMontDirectDefault = 0;
 
 struct Plan {
   int n = 0, ncol = 0, srow = 0;
   ColPassTables col[8];
-  u64 *dev = nullptr;  // one allocation holding every table
+  u64 *dev = nullptr;  // one allocation holding the host-built tables
   size_t bytes = 0;
+  std::vector<std::pair<u64 *, size_t>> big;  // device-built direct Montgomery tables
 };
+
+// A length's pass structure: NCOL column passes (row pass of n - 8 NCOL
+// bits), the largest block with the direct table (2^DM words), and whether
+// larger blocks (up to 2^kMontDirectMaxBits words) use direct Montgomery tables.
+struct Choice {
+  int ncol, dm, md;
+  bool operator==(const Choice &o) const { return ncol == o.ncol && dm == o.dm && md == o.md; }
+  bool operator!=(const Choice &o) const { return !(*this == o); }
+};
+constexpr int kMontDirectMaxBits = 28;
 
 struct Workspace {
   void *ptr = nullptr;
@@ -132,11 +150,12 @@ struct Context {
   u64 gen[3]{};
   u64 *tables = nullptr;
   std::map<int, std::unique_ptr<Plan>> plans;  // by plan_key(n, choice)
-  std::map<int, std::pair<int, int>> choice;    // tuned or set (ncol, direct_max) per length
+  std::map<int, Choice> choice;                 // tuned or set pass structure per length
   Workspace ws, stage, jobs_dev, pack, scan;
   MulJob *jobs_host = nullptr;  // pinned job-table staging
   size_t jobs_host_cap = 0;
   hipEvent_t jobs_copied = nullptr;
+  bool split_on = true;  // unbalanced products as batched chunk products
   // captured launch sequences of batched chunks (hipGraph), keyed by shape
   bool graphs_on = true;
   std::map<std::vector<u64>, hipGraphExec_t> graphs;
@@ -225,20 +244,19 @@ int ensure(Workspace &w, size_t bytes) {
 // choose others per length.
 int min_ncol(int n) { return n <= kRowMaxBits ? 0 : (n - kRowMaxBits + 7) / 8; }
 int max_ncol(int n) { return n <= kRowMaxBits ? 0 : (n - 4) / 8; }
-std::pair<int, int> choice_for(int n) {
+Choice default_choice(int n) { return Choice{min_ncol(n), kDirectMaxBits, kMontDirectDefault}; }
+Choice choice_for(int n) {
   auto it = g_ctx.choice.find(n);
-  return it != g_ctx.choice.end() ? it->second : std::make_pair(min_ncol(n), kDirectMaxBits);
+  return it != g_ctx.choice.end() ? it->second : default_choice(n);
 }
-int plan_key(int n, int ncol, int dm) { return (n << 12) | (ncol << 6) | dm; }
+int plan_key(int n, const Choice &c) { return (n << 13) | (c.md << 12) | (c.ncol << 6) | c.dm; }
 
-int build_plan_variant(int n, int ncol, int dm, Plan **out);
-int build_plan(int n, Plan **out) {
-  const auto c = choice_for(n);
-  return build_plan_variant(n, c.first, c.second, out);
-}
+int build_plan_variant(int n, const Choice &c, Plan **out);
+int build_plan(int n, Plan **out) { return build_plan_variant(n, choice_for(n), out); }
 
-int build_plan_variant(int n, int ncol, int dm, Plan **out) {
-  const int key = plan_key(n, ncol, dm);
+int build_plan_variant(int n, const Choice &choice, Plan **out) {
+  const int ncol = choice.ncol, dm = choice.dm;
+  const int key = plan_key(n, choice);
   auto it = g_ctx.plans.find(key);
   if (it != g_ctx.plans.end()) {
     *out = it->second.get();
@@ -361,15 +379,48 @@ int build_plan_variant(int n, int ncol, int dm, Plan **out) {
         }
       }
   }
+  // direct Montgomery tables, built on the device from the factored pairs
+  if (choice.md) {
+    for (int i = 0; i < plan->ncol; i++) {
+      ColPassTables &c = plan->col[i];
+      if (c.direct || c.bbits > kMontDirectMaxBits) continue;
+      const size_t bytes = (size_t)sizeof(u64) << c.bbits;
+      for (int k = 0; k < 3; k++) {
+        const Mod M{kModuli[k], 0 - kModuli[k], (u32)((kModuli[k] - 1) >> 55)};
+        for (int dir = 0; dir < 2; dir++) {
+          u64 *dmt = nullptr;
+          int rc = dev_alloc((void **)&dmt, bytes);
+          if (rc) {
+            for (auto &b : plan->big) dev_free(b.first, b.second);
+            dev_free(plan->dev, plan->bytes);
+            return rc;
+          }
+          plan->big.push_back({dmt, bytes});
+          const u64 *TS = dir == 0 ? c.TSf[k] : (i == 0 ? c.TSi_mul[k] : c.TSi[k]);
+          const u64 *TT = dir == 0 ? c.TTf[k] : c.TTi[k];
+          hgx_launch_direct_mont(g_ctx.stream, dmt, TS, TT, c.bbits - 8, M);
+          (dir == 0 ? c.DMf : c.DMi)[k] = dmt;
+        }
+      }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  }
   *out = plan.get();
   g_ctx.plans[key] = std::move(plan);
   return HGX_OK;
 }
 
+void free_plan(Plan &p) {
+  dev_free(p.dev, p.bytes);
+  for (auto &b : p.big) dev_free(b.first, b.second);
+  p.big.clear();
+}
+
 void drop_plan(int key) {
   auto it = g_ctx.plans.find(key);
   if (it == g_ctx.plans.end()) return;
-  dev_free(it->second->dev, it->second->bytes);
+  free_plan(*it->second);
   g_ctx.plans.erase(it);
 }
 
@@ -380,6 +431,9 @@ ColTw coltw(const ColPassTables &c, int which) {
   for (int k = 0; k < 3; k++) {
     if (c.direct) {
       t.D[k] = (which == 0) ? c.Df[k] : (which == 1 ? c.Di[k] : (which == 2 ? c.Di_mul[k] : c.Di_tr[k]));
+    } else if ((which == 0 && c.DMf[k]) || ((which == 1 || which == 2) && c.DMi[k])) {
+      t.D[k] = nullptr;
+      t.DM[k] = (which == 0) ? c.DMf[k] : c.DMi[k];
     } else {
       t.D[k] = nullptr;
       t.TS[k] = (which == 0) ? c.TSf[k] : (which == 1 ? c.TSi[k] : (which == 2 ? c.TSi_mul[k] : c.TSi_tr[k]));
@@ -609,9 +663,110 @@ int ws_release(hipStream_t s) {
   return HGX_OK;
 }
 
+// ---- unbalanced products ----
+// A product whose longer operand (NA limbs) is much longer than the shorter
+// (NB) is computed as K products of consecutive M-limb chunks of the longer
+// operand with the shorter one, all of transform length 2^NC (one batched
+// group): chunk i's product belongs at limb i*M.  The products of even and of
+// odd chunks go to two zeroed images, where they do not overlap (M >= NB), and
+// one addition with carries adds the images: the same integer as the one
+// transform of length >= NA + NB, for a fraction of its work.
+struct Split {
+  int nc = 0;
+  u64 m = 0, k = 0;
+};
+
+// Splitting costs a few launches, a job-table copy and an addition per
+// product: only products of transform length >= 2^kSplitMinBits are split.
+constexpr int kSplitMinBits = 19;
+
+bool choose_split(u64 na, u64 nb, Split &sp) {
+  if (na < nb) std::swap(na, nb);
+  const int n = log2_ceil(na + nb);
+  if (n < kSplitMinBits || nb == 0) return false;
+  const double full = 3.0 * std::ldexp(1.0, n) * n;
+  double best = 0.8 * full;  // split only for a clear gain
+  bool found = false;
+  for (int nc = std::max(log2_ceil(2 * nb), 1); nc < n; nc++) {
+    const u64 Lc = 1ull << nc, m = Lc - nb;
+    if (m < nb) continue;
+    const u64 k = (na + m - 1) / m;
+    // transform work, the zeroing and the addition, and a per-job overhead
+    const double c = 3.0 * (double)k * (double)Lc * nc + 8.0 * (double)(na + nb) + 2048.0 * (double)k;
+    if (c < best) {
+      best = c;
+      sp = Split{nc, m, k};
+      found = true;
+    }
+  }
+  return found;
+}
+
+int ensure_jobs_host(size_t count) {
+  HIPCHK(hipEventSynchronize(g_ctx.jobs_copied));
+  if (g_ctx.jobs_host_cap < count) {
+    if (g_ctx.jobs_host) (void)hipHostFree(g_ctx.jobs_host);
+    g_ctx.jobs_host = nullptr;
+    g_ctx.jobs_host_cap = std::max<size_t>(count, 1024);
+    HIPCHK(hipHostMalloc((void **)&g_ctx.jobs_host, g_ctx.jobs_host_cap * sizeof(MulJob), hipHostMallocDefault));
+  }
+  return HGX_OK;
+}
+
+// The split product into D_OUT on stream S; the caller holds the workspace.
+int split_product_locked(const u64 *d_a, size_t na, const u64 *d_b, size_t nb, u64 *d_out, const Split &sp,
+                         hipStream_t s) {
+  if (na < nb) {
+    std::swap(d_a, d_b);
+    std::swap(na, nb);
+  }
+  const u64 n = na + nb;
+  const WorkLayout wl = work_layout(sp.nc, false);
+  const size_t img = align_up(n * sizeof(u64));
+  const size_t tbl = align_up(sp.k * sizeof(MulJob));
+  const size_t bytes = 2 * img + tbl + sp.k * wl.total;
+  if (g_ctx.ws.bytes < bytes) HIPCHK(hipStreamSynchronize(s));  // this call's earlier work may use it
+  int rc = ensure(g_ctx.ws, bytes);
+  if (rc) return rc;
+  const u64 nblk = hgx_addsub_blocks(n);
+  if (g_ctx.scan.bytes < 2 * nblk) HIPCHK(hipStreamSynchronize(s));
+  rc = ensure(g_ctx.scan, 2 * nblk);
+  if (rc) return rc;
+  char *base = (char *)g_ctx.ws.ptr;
+  u64 *E = (u64 *)base, *O = (u64 *)(base + img);
+  MulJob *table = (MulJob *)(base + 2 * img);
+  char *work = base + 2 * img + tbl;
+  hgx_launch_fill(s, E, 2 * img / sizeof(u64), 0);
+  rc = ensure_jobs_host(sp.k);
+  if (rc) return rc;
+  for (u64 i = 0; i < sp.k; i++) {
+    const u64 off = i * sp.m;
+    const u64 mi = std::min<u64>(sp.m, na - off);
+    g_ctx.jobs_host[i] = make_job(d_a + off, mi, d_b, nb, ((i & 1) ? O : E) + off, sp.nc, false, work + i * wl.total);
+  }
+  HIPCHK(hipMemcpyAsync(table, g_ctx.jobs_host, sp.k * sizeof(MulJob), hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(g_ctx.jobs_copied, s));
+  JobRef jr{};
+  jr.table = table;
+  rc = enqueue_group(jr, (unsigned)sp.k, sp.nc, false, s);
+  if (rc) return rc;
+  uint8_t *agg = (uint8_t *)g_ctx.scan.ptr;
+  hgx_launch_addsub(false, s, E, n, O, n, d_out, n, agg, agg + nblk, nullptr);
+  HIPCHK(hipGetLastError());
+  return HGX_OK;
+}
+
 int mul_device_locked(const u64 *d_a, size_t na, const u64 *d_b, size_t nb, u64 *d_out, bool sqr, hipStream_t s) {
   const int n = log2_ceil((u64)hgx_product_length(na, nb));
   if (n > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32");
+  Split sp;
+  if (!sqr && g_ctx.split_on && choose_split(na, nb, sp)) {
+    int rc = ws_acquire(s);
+    if (rc) return rc;
+    rc = split_product_locked(d_a, na, d_b, nb, d_out, sp, s);
+    if (rc) return rc;
+    return ws_release(s);
+  }
   const WorkLayout wl = work_layout(n, sqr);
   int rc = ws_acquire(s);
   if (rc) return rc;
@@ -663,6 +818,7 @@ int launch_graph(const std::vector<u64> &key, hipStream_t s, Prepare prepare, En
 int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) {
   struct Item { size_t idx; int n; bool sqr; size_t bytes; };
   std::vector<Item> items;
+  std::vector<std::pair<size_t, Split>> splits;  // unbalanced products, done first
   items.reserve(count);
   for (size_t i = 0; i < count; i++) {
     const hgx_mul_desc &d = descs[i];
@@ -670,6 +826,11 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
     const bool sqr = (d.a == d.b && d.na == d.nb);
     const int n = log2_ceil((u64)hgx_product_length(d.na, d.nb));
     if (n > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32 in descriptor " + std::to_string(i));
+    Split sp;
+    if (!sqr && g_ctx.split_on && choose_split(d.na, d.nb, sp)) {
+      splits.push_back({i, sp});
+      continue;
+    }
     items.push_back({i, n, sqr, work_layout(n, sqr).total});
   }
   std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
@@ -677,6 +838,11 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
   });
   int rc = ws_acquire(s);
   if (rc) return rc;
+  for (auto &sv : splits) {
+    const hgx_mul_desc &d = descs[sv.first];
+    rc = split_product_locked(d.a, d.na, d.b, d.nb, d.out, sv.second, s);
+    if (rc) return rc;
+  }
   size_t pos = 0;
   while (pos < items.size()) {
     // a chunk: consecutive items that fit the budget together
@@ -689,17 +855,15 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
       bytes += add;
       end++;
     }
+    if (g_ctx.ws.bytes < bytes || g_ctx.jobs_dev.bytes < align_up((end - pos) * sizeof(MulJob)))
+      HIPCHK(hipStreamSynchronize(s));  // earlier work of this call may still use them
     rc = ensure(g_ctx.ws, bytes);
     if (rc) return rc;
     rc = ensure(g_ctx.jobs_dev, align_up((end - pos) * sizeof(MulJob)));
     if (rc) return rc;
     // fill the job table (pinned) and copy it to the device
-    HIPCHK(hipEventSynchronize(g_ctx.jobs_copied));
-    if (g_ctx.jobs_host_cap < end - pos) {
-      if (g_ctx.jobs_host) (void)hipHostFree(g_ctx.jobs_host);
-      g_ctx.jobs_host_cap = std::max<size_t>(end - pos, 1024);
-      HIPCHK(hipHostMalloc((void **)&g_ctx.jobs_host, g_ctx.jobs_host_cap * sizeof(MulJob), hipHostMallocDefault));
-    }
+    rc = ensure_jobs_host(end - pos);
+    if (rc) return rc;
     char *base = (char *)g_ctx.ws.ptr;
     for (size_t i = pos; i < end; i++) {
       const hgx_mul_desc &d = descs[items[i].idx];
@@ -845,6 +1009,25 @@ hipEvent_t take_event() {
 // Launch one admission round on stream s (called without g_srv.mu held).
 void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  // unbalanced products (split into chunk products) after the grouped ones
+  std::vector<char> split(batch.size(), 0);
+  std::vector<Split> sps(batch.size());
+  for (size_t i = 0; i < batch.size(); i++)
+    split[i] = !batch[i]->sqr && g_ctx.split_on && choose_split(batch[i]->d.na, batch[i]->d.nb, sps[i]);
+  std::vector<std::shared_ptr<SrvReq>> splits;
+  std::vector<Split> split_plans;
+  {
+    std::vector<std::shared_ptr<SrvReq>> grouped;
+    for (size_t i = 0; i < batch.size(); i++) {
+      if (split[i]) {
+        splits.push_back(batch[i]);
+        split_plans.push_back(sps[i]);
+      } else {
+        grouped.push_back(batch[i]);
+      }
+    }
+    batch = grouped;
+  }
   std::stable_sort(batch.begin(), batch.end(), [](const std::shared_ptr<SrvReq> &x, const std::shared_ptr<SrvReq> &y) {
     return x->n != y->n ? x->n < y->n : x->sqr < y->sqr;
   });
@@ -894,10 +1077,28 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
     }
     g = h;
   }
-  // products out, completion events
+  // the split products: operands in, chunk products and the image sum in the
+  // library workspace (ordered across streams by its event)
+  for (size_t i = 0; i < splits.size(); i++) {
+    SrvReq &r = *splits[i];
+    char *base = g_srv.arena + r.off;
+    u64 *da = (u64 *)base;
+    u64 *db = (u64 *)(base + align_up(r.d.na * 8));
+    u64 *dout = (u64 *)(base + align_up(r.d.na * 8) + align_up(r.d.nb * 8));
+    (void)hipMemcpyAsync(da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
+    int rc = ws_acquire(s);
+    if (!rc) rc = split_product_locked(da, r.d.na, db, r.d.nb, dout, split_plans[i], s);
+    if (!rc) rc = ws_release(s);
+    if (rc) r.err = g_err;
+    batch.push_back(splits[i]);
+  }
+  // products out, completion events (the product follows the operands in
+  // every request's arena block)
   for (size_t i = 0; i < batch.size(); i++) {
     SrvReq &r = *batch[i];
-    (void)hipMemcpyAsync(r.d.out, jobs[i].out, (r.d.na + r.d.nb) * 8, hipMemcpyDeviceToHost, s);
+    const u64 *dout = (const u64 *)(g_srv.arena + r.off + align_up(r.d.na * 8) + (r.sqr ? 0 : align_up(r.d.nb * 8)));
+    (void)hipMemcpyAsync(r.d.out, dout, (r.d.na + r.d.nb) * 8, hipMemcpyDeviceToHost, s);
     (void)hipEventRecord(r.done_ev, s);
   }
 }
@@ -1017,19 +1218,21 @@ void clear_graphs() {
   g_ctx.graphs.clear();
 }
 
-int set_choice_locked(int bits, int ncol, int dm) {
+int set_choice_locked(int bits, const Choice &c) {
+  const int ncol = c.ncol, dm = c.dm;
   if (bits < 1 || bits > kMaxBits) return fail(HGX_EINVAL, "transform bits out of range");
   if (ncol < min_ncol(bits) || ncol > max_ncol(bits))
     return fail(HGX_EINVAL, "2^" + std::to_string(bits) + " takes " + std::to_string(min_ncol(bits)) + ".." +
                                 std::to_string(max_ncol(bits)) + " column passes");
   if (dm < 12 || dm > 24) return fail(HGX_EINVAL, "direct-table limit must be 12..24 bits");
+  if (c.md != 0 && c.md != 1) return fail(HGX_EINVAL, "direct Montgomery flag must be 0 or 1");
   // products of this length may still run on any stream with the old tables
   HIPCHK(hipDeviceSynchronize());
-  const auto old = choice_for(bits);
-  g_ctx.choice[bits] = {ncol, dm};
-  if (old != std::make_pair(ncol, dm)) {
+  const Choice old = choice_for(bits);
+  g_ctx.choice[bits] = c;
+  if (old != c) {
     clear_graphs();
-    drop_plan(plan_key(bits, old.first, old.second));
+    drop_plan(plan_key(bits, old));
   }
   return HGX_OK;
 }
@@ -1041,19 +1244,19 @@ std::string device_arch() {
   return a.substr(0, a.find(':'));
 }
 
-// tuned choices persist in the file named by HGX_TUNE_CACHE: "arch bits ncol dm" lines
+// tuned choices persist in the file named by HGX_TUNE_CACHE: "arch bits ncol dm md" lines
 void tune_cache_load() {
   const char *path = getenv("HGX_TUNE_CACHE");
   if (!path) return;
   FILE *f = fopen(path, "r");
   if (!f) return;
   char arch[64];
-  int bits, ncol, dm;
+  int bits, ncol, dm, md;
   const std::string mine = device_arch();
-  while (fscanf(f, "%63s %d %d %d", arch, &bits, &ncol, &dm) == 4)
+  while (fscanf(f, "%63s %d %d %d %d", arch, &bits, &ncol, &dm, &md) == 5)
     if (mine == arch && bits >= 1 && bits <= kMaxBits && ncol >= min_ncol(bits) && ncol <= max_ncol(bits) &&
-        dm >= 12 && dm <= 24)
-      g_ctx.choice[bits] = {ncol, dm};
+        dm >= 12 && dm <= 24 && (md == 0 || md == 1))
+      g_ctx.choice[bits] = Choice{ncol, dm, md};
   fclose(f);
 }
 
@@ -1063,7 +1266,8 @@ void tune_cache_save() {
   FILE *f = fopen(path, "w");
   if (!f) return;
   const std::string arch = device_arch();
-  for (auto &kv : g_ctx.choice) fprintf(f, "%s %d %d %d\n", arch.c_str(), kv.first, kv.second.first, kv.second.second);
+  for (auto &kv : g_ctx.choice)
+    fprintf(f, "%s %d %d %d %d\n", arch.c_str(), kv.first, kv.second.ncol, kv.second.dm, kv.second.md);
   fclose(f);
 }
 
@@ -1134,7 +1338,7 @@ void hgx_shutdown(void) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   if (!g_ctx.ready) return;
   hipStreamSynchronize(g_ctx.stream);
-  for (auto &kv : g_ctx.plans) dev_free(kv.second->dev, kv.second->bytes);
+  for (auto &kv : g_ctx.plans) free_plan(*kv.second);
   g_ctx.plans.clear();
   for (auto &kv : g_ctx.graphs) (void)hipGraphExecDestroy(kv.second);
   g_ctx.graphs.clear();
@@ -1347,7 +1551,7 @@ static int addsub_device(bool sub, const uint64_t *d_a, size_t na, const uint64_
   rc = ensure(g_ctx.scan, 2 * nblk);
   if (rc) return rc;
   uint8_t *agg = (uint8_t *)g_ctx.scan.ptr;
-  hgx_launch_addsub(sub, s, d_a, na, d_b, nb, d_out, n, agg, agg + nblk);
+  hgx_launch_addsub(sub, s, d_a, na, d_b, nb, d_out, n, agg, agg + nblk, sub ? nullptr : d_out + n);
   HIPCHK(hipGetLastError());
   return ws_release(s);
 }
@@ -1423,19 +1627,21 @@ int hgx_wait(hgx_ticket ticket) {
 }
 
 // ---- pass-structure choices and the autotuner ----
-int hgx_plan_set(int bits, int ncol, int direct_max_bits) {
+int hgx_plan_set(int bits, int ncol, int direct_max_bits, int mont_direct) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   int rc = check_ready();
   if (rc) return rc;
-  return set_choice_locked(bits, ncol, direct_max_bits);
+  return set_choice_locked(bits, Choice{ncol, direct_max_bits, mont_direct});
 }
 
-int hgx_plan_get(int bits, int *ncol, int *direct_max_bits) {
+int hgx_plan_get(int bits, int *ncol, int *direct_max_bits, int *mont_direct) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
-  if (bits < 1 || bits > kMaxBits || !ncol || !direct_max_bits) return fail(HGX_EINVAL, "bad arguments");
-  const auto c = choice_for(bits);
-  *ncol = c.first;
-  *direct_max_bits = c.second;
+  if (bits < 1 || bits > kMaxBits || !ncol || !direct_max_bits || !mont_direct)
+    return fail(HGX_EINVAL, "bad arguments");
+  const Choice c = choice_for(bits);
+  *ncol = c.ncol;
+  *direct_max_bits = c.dm;
+  *mont_direct = c.md;
   return HGX_OK;
 }
 
@@ -1455,17 +1661,22 @@ int hgx_autotune(int min_bits, int max_bits, int reps) {
   HIPCHK(hipEventCreate(&e1));
   for (int n = std::max(min_bits, kRowMaxBits + 1); n <= max_bits; n++) {
     const u64 L = 1ull << n, half = L / 2;
-    // candidates: every column-pass count, and the direct-table limits that
-    // change which passes use the direct table
-    std::vector<std::pair<int, int>> cand;
+    // candidates: every column-pass count; the direct-table limits that
+    // change which passes use the direct table; direct Montgomery tables
+    // where some pass is above the direct limit and at most 2^28 words
+    std::vector<Choice> cand;
     for (int ncol = min_ncol(n); ncol <= max_ncol(n); ncol++) {
       std::vector<int> seen;
       for (int dm : {12, 16, 20, 22}) {
-        int ndirect = 0;
-        for (int i = 0; i < ncol; i++) ndirect |= (n - 8 * i <= dm) << i;
+        int ndirect = 0, nmont = 0;
+        for (int i = 0; i < ncol; i++) {
+          ndirect |= (n - 8 * i <= dm) << i;
+          nmont |= (n - 8 * i > dm && n - 8 * i <= kMontDirectMaxBits) << i;
+        }
         if (std::find(seen.begin(), seen.end(), ndirect) != seen.end()) continue;
         seen.push_back(ndirect);
-        cand.push_back({ncol, dm});
+        cand.push_back(Choice{ncol, dm, 0});
+        if (nmont) cand.push_back(Choice{ncol, dm, 1});
       }
     }
     if (cand.size() < 2) continue;
@@ -1475,9 +1686,9 @@ int hgx_autotune(int min_bits, int max_bits, int reps) {
     hgx_launch_splitmix64(dim3((unsigned)((half + 255) / 256)), s, a, half, 0x48475800 + n, 1);
     hgx_launch_splitmix64(dim3((unsigned)((half + 255) / 256)), s, b, half, 0x48475900 + n, 1);
     float best = 1e30f;
-    std::pair<int, int> pick = choice_for(n);
+    Choice pick = choice_for(n);
     for (auto &c : cand) {
-      rc = set_choice_locked(n, c.first, c.second);
+      rc = set_choice_locked(n, c);
       if (rc) break;
       float t = 1e30f;
       for (int r = 0; r <= reps && !rc; r++) {
@@ -1497,13 +1708,19 @@ int hgx_autotune(int min_bits, int max_bits, int reps) {
       // setting the next candidate drops this one's tables
     }
     if (rc) break;
-    rc = set_choice_locked(n, pick.first, pick.second);
+    rc = set_choice_locked(n, pick);
     if (rc) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   tune_cache_save();
+  return HGX_OK;
+}
+
+int hgx_split_enable(int on) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  g_ctx.split_on = on != 0;
   return HGX_OK;
 }
 

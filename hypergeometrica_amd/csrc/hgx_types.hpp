@@ -34,12 +34,17 @@ struct PrimeSet {
 // Roots of one column pass on blocks of length B = 2^b (T = B/256 columns).
 // Direct form (B <= 2^20): D[k] holds (w_B^(t q), Shoup companion) pairs at
 // index q*T + t, read coalesced along t, one Shoup product per element.
+// Direct Montgomery form (large blocks, 8 bytes per word of the block):
+// DM[k] holds w_B^(t q) (times the pass's scale) in Montgomery form at
+// index q*T + t, one REDC product per element.
 // Tile-factored form: w_B^(t q) = w_B^(t0 q) * w_B^(tt q) for the tile's
 // first column t0 = t - tt (tt < 16): TS[k] holds (w_B^(tt q), companion)
 // pairs at q*16 + tt (times the pass's scale), TT[k] holds w_B^(t0 q) in
 // Montgomery form at (t0/16)*256 + q; a Shoup product and a REDC product.
+enum { TW_FACTORED = 0, TW_DIRECT = 1, TW_DIRECT_MONT = 2 };
 struct ColTw {
   const u64 *D[3];
+  const u64 *DM[3];
   const u64 *TS[3];
   const u64 *TT[3];
   int tbits;  // log2 T

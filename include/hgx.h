@@ -130,6 +130,15 @@ int hgx_sub_device(const uint64_t *d_a, size_t na, const uint64_t *d_b,
 int hgx_fill_splitmix64(uint64_t *d_out, size_t n, uint64_t seed,
                         int force_top, void *stream);
 
+// Unbalanced products: when one operand is much longer than the other, a
+// product (single, batched or served) is computed as products of chunks of
+// the longer operand with the shorter one, all of one shorter transform
+// length, batched, and an addition of their sums -- the same integer for a
+// fraction of the work of one transform of length >= NA + NB.  No counterpart
+// in the reference (its mpz-*/ntt always transforms at that length).  ON = 0
+// turns it off.  Default: on.
+int hgx_split_enable(int on);
+
 // Batched calls (hgx_mul_batch, hgx_mul_batch_device) capture the launch
 // sequence of each chunk in a HIP graph, keyed by its shape (the transform
 // lengths and counts of its groups) and by the library's job-table and
@@ -139,13 +148,16 @@ int hgx_fill_splitmix64(uint64_t *d_out, size_t n, uint64_t seed,
 int hgx_graphs_enable(int on);
 
 // Pass structure per transform length 2^BITS: NCOL column passes of 8 bits
-// and a row pass of BITS - 8 NCOL bits (4..12), and the largest column-pass
-// block (2^DIRECT_MAX_BITS words, 12..24) that uses the direct twiddle table
-// instead of the tile-factored pair.  No counterpart in the reference (its
-// transforms are one radix-2 loop).  Every choice gives the same results;
-// the default is the fewest passes and DIRECT_MAX_BITS = 20.
-int hgx_plan_set(int bits, int ncol, int direct_max_bits);
-int hgx_plan_get(int bits, int *ncol, int *direct_max_bits);
+// and a row pass of BITS - 8 NCOL bits (4..12); the largest column-pass
+// block (2^DIRECT_MAX_BITS words, 12..24) that uses the direct twiddle table;
+// and MONT_DIRECT (0/1): whether larger blocks of at most 2^28 words use
+// direct tables of Montgomery-form roots (8 bytes per word of the block, per
+// prime and direction, built on the device) instead of the tile-factored
+// pair.  No counterpart in the reference (its transforms are one radix-2
+// loop).  Every choice gives the same results; the default is the fewest
+// passes, DIRECT_MAX_BITS = 20 and MONT_DIRECT as built.
+int hgx_plan_set(int bits, int ncol, int direct_max_bits, int mont_direct);
+int hgx_plan_get(int bits, int *ncol, int *direct_max_bits, int *mont_direct);
 
 // Autotuner: for every length 2^bits with MIN_BITS <= bits <= MAX_BITS that
 // offers a choice, time a product of two 2^(bits-1)-limb operands under each

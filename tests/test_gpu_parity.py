@@ -389,42 +389,39 @@ def test_batch_graph_replays_equal_eager(hgx, oracle):
 
 def _plan(L, bits):
     import ctypes
-    n, d = ctypes.c_int(), ctypes.c_int()
+    n, d, m = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     from hypergeometrica_amd import _lib
-    _lib.check(L.hgx_plan_get(bits, ctypes.byref(n), ctypes.byref(d)))
-    return n.value, d.value
+    _lib.check(L.hgx_plan_get(bits, ctypes.byref(n), ctypes.byref(d), ctypes.byref(m)))
+    return n.value, d.value, m.value
 
 
 @pytest.mark.parametrize("bits", [20, 21, 28])
 def test_every_pass_structure_gives_the_same_results(hgx, oracle, bits):
-    # each column-pass count and direct-table limit: products against GMP (or
-    # the mod 2^61-1 residue at 2^28), squares, and transforms against the
-    # oracle; the default is restored after
+    # each column-pass count, direct-table limit and direct Montgomery flag:
+    # products against GMP (or the mod 2^61-1 residue at 2^28), squares, and
+    # transforms against the oracle; the default is restored after.  At 2^28
+    # the direct-table limits change only the 2^20 pass: one is enough.
     from hypergeometrica_amd import _lib
     L = _lib.lib()
     default = _plan(L, bits)
     lo, hi = (bits - 12 + 7) // 8, (bits - 4) // 8
+    limits = [12, 16, 20, 22] if bits <= 21 else [12]
     try:
-        for ncol in range(lo, hi + 1):
-            for dm in [12, 16, 20, 22]:
-                say("plan", bits, ncol, dm)
-                _lib.check(L.hgx_plan_set(bits, ncol, dm))
-                assert _plan(L, bits) == (ncol, dm)
-                na = (1 << (bits - 1)) - 7
-                a = oracle.splitmix64(bits * 100 + ncol, na)
-                b = oracle.splitmix64(dm, (1 << (bits - 1)) - 3)
-                got = hgx.mul_limbs(a, b)
-                if bits <= 21:
-                    assert np.array_equal(got, _gmp.mul(a, b))
-                    sq = hgx.sqr_limbs(a)
-                    assert np.array_equal(sq, _gmp.mul(a, a))
-                    x = residues(oracle, 1, 1 << bits, 0x485 + dm)
-                    assert np.array_equal(hgx.ntt_forward(x, 1), oracle.ntt_forward(x, 1))
-                    assert np.array_equal(hgx.ntt_reverse(hgx.ntt_forward(x, 1), 1), x)
-                else:
-                    assert mod_q(got) == (mod_q(a) * mod_q(b)) % Q
-                if bits > 21 and dm == 12:
-                    break  # at 2^28 the other limits change only the 2^20 pass: one of them is enough
+        for ncol, dm, md in [(c, d, m) for c in range(lo, hi + 1) for d in limits for m in [0, 1]]:
+            say("plan", bits, ncol, dm, md)
+            _lib.check(L.hgx_plan_set(bits, ncol, dm, md))
+            assert _plan(L, bits) == (ncol, dm, md)
+            a = oracle.splitmix64(bits * 100 + ncol, (1 << (bits - 1)) - 7)
+            b = oracle.splitmix64(dm + md, (1 << (bits - 1)) - 3)
+            got = hgx.mul_limbs(a, b)
+            if bits <= 21:
+                assert np.array_equal(got, _gmp.mul(a, b))
+                assert np.array_equal(hgx.sqr_limbs(a), _gmp.mul(a, a))
+                x = residues(oracle, 1, 1 << bits, 0x485 + dm)
+                assert np.array_equal(hgx.ntt_forward(x, 1), oracle.ntt_forward(x, 1))
+                assert np.array_equal(hgx.ntt_reverse(hgx.ntt_forward(x, 1), 1), x)
+            else:
+                assert mod_q(got) == (mod_q(a) * mod_q(b)) % Q
     finally:
         _lib.check(L.hgx_plan_set(bits, *default))
 
@@ -440,13 +437,53 @@ def test_autotune_keeps_valid_choices(hgx, oracle, tmp_path, monkeypatch):
         _lib.check(L.hgx_autotune(18, 21, 1))
         lines = cache.read_text().split("\n")
         rows = [ln.split() for ln in lines if ln.strip()]
-        assert rows and all(len(r) == 4 for r in rows)
+        assert rows and all(len(r) == 5 for r in rows)
         for b in range(18, 22):
-            ncol, dm = _plan(L, b)
-            assert (b - 12 + 7) // 8 <= ncol <= (b - 4) // 8 and 12 <= dm <= 24
+            ncol, dm, md = _plan(L, b)
+            assert (b - 12 + 7) // 8 <= ncol <= (b - 4) // 8 and 12 <= dm <= 24 and md in (0, 1)
             na = (1 << (b - 1)) - 5
             a, bb = oracle.splitmix64(b, na), oracle.splitmix64(b + 50, na + 1)
             assert np.array_equal(hgx.mul_limbs(a, bb), _gmp.mul(a, bb))
     finally:
         for b, c in saved.items():
             _lib.check(L.hgx_plan_set(b, *c))
+
+
+# ---------------------------------------------------------------- unbalanced products
+
+UNBALANCED = [(300000, 3), (300000, 1500), (1 << 20, 1000), (3000000, 5000), (1 << 22, 4096), (1 << 23, 65536)]
+
+
+def test_unbalanced_products_split_equal_gmp_and_unsplit(hgx, oracle):
+    # products split into chunk products (and an image sum) against GMP and
+    # against the one-transform path, both operand orders
+    from hypergeometrica_amd import _lib
+    L = _lib.lib()
+    for na, nb in UNBALANCED:
+        say("unbalanced", na, nb)
+        a = oracle.splitmix64(na + 17, na)
+        b = oracle.splitmix64(nb + 23, nb)
+        got = hgx.mul_limbs(a, b)
+        assert np.array_equal(got, _gmp.mul(a, b)), (na, nb)
+        assert np.array_equal(hgx.mul_limbs(b, a), got)
+        _lib.check(L.hgx_split_enable(0))
+        try:
+            assert np.array_equal(hgx.mul_limbs(a, b), got)
+        finally:
+            _lib.check(L.hgx_split_enable(1))
+    # all-ones operands: carries between the two images cross every chunk
+    a = np.full(1 << 20, 2**64 - 1, dtype=np.uint64)
+    b = np.full(3000, 2**64 - 1, dtype=np.uint64)
+    A, B = 64 * a.size, 64 * b.size
+    assert oracle.limbs_to_int(hgx.mul_limbs(a, b)) == (1 << (A + B)) - (1 << A) - (1 << B) + 1
+
+
+def test_unbalanced_in_batches_and_server(hgx, oracle):
+    pairs = [(oracle.splitmix64(i, na), oracle.splitmix64(i + 50, nb))
+             for i, (na, nb) in enumerate(UNBALANCED[:4] + [(3000, 2000), (40000, 41000)])]
+    outs = hgx.mul_batch(pairs)
+    for (a, b), o in zip(pairs, outs):
+        assert np.array_equal(o, _gmp.mul(a, b))
+    tickets = [hgx.submit(a, b) for a, b in pairs]
+    for (a, b), t in zip(pairs, tickets):
+        assert np.array_equal(t.wait(), _gmp.mul(a, b))
