@@ -49,6 +49,9 @@ HGX_HD u32 mul_u24(u32 a, u32 b) {
 #endif
 }
 
+// high 32 bits of a 32 x 32-bit product (v_mul_hi_u32)
+HGX_HD u32 mulhi32(u32 a, u32 b) { return (u32)(((u64)a * b) >> 32); }
+
 // One modulus as the kernels use it: p, its negation 2^64 - p (kept as its
 // own value so that r - p compiles to a single 64-bit add, with no carry
 // through an SGPR), and the Proth factor c of p = c*2^55 + 1.
@@ -106,14 +109,18 @@ HGX_HD u64 mont_mul(u64 a, u64 b, const Mod &M) {
 // r = a*w - q*p lies in [0, 2p) for any a < 2^64 and one subtraction makes it
 // canonical.  q*p mod 2^64 = q + ((q*c mod 2^9) << 55) for p = c*2^55 + 1.
 HGX_HD u64 shoup_mul(u64 a, u64 w, u64 wq, const Mod &M) {
-  u64 qlo_unused, q;
-  mul64x64(a, wq, qlo_unused, q);
-  (void)qlo_unused;
   const u32 a0 = (u32)a, a1 = (u32)(a >> 32), w0 = (u32)w, w1 = (u32)(w >> 32);
-  // r = a*w - q*p mod 2^64: the terms that only reach the high word are
-  // gathered in 32 bits
-  const u32 hiterm = a0 * w1 + a1 * w0 - (mul_u24((u32)q & 511u, M.c) << 23);
-  const u64 r = ((u64)a0 * w0 - q) + ((u64)hiterm << 32);
+  const u32 q0 = (u32)wq, q1 = (u32)(wq >> 32);
+  // q = hi64(a*wq): the middle column is split over two multiply-adds so
+  // that no sum leaves 64 bits
+  const u64 x = (u64)a1 * q0 + mulhi32(a0, q0);
+  const u64 y = (u64)a0 * q1 + (u32)x;
+  const u64 q = (u64)a1 * q1 + (x >> 32) + (y >> 32);
+  // r = a*w - q*p mod 2^64 with q*p = q + ((q*c mod 2^9) << 55).  The terms
+  // that reach only the high word are summed in 32 bits; (q mod 2^24) *
+  // (2^24 - c) << 23 is -(q*c mod 2^9) << 23 mod 2^32, so q needs no mask.
+  const u32 h = a0 * w1 + a1 * w0 + (mul_u24((u32)q, (1u << 24) - M.c) << 23);
+  const u64 r = (u64)a0 * w0 + (((u64)h << 32) - q);
   return csub(r, M);
 }
 
