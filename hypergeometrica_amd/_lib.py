@@ -22,7 +22,7 @@ EXPORTS = (
     "hgx_mul_device", "hgx_sqr_device", "hgx_mul_batch_device", "hgx_ntt_forward",
     "hgx_ntt_reverse", "hgx_submit", "hgx_poll", "hgx_wait", "hgx_fill_splitmix64",
     "hgx_timing_enable", "hgx_timing_read", "hgx_add_device", "hgx_sub_device", "hgx_graphs_enable",
-    "hgx_plan_set", "hgx_plan_get", "hgx_autotune", "hgx_split_enable",
+    "hgx_plan_set", "hgx_plan_get", "hgx_autotune", "hgx_split_enable", "hgx_staging_limit",
 )
 
 
@@ -76,6 +76,7 @@ def load() -> ctypes.CDLL:
         _ip = ctypes.POINTER(ctypes.c_int)
         L.hgx_plan_get.argtypes = [ctypes.c_int, _ip, _ip, _ip]
         L.hgx_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.hgx_staging_limit.argtypes = [_sz]
         L.hgx_sub_device.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
         L.hgx_mul_batch_device.argtypes = [ctypes.POINTER(MulDesc), _sz, _vp]
         L.hgx_ntt_forward.argtypes = [_vp, _sz, ctypes.c_int]

@@ -158,6 +158,9 @@ struct Context {
   bool split_on = true;  // unbalanced products as batched chunk products
   // captured launch sequences of batched chunks (hipGraph), keyed by shape
   bool graphs_on = true;
+  // cap on the device memory of one host-memory product before it is staged
+  // in chunks (0: the HBM budget alone)
+  std::atomic<size_t> stage_limit{0};
   std::map<std::vector<u64>, hipGraphExec_t> graphs;
   std::recursive_mutex mu;
 };
@@ -779,6 +782,146 @@ int mul_device_locked(const u64 *d_a, size_t na, const u64 *d_b, size_t nb, u64 
   return ws_release(s);
 }
 
+// ---- products beyond the memory available to them (host-staged) ----
+// A host-memory product whose device working set (both operands, the
+// product, one transform's work arrays and its root tables) exceeds the
+// memory it may use -- the HBM budget left, capped by hgx_staging_limit -- is
+// computed from M-limb chunks a_i, b_j of the operands.  The product is
+// produced diagonal by diagonal: c_s = sum over i + j = s of a_i * b_j, each
+// chunk product on the GPU (transform length 2M) and summed there by the
+// carry-scan addition into a window W of 2M + 2 limbs that starts at limb
+// s*M.  After diagonal s the low M limbs of W are final and go to the host;
+// the next diagonal's first addition reads W from limb M on into the other
+// window.  With D pairs per diagonal, W < 2D * 2^(128M) by induction
+// (W_s = c_s + floor(W_(s-1) / 2^(64M)), c_s < D * 2^(128M)), so 2M + 1 limbs
+// hold it.  Only two chunks, one chunk product, the windows and one
+// transform's work arrays are in HBM at a time; operands and product stay in
+// host memory.  This takes the place of the reference's disk-backed vectors
+// for numbers beyond memory (src/ram-vec.lisp:31-97, src/disk-vec.lisp:78-91):
+// the result is the same exact product.
+constexpr u64 kStageMinChunk = 1ull << 12;
+constexpr size_t kStageSlack = (size_t)8 << 20;  // allocation granules, job tables
+
+// device bytes of the root tables of length 2^n still to be built
+size_t plan_bytes_needed(int n) {
+  const Choice c = choice_for(n);
+  if (g_ctx.plans.count(plan_key(n, c))) return 0;
+  size_t words = 0, big = 0;
+  for (int i = 0; i < c.ncol; i++) {
+    const int b = n - 8 * i;
+    const size_t B = (size_t)1 << b, tables = (i == 0) ? 3 : 2;
+    if (b <= c.dm) {
+      words += 3 * tables * 2 * B;
+    } else {
+      words += 3 * (tables * 2 * 4096 + 2 * (B >> 4));
+      if (c.md && b <= kMontDirectMaxBits) big += 3 * 2 * B * sizeof(u64);
+    }
+  }
+  return align_up(words * sizeof(u64)) + big;
+}
+
+// device bytes of one product computed whole from host memory
+size_t inhbm_bytes(size_t na, size_t nb, bool sqr) {
+  const int n = log2_ceil((u64)hgx_product_length(na, nb));
+  return align_up(na * 8) + (sqr ? 0 : align_up(nb * 8)) + align_up((na + nb) * 8) + work_layout(n, sqr).total +
+         plan_bytes_needed(n);
+}
+
+// device bytes of the staged product with M-limb chunks
+size_t staged_bytes(u64 M) {
+  const int n = log2_ceil(2 * M);
+  return 2 * align_up(M * 8) + align_up(2 * M * 8) + 2 * align_up((2 * M + 2) * 8) + work_layout(n, false).total +
+         align_up(2 * hgx_addsub_blocks(2 * M + 2)) + plan_bytes_needed(n);
+}
+
+// device memory a host-memory product may use: the budget not held by other
+// allocations (the workspaces it reuses count as free), capped by the limit
+size_t memory_for_product() {
+  const size_t reusable = g_ctx.ws.bytes + g_ctx.stage.bytes + g_ctx.scan.bytes;
+  const size_t held = g_ctx.allocated - reusable;
+  size_t avail = g_ctx.budget > held ? g_ctx.budget - held : 0;
+  const size_t limit = g_ctx.stage_limit.load();
+  if (limit) avail = std::min(avail, limit);
+  return avail;
+}
+
+bool needs_staging(size_t na, size_t nb, bool sqr) {
+  return inhbm_bytes(na, nb, sqr) + kStageSlack > memory_for_product();
+}
+
+int mul_staged_locked(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, bool sqr) {
+  if (na < nb) {
+    std::swap(a, b);
+    std::swap(na, nb);
+  }
+  const size_t avail = memory_for_product();
+  u64 M = 1ull << log2_ceil(na);
+  while (M > kStageMinChunk && staged_bytes(M) + kStageSlack > avail) M >>= 1;
+  if (staged_bytes(M) + kStageSlack > avail)
+    return fail(HGX_ENOMEM, "HBM budget too small even for a host-staged product: " + std::to_string(avail) +
+                                " bytes available, " + std::to_string(staged_bytes(M) + kStageSlack) + " needed");
+  const u64 ka = (na + M - 1) / M, kb = (nb + M - 1) / M, nout = na + nb, wlen = 2 * M + 2;
+  hipStream_t s = g_ctx.stream;
+  int rc = ws_acquire(s);
+  if (rc) return rc;
+  const size_t cb = align_up(M * 8), pb = align_up(2 * M * 8), wb = align_up(wlen * 8);
+  rc = ensure(g_ctx.stage, 2 * cb + pb + 2 * wb);
+  if (rc) return rc;
+  const u64 nblk = hgx_addsub_blocks(wlen);
+  rc = ensure(g_ctx.scan, 2 * nblk);
+  if (rc) return rc;
+  char *base = (char *)g_ctx.stage.ptr;
+  u64 *ca = (u64 *)base, *cbf = (u64 *)(base + cb), *P = (u64 *)(base + 2 * cb);
+  u64 *W[2] = {(u64 *)(base + 2 * cb + pb), (u64 *)(base + 2 * cb + pb + wb)};
+  uint8_t *agg = (uint8_t *)g_ctx.scan.ptr, *blkin = agg + nblk;
+  // chunk products are balanced: no split (its images would not fit the estimate)
+  struct SplitOff {
+    bool was = g_ctx.split_on;
+    SplitOff() { g_ctx.split_on = false; }
+    ~SplitOff() { g_ctx.split_on = was; }
+  } split_off;
+  u64 in_a = ~0ull, in_b = ~0ull;  // chunks now in the device buffers
+  int cur = 0;
+  for (u64 d = 0; d + 1 < ka + kb; d++) {
+    u64 *Wc = W[cur], *Wp = W[cur ^ 1];
+    bool first = true;
+    const u64 ilo = d >= kb - 1 ? d - (kb - 1) : 0, ihi = std::min<u64>(d, ka - 1);
+    for (u64 i = ilo; i <= ihi; i++) {
+      const u64 j = d - i;
+      if (sqr && j < i) break;  // a square takes each unordered pair once
+      const u64 mi = std::min<u64>(M, na - i * M), mj = std::min<u64>(M, nb - j * M);
+      const bool sq = sqr && i == j;
+      if (in_a != i) {
+        HIPCHK(hipMemcpyAsync(ca, a + i * M, mi * 8, hipMemcpyHostToDevice, s));
+        in_a = i;
+      }
+      if (!sq && in_b != j) {
+        HIPCHK(hipMemcpyAsync(cbf, b + j * M, mj * 8, hipMemcpyHostToDevice, s));
+        in_b = j;
+      }
+      rc = mul_device_locked(ca, mi, sq ? ca : cbf, sq ? mi : mj, P, sq, s);
+      if (rc) return rc;
+      for (int t = 0; t < ((sqr && !sq) ? 2 : 1); t++) {
+        HGX_TIMED("stage_add", s);
+        if (first) {  // W = (previous window from limb M) + P
+          hgx_launch_addsub(false, s, d ? Wp + M : nullptr, d ? M + 2 : 0, P, mi + (sq ? mi : mj), Wc, wlen, agg,
+                            blkin, nullptr);
+          first = false;
+        } else {
+          hgx_launch_addsub(false, s, Wc, wlen, P, mi + (sq ? mi : mj), Wc, wlen, agg, blkin, nullptr);
+        }
+      }
+    }
+    HIPCHK(hipGetLastError());
+    const u64 at = d * M;
+    const u64 cnt = (d + 2 == ka + kb) ? nout - at : M;
+    HIPCHK(hipMemcpyAsync(out + at, Wc, cnt * 8, hipMemcpyDeviceToHost, s));
+    cur ^= 1;
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return ws_release(s);
+}
+
 // Replay the captured launch sequence for KEY on stream S, capturing it on
 // first use (PREPARE runs first, outside the capture).  The job table the
 // kernels read is copied before every replay, so a graph depends only on the
@@ -1126,11 +1269,23 @@ void server_loop() {
     std::vector<std::shared_ptr<SrvReq>> round;
     while (!g_srv.queue.empty()) {
       auto r = g_srv.queue.front();
-      if (r->bytes > g_srv.arena_bytes) {
-        r->state = HGX_ENOMEM;
-        r->err = "request needs " + std::to_string(r->bytes) + " bytes, the serving arena holds " +
-                 std::to_string(g_srv.arena_bytes);
+      const size_t limit = g_ctx.stage_limit.load();
+      if (r->bytes > g_srv.arena_bytes || (limit && r->bytes > limit)) {
+        // larger than the arena: a host-staged product, here in the server
+        // thread (rounds already in flight keep running on their streams)
         g_srv.queue.pop_front();
+        r->state = 1;
+        lk.unlock();
+        int rc;
+        std::string err;
+        {
+          std::lock_guard<std::recursive_mutex> lk2(g_ctx.mu);
+          rc = mul_staged_locked(r->d.a, r->d.na, r->d.b, r->d.nb, r->d.out, r->sqr);
+          if (rc) err = g_err;
+        }
+        lk.lock();
+        r->err = err;
+        r->state = rc ? rc : 2;
         g_srv.cv.notify_all();
         continue;
       }
@@ -1396,6 +1551,8 @@ int hgx_sqr_device(const uint64_t *d_a, size_t na, uint64_t *d_out, void *stream
 
 // host-memory product: stage operands into device memory, multiply, copy back
 static int mul_host_locked(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out, bool sqr) {
+  if (log2_ceil((u64)hgx_product_length(na, nb)) > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32");
+  if (needs_staging(na, nb, sqr)) return mul_staged_locked(a, na, b, nb, out, sqr);
   hipStream_t s = g_ctx.stream;
   const size_t nout = na + nb;
   const size_t bytes_a = align_up(na * 8), bytes_b = sqr ? 0 : align_up(nb * 8), bytes_o = align_up(nout * 8);
@@ -1436,52 +1593,67 @@ int hgx_mul_batch(const hgx_mul_desc *descs, size_t count) {
   if (rc) return rc;
   if (count == 0) return HGX_OK;
   if (!descs) return fail(HGX_EINVAL, "null descriptor array");
-  // pack every operand into one device buffer, run the device batch, unpack
-  size_t in_words = 0, out_words = 0;
   for (size_t i = 0; i < count; i++) {
     const hgx_mul_desc &d = descs[i];
     if (!d.a || !d.b || !d.out || d.na == 0 || d.nb == 0) return fail(HGX_EINVAL, "bad descriptor " + std::to_string(i));
+    if (log2_ceil((u64)hgx_product_length(d.na, d.nb)) > kMaxBits)
+      return fail(HGX_ETOOBIG, "transform length above 2^32 in descriptor " + std::to_string(i));
+  }
+  // products too large for the device memory are staged one by one; the
+  // others are packed into one device buffer, run as a device batch, unpacked
+  std::vector<size_t> rest;
+  size_t in_words = 0, out_words = 0;
+  for (size_t i = 0; i < count; i++) {
+    const hgx_mul_desc &d = descs[i];
     const bool sqr = (d.a == d.b && d.na == d.nb);
+    if (needs_staging(d.na, d.nb, sqr)) {
+      rc = mul_staged_locked(d.a, d.na, d.b, d.nb, d.out, sqr);
+      if (rc) return rc;
+      continue;
+    }
+    rest.push_back(i);
     in_words += d.na + (sqr ? 0 : d.nb);
     out_words += d.na + d.nb;
   }
+  if (rest.empty()) return HGX_OK;
   hipStream_t s = g_ctx.stream;
   rc = ws_acquire(s);
   if (rc) return rc;
   rc = ensure(g_ctx.pack, align_up((in_words + out_words) * 8));
   if (rc) return rc;
   std::vector<u64> hin(in_words);
-  std::vector<hgx_mul_desc> dd(count);
+  std::vector<hgx_mul_desc> dd(rest.size());
   u64 *din = (u64 *)g_ctx.pack.ptr, *dout = din + in_words;
   size_t at = 0, ot = 0;
-  for (size_t i = 0; i < count; i++) {
-    const hgx_mul_desc &d = descs[i];
+  for (size_t k = 0; k < rest.size(); k++) {
+    const hgx_mul_desc &d = descs[rest[k]];
     const bool sqr = (d.a == d.b && d.na == d.nb);
     std::memcpy(&hin[at], d.a, d.na * 8);
-    dd[i].a = din + at;
-    dd[i].na = d.na;
+    dd[k].a = din + at;
+    dd[k].na = d.na;
     at += d.na;
     if (sqr) {
-      dd[i].b = dd[i].a;
+      dd[k].b = dd[k].a;
     } else {
       std::memcpy(&hin[at], d.b, d.nb * 8);
-      dd[i].b = din + at;
+      dd[k].b = din + at;
       at += d.nb;
     }
-    dd[i].nb = d.nb;
-    dd[i].out = dout + ot;
+    dd[k].nb = d.nb;
+    dd[k].out = dout + ot;
     ot += d.na + d.nb;
   }
   HIPCHK(hipMemcpyAsync(din, hin.data(), in_words * 8, hipMemcpyHostToDevice, s));
-  rc = batch_device_locked(dd.data(), count, s);
+  rc = batch_device_locked(dd.data(), dd.size(), s);
   if (rc) return rc;
   std::vector<u64> hout(out_words);
   HIPCHK(hipMemcpyAsync(hout.data(), dout, out_words * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   ot = 0;
-  for (size_t i = 0; i < count; i++) {
-    std::memcpy(descs[i].out, &hout[ot], (descs[i].na + descs[i].nb) * 8);
-    ot += descs[i].na + descs[i].nb;
+  for (size_t k = 0; k < rest.size(); k++) {
+    const hgx_mul_desc &d = descs[rest[k]];
+    std::memcpy(d.out, &hout[ot], (d.na + d.nb) * 8);
+    ot += d.na + d.nb;
   }
   return HGX_OK;
 }
@@ -1715,6 +1887,11 @@ int hgx_autotune(int min_bits, int max_bits, int reps) {
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   tune_cache_save();
+  return HGX_OK;
+}
+
+int hgx_staging_limit(size_t bytes) {
+  g_ctx.stage_limit.store(bytes);
   return HGX_OK;
 }
 

@@ -130,6 +130,19 @@ int hgx_sub_device(const uint64_t *d_a, size_t na, const uint64_t *d_b,
 int hgx_fill_splitmix64(uint64_t *d_out, size_t n, uint64_t seed,
                         int force_top, void *stream);
 
+// Host-staged products, for operands and products beyond the device memory:
+// a product from host memory (hgx_mul, hgx_sqr, hgx_mul_batch, hgx_submit)
+// whose device working set -- operands, product, one transform's work
+// arrays and root tables -- exceeds the memory available to it is computed
+// from chunks of the operands, diagonal by diagonal of the chunk products,
+// with two chunks, one chunk product and a window of the result in HBM at a
+// time.  Same exact product.  The successor of the reference's disk-backed
+// vectors for numbers beyond memory (make-storage, src/mpz-ram.lisp:12-17;
+// disk-vec, src/disk-vec.lisp:78-91).  BYTES caps the device memory of one
+// such product below the HBM budget; 0 (the default) leaves the budget as
+// the only cap.
+int hgx_staging_limit(size_t bytes);
+
 // Unbalanced products: when one operand is much longer than the other, a
 // product (single, batched or served) is computed as products of chunks of
 // the longer operand with the shorter one, all of one shorter transform

@@ -487,3 +487,76 @@ def test_unbalanced_in_batches_and_server(hgx, oracle):
     tickets = [hgx.submit(a, b) for a, b in pairs]
     for (a, b), t in zip(pairs, tickets):
         assert np.array_equal(t.wait(), _gmp.mul(a, b))
+
+
+# ---------------------------------------------------------------- host-staged products
+# A limit on the device memory of one product forces the chunked path that
+# products beyond HBM take; the kernel timing shows its window additions.
+
+STAGED = [(300001, 200003), (1 << 20, 1 << 20), (1 << 21, 5), (1 << 21, 70000), (777777, 777776)]
+
+
+@pytest.fixture
+def staging_limit(hgx):
+    from hypergeometrica_amd import _lib
+    L = _lib.lib()
+    yield lambda nbytes: _lib.check(L.hgx_staging_limit(nbytes))
+    _lib.check(L.hgx_staging_limit(0))
+
+
+def _mul_staged(hgx, a, b):
+    from hypergeometrica_amd import _lib
+    _lib.timing_enable(True)
+    try:
+        got = hgx.mul_limbs(a, b)
+        assert "stage_add" in _lib.timing_read(), "the product was not staged"
+    finally:
+        _lib.timing_enable(False)
+    return got
+
+
+def test_staged_products_equal_gmp(hgx, oracle, staging_limit):
+    staging_limit(16 << 20)
+    for na, nb in STAGED:
+        say("staged", na, nb)
+        a = oracle.splitmix64(na + 31, na)
+        b = oracle.splitmix64(nb + 37, nb)
+        got = _mul_staged(hgx, a, b)
+        assert np.array_equal(got, _gmp.mul(a, b)), (na, nb)
+        assert np.array_equal(_mul_staged(hgx, b, a), got)
+    say("staged square")
+    a = oracle.splitmix64(99, 600001)
+    assert np.array_equal(_mul_staged(hgx, a, a), _gmp.mul(a, a))
+    # all-ones operands: carries run through every window
+    say("staged all-ones")
+    a = np.full(700000, 2**64 - 1, dtype=np.uint64)
+    b = np.full(650001, 2**64 - 1, dtype=np.uint64)
+    A, B = 64 * a.size, 64 * b.size
+    assert oracle.limbs_to_int(_mul_staged(hgx, a, b)) == (1 << (A + B)) - (1 << A) - (1 << B) + 1
+    assert oracle.limbs_to_int(_mul_staged(hgx, a, a)) == (1 << (2 * A)) - (1 << (A + 1)) + 1
+
+
+def test_staged_in_batches_and_server(hgx, oracle, staging_limit):
+    staging_limit(16 << 20)
+    pairs = [(oracle.splitmix64(i, na), oracle.splitmix64(i + 70, nb))
+             for i, (na, nb) in enumerate([(400000, 300000), (3000, 2000), (1 << 20, 4000), (40000, 41000)])]
+    outs = hgx.mul_batch(pairs)
+    for (a, b), o in zip(pairs, outs):
+        assert np.array_equal(o, _gmp.mul(a, b))
+    tickets = [hgx.submit(a, b) for a, b in pairs]
+    for (a, b), t in zip(pairs, tickets):
+        assert np.array_equal(t.wait(), _gmp.mul(a, b))
+
+
+def test_staged_headline_size_equals_in_hbm(hgx, oracle, staging_limit):
+    # 2^26 x 2^26 limbs within 1 GiB of device memory: identical to the
+    # in-HBM product, which needs about 8 GiB
+    n = 1 << 26
+    say("staged 2^26 x 2^26")
+    a = oracle.splitmix64(0x48475801, n)
+    b = oracle.splitmix64(0x48475802, n)
+    whole = hgx.mul_limbs(a, b)
+    staging_limit(1 << 30)
+    got = _mul_staged(hgx, a, b)
+    assert np.array_equal(got, whole)
+    assert mod_q(got) == (mod_q(a) * mod_q(b)) % Q
