@@ -8,7 +8,7 @@
 // registers, one LDS exchange, stages 3-0) and rows g + 16 j go straight
 // back, in place on the residue array of prime kbase + y.
 // grid = (jobs << jt_bits, nprimes)
-template <int FORM>
+template <int FORM, bool BIG = false>
 __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
                                                       int kbase) {
   __shared__ u64 lds[HGX_TILE_LDS];
@@ -20,17 +20,21 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
   const u64 blk = tile >> tpb_bits;
   const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
   const u64 rs = 1ull << (bbits - 8);
-  const u64 gbase = (blk << bbits) + t0 + t;
   // LDS slots (padded, HGX_PAD) of rows g + 16 j and 16 g + j of column t
   const int l1 = 17 * g + t, l2 = 272 * g + t;
+  // word offsets of rows g + 16 j (o1 + 16 rs j) and 16 g + j (o2 + rs j)
+  // from the tile's first column; BIG: blocks above 2^28 words
+  const u32 o1 = t + (u32)rs * g;
+  const u64 o2 = t + rs * 16 * g;
   const int k = kbase + blockIdx.y;
   const Mod M = mod_of(ps, k);
-  u64 *arr = J.res0 + (u64)blockIdx.y * L;
+  gu64 *arr = gmem(J.res0 + (u64)blockIdx.y * L + (blk << bbits) + t0);
   u64 x[16];
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     const u32 m = 16 * g + j;
-    x[j] = apply_twiddle<FORM>(arr[gbase + rs * m], tw, k, t0 + t, bitrev8(m), M);
+    const u64 v = BIG ? (u64)arr[rs * j + o2] : (u64)gat(arr + rs * j, (u32)o2);
+    x[j] = apply_twiddle<FORM>(v, tw, k, (u32)t0 + t, bitrev8(m), M);
   }
   round_regs<8, 4, 4, true>(x, 0, ps.tw_inv[k], M);
 #pragma unroll
@@ -40,7 +44,7 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
   for (int j = 0; j < 16; j++) x[j] = lds[l1 + 272 * j];
   round_regs<8, 0, 4, true>(x, g, ps.tw_inv[k], M);
 #pragma unroll
-  for (int j = 0; j < 16; j++) arr[gbase + rs * (u64)(g + 16 * j)] = x[j];
+  for (int j = 0; j < 16; j++) gat(arr + 16 * rs * j, o1) = x[j];
 }
 
 // CRT and carry sums.  A workgroup takes a block of min(L, 1024) consecutive
@@ -68,9 +72,9 @@ __global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_
   const int nruns = P / rl;
   const bool act = tid < nruns;
   const u64 base = blk * HGX_CRT_BLOCK;
+  const gu64 *r0 = gmem(J.res0 + base), *r1 = gmem(J.res0 + L + base), *r2 = gmem(J.res0 + 2 * L + base);
   for (int x = tid; x < P; x += HGX_NT) {
-    const u64 pos = base + x;
-    const U192 cf = garner3(J.res0[pos], J.res0[L + pos], J.res0[2 * L + pos], ps);
+    const U192 cf = garner3(gat(r0, x), gat(r1, x), gat(r2, x), ps);
     w0[HGX_PAD(x)] = cf.w0;
     w1[HGX_PAD(x)] = cf.w1;
     w2[HGX_PAD(x)] = cf.w2;
@@ -155,10 +159,10 @@ __global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_
   for (int j = 0; j < HGX_RUN; j++)
     if (act && j < rl) w0[HGX_PAD(tid * rl + j)] = d[j];
   const int allones = __syncthreads_and(andall == ~0ull);
-  for (int x = tid; x < P; x += HGX_NT) {
-    const u64 pos = base + x;
-    if (pos < J.nout) J.out[pos] = w0[HGX_PAD(x)];
-  }
+  gu64 *outb = gmem(J.out + base);
+  const u64 nleft = J.nout > base ? J.nout - base : 0;  // product limbs from this block on
+  for (int x = tid; x < P; x += HGX_NT)
+    if ((u64)x < nleft) gat(outb, x) = w0[HGX_PAD(x)];
   if (tid == 0) J.flags[blk] = allones ? 1u : 0u;
 }
 
@@ -267,9 +271,15 @@ static int tw_form(const ColTw &tw) { return tw.D[0] ? TW_DIRECT : (tw.DM[0] ? T
 
 void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr,
                         int jt_bits, u64 L, int bbits, int kbase) {
+  const bool big = bbits > 28;  // tile-factored twiddles only (direct forms stop at 2^28)
   switch (tw_form(tw) + (first ? 3 : 0)) {
     case TW_FACTORED:
-      hipLaunchKernelGGL((k_fwd_col<false, TW_FACTORED>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      if (big)
+        hipLaunchKernelGGL((k_fwd_col<false, TW_FACTORED, true>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits,
+                           kbase);
+      else
+        hipLaunchKernelGGL((k_fwd_col<false, TW_FACTORED>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits,
+                           kbase);
       break;
     case TW_DIRECT:
       hipLaunchKernelGGL((k_fwd_col<false, TW_DIRECT>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
@@ -279,7 +289,12 @@ void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps
                          kbase);
       break;
     case 3 + TW_FACTORED:
-      hipLaunchKernelGGL((k_fwd_col<true, TW_FACTORED>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      if (big)
+        hipLaunchKernelGGL((k_fwd_col<true, TW_FACTORED, true>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits,
+                           kbase);
+      else
+        hipLaunchKernelGGL((k_fwd_col<true, TW_FACTORED>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits,
+                           kbase);
       break;
     case 3 + TW_DIRECT:
       hipLaunchKernelGGL((k_fwd_col<true, TW_DIRECT>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
@@ -295,7 +310,10 @@ void hgx_launch_inv_col(dim3 grid, hipStream_t s, const PrimeSet &ps, const ColT
                         u64 L, int bbits, int kbase) {
   switch (tw_form(tw)) {
     case TW_FACTORED:
-      hipLaunchKernelGGL(k_inv_col<TW_FACTORED>, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      if (bbits > 28)
+        hipLaunchKernelGGL((k_inv_col<TW_FACTORED, true>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
+      else
+        hipLaunchKernelGGL((k_inv_col<TW_FACTORED>), grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);
       break;
     case TW_DIRECT:
       hipLaunchKernelGGL(k_inv_col<TW_DIRECT>, grid, dim3(HGX_NT), 0, s, ps, tw, jr, jt_bits, L, bbits, kbase);

@@ -40,16 +40,19 @@ __device__ __forceinline__ u32 bitrev8(u32 m) { return __builtin_bitreverse32(m)
 // x * w_B^(t q) for column t, frequency q: one Shoup product from the direct
 // table, one REDC product from the direct Montgomery table, or a Shoup and a
 // REDC product from the tile-factored tables.
+// The tables are indexed with 32-bit byte offsets: the direct tables hold
+// 2^bbits pairs with bbits <= 22, the direct Montgomery tables 2^bbits words
+// with bbits <= 28, the tile tables 2^(bbits-4) words.
 template <int FORM>
-__device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u64 t, u32 q, const Mod &M) {
+__device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u32 t, u32 q, const Mod &M) {
   if constexpr (FORM == TW_DIRECT) {
-    const ulonglong2 w = *(const ulonglong2 *)(tw.D[k] + 2 * (((u64)q << tw.tbits) + t));
+    const ulonglong2 w = gat2(gmem(tw.D[k]), (q << tw.tbits) + t);
     return shoup_mul(x, w.x, w.y, M);
   } else if constexpr (FORM == TW_DIRECT_MONT) {
-    return mont_mul(x, tw.DM[k][((u64)q << tw.tbits) + t], M);
+    return mont_mul(x, gat(gmem(tw.DM[k]), (q << tw.tbits) + t), M);
   } else {
-    const ulonglong2 s = *(const ulonglong2 *)(tw.TS[k] + 2 * ((q << 4) + (u32)(t & 15)));
-    const u64 f = tw.TT[k][((t >> 4) << 8) + q];
+    const ulonglong2 s = gat2(gmem(tw.TS[k]), (q << 4) + (t & 15));
+    const u64 f = gat(gmem(tw.TT[k]), ((t >> 4) << 8) + q);
     return mont_mul(shoup_mul(x, s.x, s.y, M), f, M);
   }
 }
@@ -256,7 +259,10 @@ __device__ __forceinline__ U192 garner3(u64 r0, u64 r1, u64 r2, const PrimeSet &
 // into residue array y.  Otherwise: in place on the residue array of prime
 // kbase + y.
 // grid = (jobs << jt_bits, nprimes, noperands)
-template <bool FIRST, int FORM>
+// BIG: blocks above 2^28 words (tile-factored twiddles only), whose in-block
+// offsets need 64 bits; otherwise every access is a uniform base plus a
+// 32-bit per-thread offset.
+template <bool FIRST, int FORM, bool BIG = false>
 __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
                                                       int kbase) {
   // The 256-point column DFT is two rounds of 16: round 1 (stages 0-3) takes
@@ -272,28 +278,33 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
   const u64 blk = tile >> tpb_bits;
   const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
   const u64 rs = 1ull << (bbits - 8);
-  const u64 gbase = (blk << bbits) + t0 + t;
+  const u64 cbase = (blk << bbits) + t0;  // the tile's first column in the block
   // LDS slots (padded, HGX_PAD) of rows g + 16 j and 16 g + j of column t
   const int l1 = 17 * g + t, l2 = 272 * g + t;
+  // word offsets of rows g + 16 j (o1 + 16 rs j) and 16 g + j (o2 + rs j) of
+  // column t from the tile's first column
+  const u32 o1 = t + (u32)rs * g;
+  const u64 o2 = t + rs * 16 * g;
   const int op = blockIdx.z;
   u64 *res = op ? J.res1 : J.res0;
   u64 x[16];
 
   const int k = kbase + blockIdx.y;
   const Mod M = mod_of(ps, k);
-  u64 *arr = res + (u64)blockIdx.y * L;
+  gu64 *arr = gmem(res + (u64)blockIdx.y * L + cbase);
   if (FIRST) {
     // limbs -> residues mod p_k (the limbs are read once per prime)
     const u64 *limbs = op ? J.b : J.a;
     const u64 n = op ? J.nb : J.na;
+    const gu64 *lb = gmem(limbs + cbase);
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-      const u64 pos = gbase + rs * (u64)(g + 16 * j);
-      x[j] = mod_reduce64((pos < n) ? limbs[pos] : 0, M);
+      const u64 pos = cbase + 16 * rs * j + o1;
+      x[j] = mod_reduce64((pos < n) ? gat(lb + 16 * rs * j, o1) : 0, M);
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < 16; j++) x[j] = arr[gbase + rs * (u64)(g + 16 * j)];
+    for (int j = 0; j < 16; j++) x[j] = gat(arr + 16 * rs * j, o1);
   }
   round_regs<8, 0, 4, false>(x, g, ps.tw_fwd[k], M);
 #pragma unroll
@@ -305,7 +316,11 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     const u32 m = 16 * g + j;
-    arr[gbase + rs * m] = apply_twiddle<FORM>(x[j], tw, k, t0 + t, bitrev8(m), M);
+    const u64 v = apply_twiddle<FORM>(x[j], tw, k, (u32)t0 + t, bitrev8(m), M);
+    if (BIG)
+      arr[rs * j + o2] = v;
+    else
+      gat(arr + rs * j, (u32)o2) = v;
   }
 }
 

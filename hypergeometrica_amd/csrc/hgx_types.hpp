@@ -16,6 +16,34 @@
 
 enum { MODE_MUL = 0, MODE_SQR = 1, MODE_FWD = 2, MODE_INV = 3 };
 
+// HBM arrays as global-address-space pointers: accesses through them compile
+// to global loads and stores (a uniform base in SGPRs plus a 32-bit VGPR
+// offset where the offset fits) instead of flat accesses, which take 64-bit
+// VGPR addresses and also count against the LDS wait counter.
+// gat(base, i): word i of a global array whose offsets in bytes stay below
+// 2^32 (i < 2^29), addressed as base + a 32-bit byte offset.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) u64 gu64;
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ gu64 *gmem(u64 *p) { return (gu64 *)p; }
+__device__ __forceinline__ const gu64 *gmem(const u64 *p) { return (const gu64 *)p; }
+__device__ __forceinline__ gu64 &gat(gu64 *base, u32 i) { return *(gu64 *)((gchar *)base + (i << 3)); }
+__device__ __forceinline__ const gu64 &gat(const gu64 *base, u32 i) {
+  return *(const gu64 *)((const gchar *)base + (i << 3));
+}
+// pair i (16 bytes) of a global table of pairs
+__device__ __forceinline__ ulonglong2 gat2(const gu64 *base, u32 i) {
+  return *(const __attribute__((address_space(1))) ulonglong2 *)((const gchar *)base + (i << 4));
+}
+#else
+typedef u64 gu64;
+HGX_HD u64 *gmem(u64 *p) { return p; }
+HGX_HD const u64 *gmem(const u64 *p) { return p; }
+HGX_HD u64 &gat(u64 *base, u32 i) { return base[i]; }
+HGX_HD const u64 &gat(const u64 *base, u32 i) { return base[i]; }
+HGX_HD ulonglong2 gat2(const u64 *base, u32 i) { return *(const ulonglong2 *)(base + 2 * (size_t)i); }
+#endif
+
 // Moduli and in-tile root tables.  tw_fwd[k] holds the pairs
 // (w_4096^i, floor(w_4096^i * 2^64 / p_k)) for i < 2048 -- the root and its
 // Shoup companion, as the reference's roots table holds (w, w~)
@@ -76,7 +104,17 @@ struct JobRef {
   const MulJob *table;  // null: every block works on ONE
 };
 
-__device__ __forceinline__ MulJob load_job(const JobRef &jr, u32 j) { return jr.table ? jr.table[j] : jr.one; }
+// The job table does not change during a launch: read through the constant
+// address space, the uniform job lands in SGPRs (scalar loads), and so do the
+// array bases derived from it.
+__device__ __forceinline__ MulJob load_job(const JobRef &jr, u32 j) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (jr.table) return ((const __attribute__((address_space(4))) MulJob *)jr.table)[j];
+  return jr.one;
+#else
+  return jr.table ? jr.table[j] : jr.one;
+#endif
+}
 
 // Parameters of a row pass (see k_row).
 struct RowParams {
