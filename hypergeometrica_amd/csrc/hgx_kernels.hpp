@@ -89,17 +89,37 @@ struct Log2<1> { static constexpr int value = 0; };
 // Butterflies of one round on the 2^R values of one group, in registers:
 // x[j] is element base + j*D of the group, and goff = base mod D selects the
 // twiddles (w_2h^e from the w_4096 table, e = goff + i*D).
-template <int S_BITS, int A, int R, bool INV>
+// ZHI (forward only): the upper half x[RR/2 ..] is zero and the lower half
+// holds unreduced 64-bit words, so the first stage is x[j + RR/2] = x[j] w
+// (Shoup's product takes any 64-bit multiplicand) and x[j] = x[j] mod p.
+template <int S_BITS, int A, int R, bool INV, bool ZHI = false>
 __device__ __forceinline__ void round_regs(u64 *x, int goff, const u64 *__restrict__ tw, const Mod &M) {
   constexpr int S = 1 << S_BITS;
   constexpr int D = S >> (A + R);
   constexpr int RR = 1 << R;
+  static_assert(!(ZHI && INV), "zero upper half is a forward-only case");
   if (!INV) {
 #pragma unroll
     for (int l = 0; l < R; l++) {
       const int half = 1 << (R - 1 - l);
       const int h = half * D;
       const int tstride = 2048 / h;
+      if (ZHI && l == 0) {
+#pragma unroll
+        for (int j = 0; j < half; j++) {
+          const u64 u = x[j];
+          if (D == 1 && j == 0) {
+            x[j + half] = mod_reduce64(u, M);  // twiddle w^0 = 1
+          } else {
+            const int e = goff + j * D;
+            const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
+            x[j + half] = shoup_mul(u, w.x, w.y, M);
+          }
+          x[j] = mod_reduce64(u, M);
+        }
+        HGX_STAGE_FENCE();
+        continue;
+      }
 #pragma unroll
       for (int j = 0; j < RR; j++) {
         if (j & half) continue;
@@ -297,16 +317,28 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
     const u64 *limbs = op ? J.b : J.a;
     const u64 n = op ? J.nb : J.na;
     const gu64 *lb = gmem(limbs + cbase);
+    // rows 128.. of the block (j >= 8 here) lie at or above the operand's
+    // length: the first stage has a zero upper half
+    if ((blk << bbits) + (rs << 7) >= n) {
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const u64 pos = cbase + 16 * rs * j + o1;
-      x[j] = mod_reduce64((pos < n) ? gat(lb + 16 * rs * j, o1) : 0, M);
+      for (int j = 0; j < 8; j++) {
+        const u64 pos = cbase + 16 * rs * j + o1;
+        x[j] = (pos < n) ? gat(lb + 16 * rs * j, o1) : 0;
+      }
+      round_regs<8, 0, 4, false, true>(x, g, ps.tw_fwd[k], M);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const u64 pos = cbase + 16 * rs * j + o1;
+        x[j] = mod_reduce64((pos < n) ? gat(lb + 16 * rs * j, o1) : 0, M);
+      }
+      round_regs<8, 0, 4, false>(x, g, ps.tw_fwd[k], M);
     }
   } else {
 #pragma unroll
     for (int j = 0; j < 16; j++) x[j] = gat(arr + 16 * rs * j, o1);
+    round_regs<8, 0, 4, false>(x, g, ps.tw_fwd[k], M);
   }
-  round_regs<8, 0, 4, false>(x, g, ps.tw_fwd[k], M);
 #pragma unroll
   for (int j = 0; j < 16; j++) lds[l1 + 272 * j] = x[j];
   __syncthreads();
@@ -342,13 +374,40 @@ struct RowRound {
   }
 };
 
+// One forward round on the same groups of two row tiles: both tiles' values
+// are read from LDS before either is transformed, so the second read's
+// latency hides behind the first tile's butterflies.
+template <int S_BITS, int I>
+__device__ __forceinline__ void row_round_two(u64 *ldsA, u64 *ldsB, int wbits, const u64 *__restrict__ tw,
+                                              const Mod &M) {
+  using RI = RowRound<S_BITS, I>;
+#pragma unroll 1
+  for (int gid = threadIdx.x; gid < (1 << (wbits + RI::GPC_BITS)); gid += HGX_NT) {
+    int row, goff, base;
+    RI::where(gid, row, goff, base);
+    u64 a[RI::RR], b[RI::RR];
+#pragma unroll
+    for (int j = 0; j < RI::RR; j++) a[j] = ldsA[HGX_PAD(row + base + j * RI::D)];
+#pragma unroll
+    for (int j = 0; j < RI::RR; j++) b[j] = ldsB[HGX_PAD(row + base + j * RI::D)];
+    round_regs<S_BITS, RI::A, RI::R, false>(a, goff, tw, M);
+#pragma unroll
+    for (int j = 0; j < RI::RR; j++) ldsA[HGX_PAD(row + base + j * RI::D)] = a[j];
+    round_regs<S_BITS, RI::A, RI::R, false>(b, goff, tw, M);
+#pragma unroll
+    for (int j = 0; j < RI::RR; j++) ldsB[HGX_PAD(row + base + j * RI::D)] = b[j];
+  }
+}
+
 // Forward rounds I .. I1 in LDS on one or two tiles, a barrier after each.
 template <int S_BITS, int I, int I1, bool TWO>
 struct FwdRange {
   static __device__ __forceinline__ void run(u64 *ldsA, u64 *ldsB, int wbits, const u64 *tw, const Mod &M) {
     if constexpr (I <= I1) {
-      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), false, false>(ldsA, wbits, tw, M);
-      if (TWO) tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), false, false>(ldsB, wbits, tw, M);
+      if constexpr (TWO)
+        row_round_two<S_BITS, I>(ldsA, ldsB, wbits, tw, M);
+      else
+        tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), false, false>(ldsA, wbits, tw, M);
       __syncthreads();
       FwdRange<S_BITS, I + 1, I1, TWO>::run(ldsA, ldsB, wbits, tw, M);
     }
@@ -399,23 +458,26 @@ __device__ __forceinline__ void row_mul_tile(u64 *A, const u64 *Bp, u64 *ldsA, u
       for (int j = 0; j < R0::RR; j++) A[row + base + j * R0::D] = a[j];
     }
   } else {
-    // forward round 0 from HBM
+    // forward round 0 from HBM; both operands' loads are issued before the
+    // first butterfly, so the second operand's HBM latency is hidden
 #pragma unroll 1
     for (int gid = tid; gid < (1 << (wbits + R0::GPC_BITS)); gid += HGX_NT) {
       int row, goff, base;
       R0::where(gid, row, goff, base);
-      u64 x[R0::RR];
+      u64 x[R0::RR], y[R0::RR];
 #pragma unroll
       for (int j = 0; j < R0::RR; j++) x[j] = A[row + base + j * R0::D];
+      if (TWO) {
+#pragma unroll
+        for (int j = 0; j < R0::RR; j++) y[j] = Bp[row + base + j * R0::D];
+      }
       round_regs<S_BITS, 0, R0::R, false>(x, goff, twf, M);
 #pragma unroll
       for (int j = 0; j < R0::RR; j++) ldsA[HGX_PAD(row + base + j * R0::D)] = x[j];
       if (TWO) {
+        round_regs<S_BITS, 0, R0::R, false>(y, goff, twf, M);
 #pragma unroll
-        for (int j = 0; j < R0::RR; j++) x[j] = Bp[row + base + j * R0::D];
-        round_regs<S_BITS, 0, R0::R, false>(x, goff, twf, M);
-#pragma unroll
-        for (int j = 0; j < R0::RR; j++) ldsB[HGX_PAD(row + base + j * R0::D)] = x[j];
+        for (int j = 0; j < R0::RR; j++) ldsB[HGX_PAD(row + base + j * R0::D)] = y[j];
       }
     }
     __syncthreads();
@@ -428,12 +490,12 @@ __device__ __forceinline__ void row_mul_tile(u64 *A, const u64 *Bp, u64 *ldsA, u
       u64 a[RL::RR], b[RL::RR];
 #pragma unroll
       for (int j = 0; j < RL::RR; j++) a[j] = ldsA[HGX_PAD(row + base + j * RL::D)];
-      round_regs<S_BITS, RL::A, RL::R, false>(a, goff, twf, M);
       if (TWO) {
 #pragma unroll
         for (int j = 0; j < RL::RR; j++) b[j] = ldsB[HGX_PAD(row + base + j * RL::D)];
-        round_regs<S_BITS, RL::A, RL::R, false>(b, goff, twf, M);
       }
+      round_regs<S_BITS, RL::A, RL::R, false>(a, goff, twf, M);
+      if (TWO) round_regs<S_BITS, RL::A, RL::R, false>(b, goff, twf, M);
 #pragma unroll
       for (int j = 0; j < RL::RR; j++) a[j] = mont_mul(a[j], TWO ? b[j] : a[j], M);
       round_regs<S_BITS, RL::A, RL::R, true>(a, goff, twi, M);
