@@ -325,20 +325,24 @@ void hgx_launch_inv_col(dim3 grid, hipStream_t s, const PrimeSet &ps, const ColT
 }
 
 // direct Montgomery table of one pass from its tile-factored pair:
-// DM[q*T + t] = TT[t/16][q] * TS[q][t mod 16] (Montgomery form times plain
-// value: the product stays in Montgomery form)
-__global__ __launch_bounds__(256) void k_direct_mont(u64 *DM, const u64 *TS, const u64 *TT, int tbits, Mod M) {
+// DM[q*T + t] = TT[t/16][q] * TS[q][t mod 16] * 2^64 mod p (both factors are
+// plain values with Shoup companions; rm = 2^64 mod p with companion rmq
+// turns the product into Montgomery form)
+__global__ __launch_bounds__(256) void k_direct_mont(u64 *DM, const u64 *TS, const u64 *TT, int tbits, Mod M, u64 rm,
+                                                     u64 rmq) {
   const u64 total = 256ull << tbits;
   for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < total; i += (u64)gridDim.x * 256) {
     const u32 q = (u32)(i >> tbits);
     const u64 t = i & ((1ull << tbits) - 1);
     const ulonglong2 w = *(const ulonglong2 *)(TS + 2 * ((q << 4) + (u32)(t & 15)));
-    DM[i] = shoup_mul(TT[((t >> 4) << 8) + q], w.x, w.y, M);
+    const u64 f = TT[2 * (((t >> 4) << 8) + q)];
+    DM[i] = shoup_mul(shoup_mul(f, w.x, w.y, M), rm, rmq, M);
   }
 }
 
-void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M) {
-  hipLaunchKernelGGL(k_direct_mont, dim3(4096), dim3(256), 0, s, DM, TS, TT, tbits, M);
+void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M, u64 rm,
+                            u64 rmq) {
+  hipLaunchKernelGGL(k_direct_mont, dim3(4096), dim3(256), 0, s, DM, TS, TT, tbits, M, rm, rmq);
 }
 
 void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &jr, int bpj_bits, u64 L) {

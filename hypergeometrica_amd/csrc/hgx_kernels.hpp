@@ -38,11 +38,11 @@
 __device__ __forceinline__ u32 bitrev8(u32 m) { return __builtin_bitreverse32(m) >> 24; }
 
 // x * w_B^(t q) for column t, frequency q: one Shoup product from the direct
-// table, one REDC product from the direct Montgomery table, or a Shoup and a
-// REDC product from the tile-factored tables.
+// table, one REDC product from the direct Montgomery table, or two Shoup
+// products from the tile-factored tables.
 // The tables are indexed with 32-bit byte offsets: the direct tables hold
 // 2^bbits pairs with bbits <= 22, the direct Montgomery tables 2^bbits words
-// with bbits <= 28, the tile tables 2^(bbits-4) words.
+// with bbits <= 28, the tile tables 2^(bbits-4) pairs with bbits <= 32.
 template <int FORM>
 __device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u32 t, u32 q, const Mod &M) {
   if constexpr (FORM == TW_DIRECT) {
@@ -52,8 +52,8 @@ __device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u32 
     return mont_mul(x, gat(gmem(tw.DM[k]), (q << tw.tbits) + t), M);
   } else {
     const ulonglong2 s = gat2(gmem(tw.TS[k]), (q << 4) + (t & 15));
-    const u64 f = gat(gmem(tw.TT[k]), ((t >> 4) << 8) + q);
-    return mont_mul(shoup_mul(x, s.x, s.y, M), f, M);
+    const ulonglong2 f = gat2(gmem(tw.TT[k]), ((t >> 4) << 8) + q);
+    return shoup_mul(shoup_mul(x, s.x, s.y, M), f.x, f.y, M);
   }
 }
 
@@ -243,12 +243,12 @@ __device__ __forceinline__ U192 garner3(u64 r0, u64 r1, u64 r2, const PrimeSet &
   const Mod M1 = mod_of(ps, 1), M2 = mod_of(ps, 2);
   // v1 = (r1 - r0) p0^-1 mod p1
   const u64 r0m1 = (r0 >= p1) ? r0 - p1 : r0;
-  const u64 v1 = mont_mul(mod_sub(r1, r0m1, M1), ps.g01, M1);
+  const u64 v1 = shoup_mul(mod_sub(r1, r0m1, M1), ps.g01, ps.g01q, M1);
   // v2 = ((r2 - r0) p0^-1 - v1) p1^-1 mod p2
   const u64 r0m2 = (r0 >= p2) ? r0 - p2 : r0;
   const u64 v1m2 = (v1 >= p2) ? v1 - p2 : v1;
-  u64 t = mont_mul(mod_sub(r2, r0m2, M2), ps.g02, M2);
-  const u64 v2 = mont_mul(mod_sub(t, v1m2, M2), ps.g12, M2);
+  u64 t = shoup_mul(mod_sub(r2, r0m2, M2), ps.g02, ps.g02q, M2);
+  const u64 v2 = shoup_mul(mod_sub(t, v1m2, M2), ps.g12, ps.g12q, M2);
   // u = v1 + p1 * v2  (< p1 p2 < 2^126)
   u64 ulo, uhi;
   mul64x64(p1, v2, ulo, uhi);

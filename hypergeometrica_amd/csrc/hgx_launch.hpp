@@ -28,4 +28,5 @@ void hgx_launch_addsub(bool sub, hipStream_t s, const u64 *a, u64 na, const u64 
                        uint8_t *agg, uint8_t *blkin, u64 *carry_out);
 void hgx_launch_fill(hipStream_t s, u64 *p, u64 n, u64 v);
 u64 hgx_addsub_blocks(u64 n);
-void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M);
+void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M, u64 rm,
+                            u64 rmq);

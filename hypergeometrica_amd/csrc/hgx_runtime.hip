@@ -61,6 +61,18 @@ u64 powmod(u64 a, u64 e, u64 m) {
 u64 invmod(u64 a, u64 m) { return powmod(a, m - 2, m); }  // m prime
 u64 to_mont(u64 a, u64 m) { return (u64)((((u128)(a % m)) << 64) % m); }
 u64 shoup_companion(u64 w, u64 m) { return (u64)((((u128)w) << 64) / m); }  // calc-b-inv
+// m^-1 mod 2^64 for odd m (Newton's iteration doubles the correct low bits)
+u64 inv_pow2_64(u64 m) {
+  u64 x = m;
+  for (int i = 0; i < 6; i++) x *= 2 - m * x;
+  return x;
+}
+// T * 2^-64 mod m for T < m * 2^64 (REDC; nm = -m^-1 mod 2^64)
+u64 redc_host(u128 T, u64 m, u64 nm) {
+  const u64 q = (u64)T * nm;
+  const u64 r = (u64)((T + (u128)q * m) >> 64);
+  return r >= m ? r - m : r;
+}
 
 bool is_prime_small(u64 n) {
   if (n < 2) return false;
@@ -321,19 +333,24 @@ int build_plan_variant(int n, const Choice &choice, Plan **out) {
         }
         return at;
       };
-      // tile table: root^(16 c q) in Montgomery form at c*256 + q, c < T/16
-      // (built with host REDC products: every value stays in Montgomery form)
+      // tile table: root^(16 c q) at c*256 + q, c < T/16, as (value, Shoup
+      // companion) pairs.  The loop runs in Montgomery form: x~ = value * 2^64
+      // mod p, so value * 2^64 = companion * p + x~ and the companion is the
+      // exact quotient -x~ * p^-1 mod 2^64 (no 128-bit division).
       auto push_tile = [&](u64 root) {
         size_t at = host.size();
+        const u64 pinv = inv_pow2_64(p), npinv = 0 - pinv;
+        auto mm = [&](u64 a, u64 b) { return redc_host((u128)a * b, p, npinv); };
         const u64 r16 = to_mont(powmod(root, 16, p), p);
         u64 base = to_mont(1, p);
         for (size_t cidx = 0; cidx < T / 16; cidx++) {
           u64 x = to_mont(1, p);
           for (int q = 0; q < 256; q++) {
-            host.push_back(x);
-            x = mont_mul(x, base, M);
+            host.push_back(mm(x, 1));
+            host.push_back((0 - x) * pinv);
+            x = mm(x, base);
           }
-          base = mont_mul(base, r16, M);
+          base = mm(base, r16);
         }
         return at;
       };
@@ -384,6 +401,8 @@ int build_plan_variant(int n, const Choice &choice, Plan **out) {
   }
   // direct Montgomery tables, built on the device from the factored pairs
   if (choice.md) {
+    u64 rmont[3];  // 2^64 mod p_k: turns a plain value into Montgomery form
+    for (int k = 0; k < 3; k++) rmont[k] = to_mont(1, kModuli[k]);
     for (int i = 0; i < plan->ncol; i++) {
       ColPassTables &c = plan->col[i];
       if (c.direct || c.bbits > kMontDirectMaxBits) continue;
@@ -401,7 +420,7 @@ int build_plan_variant(int n, const Choice &choice, Plan **out) {
           plan->big.push_back({dmt, bytes});
           const u64 *TS = dir == 0 ? c.TSf[k] : (i == 0 ? c.TSi_mul[k] : c.TSi[k]);
           const u64 *TT = dir == 0 ? c.TTf[k] : c.TTi[k];
-          hgx_launch_direct_mont(g_ctx.stream, dmt, TS, TT, c.bbits - 8, M);
+          hgx_launch_direct_mont(g_ctx.stream, dmt, TS, TT, c.bbits - 8, M, rmont[k], shoup_companion(rmont[k], M.p));
           (dir == 0 ? c.DMf : c.DMi)[k] = dmt;
         }
       }
@@ -813,7 +832,7 @@ size_t plan_bytes_needed(int n) {
     if (b <= c.dm) {
       words += 3 * tables * 2 * B;
     } else {
-      words += 3 * (tables * 2 * 4096 + 2 * (B >> 4));
+      words += 3 * (tables * 2 * 4096 + 4 * (B >> 4));
       if (c.md && b <= kMontDirectMaxBits) big += 3 * 2 * B * sizeof(u64);
     }
   }
@@ -1472,9 +1491,12 @@ int hgx_init(int device, size_t hbm_budget_bytes) {
       y = mulmod(y, wi, p);
     }
   }
-  ps.g01 = to_mont(invmod(kModuli[0] % kModuli[1], kModuli[1]), kModuli[1]);
-  ps.g02 = to_mont(invmod(kModuli[0] % kModuli[2], kModuli[2]), kModuli[2]);
-  ps.g12 = to_mont(invmod(kModuli[1] % kModuli[2], kModuli[2]), kModuli[2]);
+  ps.g01 = invmod(kModuli[0] % kModuli[1], kModuli[1]);
+  ps.g02 = invmod(kModuli[0] % kModuli[2], kModuli[2]);
+  ps.g12 = invmod(kModuli[1] % kModuli[2], kModuli[2]);
+  ps.g01q = shoup_companion(ps.g01, kModuli[1]);
+  ps.g02q = shoup_companion(ps.g02, kModuli[2]);
+  ps.g12q = shoup_companion(ps.g12, kModuli[2]);
   int rc = dev_alloc((void **)&g_ctx.tables, host.size() * sizeof(u64));
   if (rc) return rc;
   HIPCHK(hipMemcpy(g_ctx.tables, host.data(), host.size() * sizeof(u64), hipMemcpyHostToDevice));

@@ -55,8 +55,10 @@ struct PrimeSet {
   u32 c[3];
   const u64 *tw_fwd[3];
   const u64 *tw_inv[3];
-  // Garner constants in Montgomery form: p0^-1 mod p1, p0^-1 mod p2, p1^-1 mod p2
+  // Garner constants p0^-1 mod p1, p0^-1 mod p2, p1^-1 mod p2 with their
+  // Shoup companions
   u64 g01, g02, g12;
+  u64 g01q, g02q, g12q;
 };
 
 // Roots of one column pass on blocks of length B = 2^b (T = B/256 columns).
@@ -67,8 +69,8 @@ struct PrimeSet {
 // index q*T + t, one REDC product per element.
 // Tile-factored form: w_B^(t q) = w_B^(t0 q) * w_B^(tt q) for the tile's
 // first column t0 = t - tt (tt < 16): TS[k] holds (w_B^(tt q), companion)
-// pairs at q*16 + tt (times the pass's scale), TT[k] holds w_B^(t0 q) in
-// Montgomery form at (t0/16)*256 + q; a Shoup product and a REDC product.
+// pairs at q*16 + tt (times the pass's scale), TT[k] holds (w_B^(t0 q),
+// companion) pairs at (t0/16)*256 + q; two Shoup products.
 enum { TW_FACTORED = 0, TW_DIRECT = 1, TW_DIRECT_MONT = 2 };
 struct ColTw {
   const u64 *D[3];
