@@ -30,11 +30,21 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
   const Mod M = mod_of(ps, k);
   gu64 *arr = gmem(J.res0 + (u64)blockIdx.y * L + (blk << bbits) + t0);
   u64 x[16];
+  if constexpr (FORM == TW_FACTORED) {
+    __shared__ ulonglong2 tts[256];
+    tts[tid] = gat2(gmem(tw.TT[k]), (u32)((t0 >> 4) << 8) + tid);
 #pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const u32 m = 16 * g + j;
-    const u64 v = BIG ? (u64)arr[rs * j + o2] : (u64)gat(arr + rs * j, (u32)o2);
-    x[j] = apply_twiddle<FORM>(v, tw, k, (u32)t0 + t, bitrev8(m), M);
+    for (int j = 0; j < 16; j++) x[j] = BIG ? (u64)arr[rs * j + o2] : (u64)gat(arr + rs * j, (u32)o2);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = apply_twiddle_tt(x[j], tw, k, (u32)t0 + t, bitrev8(16 * g + j), tts, M);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const u32 m = 16 * g + j;
+      const u64 v = BIG ? (u64)arr[rs * j + o2] : (u64)gat(arr + rs * j, (u32)o2);
+      x[j] = apply_twiddle<FORM>(v, tw, k, (u32)t0 + t, bitrev8(m), M);
+    }
   }
   round_regs<8, 4, 4, true>(x, 0, ps.tw_inv[k], M);
 #pragma unroll

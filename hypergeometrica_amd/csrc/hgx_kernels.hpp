@@ -57,6 +57,15 @@ __device__ __forceinline__ u64 apply_twiddle(u64 x, const ColTw &tw, int k, u32 
   }
 }
 
+// Tile-factored twiddle with the tile's 256 w_B^(t0 q) pairs staged in LDS
+// (tts[q]); t0 is the same for every column of a tile.
+__device__ __forceinline__ u64 apply_twiddle_tt(u64 x, const ColTw &tw, int k, u32 t, u32 q, const ulonglong2 *tts,
+                                                const Mod &M) {
+  const ulonglong2 s = gat2(gmem(tw.TS[k]), (q << 4) + (t & 15));
+  const ulonglong2 f = tts[q];
+  return shoup_mul(shoup_mul(x, s.x, s.y, M), f.x, f.y, M);
+}
+
 // ---------------------------------------------------------------------------
 // In-tile rounds.  A round does R radix-2 stages (A .. A+R-1 of S_BITS) on
 // groups of 2^R elements spaced D = S >> (A+R) rows apart, one group per
@@ -278,7 +287,10 @@ __device__ __forceinline__ U192 garner3(u64 r0, u64 r1, u64 r2, const PrimeSet &
 // reduced mod prime y (as %make-ntt-work, src/number-theoretic-transform.lisp:7-22)
 // into residue array y.  Otherwise: in place on the residue array of prime
 // kbase + y.
-// grid = (jobs << jt_bits, nprimes, noperands)
+// grid = (jobs << jt_bits, nprimes, noperands); FIRST: grid =
+// (3 * (jobs << jt_bits), 1, noperands) with the prime in blockIdx.x mod 3.
+// Tile-factored twiddles: the tile's 256 pairs w_B^(t0 q) are staged in LDS
+// by one coalesced load at the start.
 // BIG: blocks above 2^28 words (tile-factored twiddles only), whose in-block
 // offsets need 64 bits; otherwise every access is a uniform base plus a
 // 32-bit per-thread offset.
@@ -290,15 +302,21 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
   // 16 g + j, which go straight back to HBM with their twiddles; one LDS
   // exchange between the rounds.  Every HBM access is a 128-byte row segment.
   __shared__ u64 lds[HGX_TILE_LDS];
+  __shared__ ulonglong2 tts[FORM == TW_FACTORED ? 256 : 1];
   const int tid = threadIdx.x;
   const int t = tid & 15, g = tid >> 4;
-  const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
-  const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
+  // FIRST: the three primes of a tile are consecutive blocks (grid.y = 1), so
+  // the limbs they share are read from HBM close together in time
+  const u32 bx = FIRST ? blockIdx.x / 3 : blockIdx.x;
+  const int ky = FIRST ? (int)(blockIdx.x - 3 * bx) : (int)blockIdx.y;
+  const MulJob J = load_job(jr, bx >> jt_bits);
+  const u64 tile = bx & ((1u << jt_bits) - 1);
   const int tpb_bits = bbits - 12;  // tiles per block
   const u64 blk = tile >> tpb_bits;
   const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
   const u64 rs = 1ull << (bbits - 8);
   const u64 cbase = (blk << bbits) + t0;  // the tile's first column in the block
+  if constexpr (FORM == TW_FACTORED) tts[tid] = gat2(gmem(tw.TT[kbase + ky]), (u32)((t0 >> 4) << 8) + tid);
   // LDS slots (padded, HGX_PAD) of rows g + 16 j and 16 g + j of column t
   const int l1 = 17 * g + t, l2 = 272 * g + t;
   // word offsets of rows g + 16 j (o1 + 16 rs j) and 16 g + j (o2 + rs j) of
@@ -309,9 +327,9 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
   u64 *res = op ? J.res1 : J.res0;
   u64 x[16];
 
-  const int k = kbase + blockIdx.y;
+  const int k = kbase + ky;
   const Mod M = mod_of(ps, k);
-  gu64 *arr = gmem(res + (u64)blockIdx.y * L + cbase);
+  gu64 *arr = gmem(res + (u64)ky * L + cbase);
   if (FIRST) {
     // limbs -> residues mod p_k (the limbs are read once per prime)
     const u64 *limbs = op ? J.b : J.a;
@@ -348,7 +366,11 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
 #pragma unroll
   for (int j = 0; j < 16; j++) {
     const u32 m = 16 * g + j;
-    const u64 v = apply_twiddle<FORM>(x[j], tw, k, (u32)t0 + t, bitrev8(m), M);
+    u64 v;
+    if constexpr (FORM == TW_FACTORED)
+      v = apply_twiddle_tt(x[j], tw, k, (u32)t0 + t, bitrev8(m), tts, M);
+    else
+      v = apply_twiddle<FORM>(x[j], tw, k, (u32)t0 + t, bitrev8(m), M);
     if (BIG)
       arr[rs * j + o2] = v;
     else

@@ -576,7 +576,7 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     const unsigned grid = njobs << jt;
     {
       HGX_TIMED("fwd_col_first", s);
-      hgx_launch_fwd_col(true, dim3(grid, 3, nops), s, g_ctx.ps, coltw(plan->col[0], 0), jr, jt, L,
+      hgx_launch_fwd_col(true, dim3(grid * 3, 1, nops), s, g_ctx.ps, coltw(plan->col[0], 0), jr, jt, L,
                          plan->col[0].bbits, 0);
     }
     for (int i = 1; i < plan->ncol; i++) {
