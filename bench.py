@@ -53,7 +53,7 @@ def kernel_bytes(name: str, L: int, na: int, nb: int, nops: int) -> float | None
     return None
 
 
-def cpu_baseline(sample_bits: int = 20) -> dict:
+def cpu_baseline(sample_bits: int = 22) -> dict:
     """Time the oracle's mpz-*/ntt restatement on one 2^sample_bits x
     2^sample_bits multiply, single-threaded."""
     from oracle import oracle as O
@@ -347,7 +347,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--bits", type=int, default=26, help="operand size 2^bits limbs each")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-bits", type=int, default=20)
+    ap.add_argument("--cpu-bits", type=int, default=22)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-server leg")
     ap.add_argument("--no-latency", action="store_true", help="skip the 2^16 x 2^16 per-call latency leg")
