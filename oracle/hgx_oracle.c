@@ -25,6 +25,7 @@
 // when the restatement is wrong.
 #include "hgx_oracle.h"
 
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -581,6 +582,89 @@ size_t hgxo_mpz_mul_ntt(const uint64_t *x, size_t nx, const uint64_t *y,
   crt_and_carry(nx_work, length, out);
   for (int k = 0; k < 3; k++) { free(nx_work[k]); free(ny_work[k]); }
   return length;
+}
+
+// mpz-*/ntt with the reference's :hypergeometrica-parallel feature:
+// with-parallel-work / with-task (src/ntt-multiply.lisp:60-86) submit one task
+// per (operand, modulus) for the forward transforms (:202-209), one per modulus
+// for the pointwise products (:214-220) and one per modulus for the inverse
+// transforms (:229-234), joining after each group; residue preparation and the
+// CRT stay serial (:178-179, :242-252).  At most NTHREADS tasks run at once
+// (lparallel's pool, src/config.lisp:18).  Same product as hgxo_mpz_mul_ntt.
+typedef struct {
+  uint64_t *a;
+  const uint64_t *b;
+  size_t n;
+  int k;
+} par_task;
+
+static void *par_forward(void *p) {
+  par_task *t = (par_task *)p;
+  hgxo_ntt_forward(t->a, t->n, t->k);
+  return NULL;
+}
+
+static void *par_pointwise(void *p) {
+  par_task *t = (par_task *)p;
+  const hgxo_scheme *s = hgxo_default_scheme();
+  for (size_t i = 0; i < t->n; i++)
+    t->a[i] = hgxo_m_mul_fast(t->a[i], t->b[i], s->moduli[t->k], s->inverses[t->k]);
+  return NULL;
+}
+
+static void *par_reverse(void *p) {
+  par_task *t = (par_task *)p;
+  hgxo_ntt_reverse(t->a, t->n, t->k);
+  return NULL;
+}
+
+// run NTASKS tasks, at most NTHREADS at a time; 0 on success
+static int run_tasks(void *(*fn)(void *), par_task *tasks, int ntasks, int nthreads) {
+  pthread_t th[6];
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 6) nthreads = 6;
+  for (int i = 0; i < ntasks; i += nthreads) {
+    const int m = (ntasks - i < nthreads) ? ntasks - i : nthreads;
+    int started = 0, rc = 0;
+    for (int j = 0; j < m; j++) {
+      if (pthread_create(&th[j], NULL, fn, &tasks[i + j]) != 0) {
+        rc = -1;
+        break;
+      }
+      started++;
+    }
+    for (int j = 0; j < started; j++) pthread_join(th[j], NULL);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+size_t hgxo_mpz_mul_ntt_parallel(const uint64_t *x, size_t nx, const uint64_t *y,
+                                 size_t ny, uint64_t *out, int nthreads) {
+  const hgxo_scheme *s = hgxo_default_scheme();
+  size_t length = hgxo_product_length(nx, ny);
+  int num_moduli = hgxo_num_moduli_needed_for_bits(s, hgxo_convolution_bits(length));
+  if (num_moduli != 3) return 0;
+  uint64_t *nx_work[3], *ny_work[3];
+  for (int k = 0; k < 3; k++) {
+    nx_work[k] = make_ntt_work(x, nx, length, s->moduli[k]);
+    ny_work[k] = make_ntt_work(y, ny, length, s->moduli[k]);
+  }
+  par_task fwd[6], pw[3], inv[3];
+  for (int k = 0; k < 3; k++) {
+    fwd[2 * k] = (par_task){nx_work[k], NULL, length, k};
+    fwd[2 * k + 1] = (par_task){ny_work[k], NULL, length, k};
+    pw[k] = (par_task){nx_work[k], ny_work[k], length, k};
+    inv[k] = (par_task){nx_work[k], NULL, length, k};
+  }
+  size_t rc = length;
+  if (run_tasks(par_forward, fwd, 6, nthreads) || run_tasks(par_pointwise, pw, 3, nthreads) ||
+      run_tasks(par_reverse, inv, 3, nthreads))
+    rc = 0;
+  else
+    crt_and_carry(nx_work, length, out);
+  for (int k = 0; k < 3; k++) { free(nx_work[k]); free(ny_work[k]); }
+  return rc;
 }
 
 // mpz-square, src/ntt-multiply.lisp:101-172: length = least power of two

@@ -86,6 +86,11 @@ void hgxo_ntt_reverse(uint64_t *a, size_t n, int mod_num);
    storage is the full transform length); returns LENGTH. */
 size_t hgxo_mpz_mul_ntt(const uint64_t *x, size_t nx, const uint64_t *y,
                         size_t ny, uint64_t *out);
+/* The NTT multiply with :hypergeometrica-parallel: one thread per (operand, modulus)
+   forward task, per modulus pointwise and inverse task, at most NTHREADS (<= 6)
+   at once (src/ntt-multiply.lisp:60-86, :202-234).  0 if a thread fails. */
+size_t hgxo_mpz_mul_ntt_parallel(const uint64_t *x, size_t nx, const uint64_t *y,
+                                 size_t ny, uint64_t *out, int nthreads);
 size_t hgxo_mpz_square(const uint64_t *x, size_t nx, uint64_t *out);
 size_t hgxo_product_length(size_t nx, size_t ny);
 

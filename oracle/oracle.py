@@ -74,6 +74,8 @@ def lib():
         L.hgxo_ntt_reverse.argtypes = [_u64p, _sz, ctypes.c_int]
         L.hgxo_mpz_mul_ntt.argtypes = [_u64p, _sz, _u64p, _sz, _u64p]
         L.hgxo_mpz_mul_ntt.restype = _sz
+        L.hgxo_mpz_mul_ntt_parallel.argtypes = [_u64p, _sz, _u64p, _sz, _u64p, ctypes.c_int]
+        L.hgxo_mpz_mul_ntt_parallel.restype = _sz
         L.hgxo_mpz_square.argtypes = [_u64p, _sz, _u64p]
         L.hgxo_mpz_square.restype = _sz
         L.hgxo_product_length.argtypes = [_sz, _sz]
@@ -189,6 +191,18 @@ def mpz_mul_ntt(x: np.ndarray, y: np.ndarray) -> np.ndarray:
     out = np.zeros(L, dtype=np.uint64)
     got = lib().hgxo_mpz_mul_ntt(_p(x), x.size, _p(y), y.size, _p(out))
     assert got == L
+    return out
+
+
+def mpz_mul_ntt_parallel(x: np.ndarray, y: np.ndarray, threads: int = 6) -> np.ndarray:
+    """mpz-*/ntt with the reference's parallel tasks (src/ntt-multiply.lisp:60-86,
+    :202-234), at most THREADS at once; returns all L limbs."""
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    y = np.ascontiguousarray(y, dtype=np.uint64)
+    L = product_length(x.size, y.size)
+    out = np.zeros(L, dtype=np.uint64)
+    got = lib().hgxo_mpz_mul_ntt_parallel(_p(x), x.size, _p(y), y.size, _p(out), threads)
+    assert got == L, "thread creation failed"
     return out
 
 

@@ -49,6 +49,19 @@ def test_products_vs_gmp(oracle):
         assert np.array_equal(got[: 2 * n], _gmp.mul(a, b))
 
 
+@pytest.mark.parametrize("threads", [1, 2, 6])
+def test_parallel_tasks_same_product(oracle, threads):
+    """The with-parallel-work form (tasks per operand and modulus,
+    src/ntt-multiply.lisp:60-86, :202-234) gives the serial product and the
+    exact one."""
+    for na, nb in [(1, 1), (101, 7), (3000, 2000), (40000, 25000)]:
+        a = oracle.splitmix64(0x5100 + na, na)
+        b = oracle.splitmix64(0x5200 + nb, nb)
+        got = oracle.mpz_mul_ntt_parallel(a, b, threads)
+        assert np.array_equal(got, oracle.mpz_mul_ntt(a, b)), (na, nb)
+        assert oracle.limbs_to_int(got[: na + nb]) == exact(a, b)
+
+
 def test_reference_style_random_mpz(oracle):
     # test-mpz-plus-minus-times-randomly (tests/mpz.lisp:32-47): ~333-bit values
     rng = np.random.default_rng(3)
