@@ -221,6 +221,33 @@ __global__ __launch_bounds__(1024) void k_carry_blocks(JobRef jr, u64 nblocks) {
   }
 }
 
+// Run r receives the previous run's overflow (two words added at its limbs 0
+// and 1) and the carry bit E; the carry ripples until it stops.
+__device__ __forceinline__ void carry_into_run(const MulJob &J, u64 r, int rl, u64 e) {
+  u64 x0 = 0, x1 = 0;
+  if (r > 0) {
+    x0 = J.ovf[2 * (r - 1)];
+    x1 = J.ovf[2 * (r - 1) + 1];
+  }
+  const u64 base = r * (u64)rl;
+  if (x0 | x1 | e) {
+    u64 carry = e;
+    for (int j = 0; j < rl; j++) {
+      const u64 pos = base + j;
+      if (pos >= J.nout) break;
+      const u64 add = (j == 0) ? x0 : (j == 1 ? x1 : 0);
+      if (j >= 2 && carry == 0) break;
+      const u64 d = J.out[pos];
+      const u64 s = d + add;
+      u64 cy = (s < d) ? 1 : 0;
+      const u64 s2 = s + carry;
+      cy += (s2 < s) ? 1 : 0;
+      J.out[pos] = s2;
+      carry = cy;
+    }
+  }
+}
+
 // add ovf[r-1] and the carry bit into every run (grid as k_carry_map)
 __global__ __launch_bounds__(HGX_NT) void k_carry_apply(JobRef jr, int bpj_bits, u64 nruns, int rl) {
   __shared__ u32 sh[HGX_NT];
@@ -239,30 +266,22 @@ __global__ __launch_bounds__(HGX_NT) void k_carry_apply(JobRef jr, int bpj_bits,
   for (int i = 0; i < HGX_CARRY_RUNS_PER_THREAD; i++) {
     const u64 r = r0 + i;
     if (r >= nruns) break;
-    u64 x0 = 0, x1 = 0;
-    if (r > 0) {
-      x0 = J.ovf[2 * (r - 1)];
-      x1 = J.ovf[2 * (r - 1) + 1];
-    }
-    const u64 base = r * (u64)rl;
-    if (x0 | x1 | e) {
-      u64 carry = e;
-      for (int j = 0; j < rl; j++) {
-        const u64 pos = base + j;
-        if (pos >= J.nout) break;
-        const u64 add = (j == 0) ? x0 : (j == 1 ? x1 : 0);
-        if (j >= 2 && carry == 0) break;
-        const u64 d = J.out[pos];
-        const u64 s = d + add;
-        u64 cy = (s < d) ? 1 : 0;
-        const u64 s2 = s + carry;
-        cy += (s2 < s) ? 1 : 0;
-        J.out[pos] = s2;
-        carry = cy;
-      }
-    }
+    carry_into_run(J, r, rl, e);
     e = (J.maps[r] >> e) & 1u;
   }
+}
+
+// All three steps in one workgroup of 1024 threads per job when a job has at
+// most 1024 runs (L <= 2^20): thread r takes run r's map, one scan over the
+// workgroup gives every run its incoming carry, and the thread adds it.
+// grid = jobs
+__global__ __launch_bounds__(1024) void k_carry_small(JobRef jr, u64 nruns, int rl) {
+  __shared__ u32 sh[1024];
+  const u64 r = threadIdx.x;
+  const MulJob J = load_job(jr, blockIdx.x);
+  const u32 m = (r < nruns) ? run_map(J.out, J.nout, J.ovf, J.flags, r, rl) : 2u;
+  const u32 ex = block_exclusive_last(m, sh);
+  if (r < nruns) carry_into_run(J, r, rl, (ex == 3u) ? 1ull : 0ull);
 }
 
 // SplitMix64 stream: out[i] = mix(seed + (i+1) * 0x9E3779B97F4A7C15)
@@ -369,6 +388,10 @@ void hgx_launch_carry_blocks(dim3 grid, hipStream_t s, const JobRef &jr, u64 nbl
 
 void hgx_launch_carry_apply(dim3 grid, hipStream_t s, const JobRef &jr, int bpj_bits, u64 nruns, int rl) {
   hipLaunchKernelGGL(k_carry_apply, grid, dim3(HGX_NT), 0, s, jr, bpj_bits, nruns, rl);
+}
+
+void hgx_launch_carry_small(dim3 grid, hipStream_t s, const JobRef &jr, u64 nruns, int rl) {
+  hipLaunchKernelGGL(k_carry_small, grid, dim3(1024), 0, s, jr, nruns, rl);
 }
 
 void hgx_launch_splitmix64(dim3 grid, hipStream_t s, u64 *out, u64 n, u64 seed, int force_top) {

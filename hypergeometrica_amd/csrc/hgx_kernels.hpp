@@ -546,9 +546,9 @@ __device__ __forceinline__ void row_mul_tile(u64 *A, const u64 *Bp, u64 *ldsA, u
 //   MODE_MUL / MODE_SQR, !SINGLE: last forward pass of both operands (or the
 //     one operand), pointwise product (a*b/R by REDC), first inverse pass.
 //   MODE_FWD / MODE_INV: transform-only row pass on res0 (prime kbase + y).
-//   SINGLE (L <= 4096): the limbs come in, all three primes are done in
-//     turn, and the three residue arrays of the product go out (k_crt
-//     finishes).
+//   SINGLE (L <= 4096): the limbs come in, one workgroup per prime
+//     (grid = (jobs, 3)), and the three residue arrays of the product go out
+//     (k_crt finishes).
 
 template <int S_BITS, int MODE, bool SINGLE>
 __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRef jr, int jt_bits) {
@@ -579,30 +579,27 @@ __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRe
       for (int x = tid; x < TW; x += HGX_NT) A[x] = ldsA[HGX_PAD(x)];
     }
   } else {
-    // whole transform in one tile (L <= 4096): limbs in, all three primes in
-    // turn, residues of the product out (k_crt finishes)
-#pragma unroll 1
-    for (int k = 0; k < 3; k++) {
-      const Mod M = mod_of(ps, k);
-      for (int x = tid; x < TW; x += HGX_NT) {
-        const u64 pos = gbase + x;
-        ldsA[HGX_PAD(x)] = (pos < J.na) ? mod_reduce64(J.a[pos], M) : 0;
-        if (TWO) ldsB[HGX_PAD(x)] = (pos < J.nb) ? mod_reduce64(J.b[pos], M) : 0;
-      }
-      __syncthreads();
-      tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], M);
-      if (TWO) tile_forward<S_BITS, false>(ldsB, wbits, ps.tw_fwd[k], M);
-      for (int x = tid; x < TW; x += HGX_NT) {
-        const u64 a = ldsA[HGX_PAD(x)];
-        const u64 b = TWO ? ldsB[HGX_PAD(x)] : a;
-        ldsA[HGX_PAD(x)] = mont_mul(mont_mul(a, b, M), rp.scale[k], M);
-      }
-      __syncthreads();
-      tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], M);
-      u64 *dst = J.res0 + (u64)k * rp.L + gbase;
-      for (int x = tid; x < TW; x += HGX_NT) dst[x] = ldsA[HGX_PAD(x)];
-      __syncthreads();
+    // whole transform in one tile (L <= 4096): limbs in, one prime per
+    // workgroup (blockIdx.y), residues of the product out (k_crt finishes)
+    const int k = blockIdx.y;
+    const Mod M = mod_of(ps, k);
+    for (int x = tid; x < TW; x += HGX_NT) {
+      const u64 pos = gbase + x;
+      ldsA[HGX_PAD(x)] = (pos < J.na) ? mod_reduce64(J.a[pos], M) : 0;
+      if (TWO) ldsB[HGX_PAD(x)] = (pos < J.nb) ? mod_reduce64(J.b[pos], M) : 0;
     }
+    __syncthreads();
+    tile_forward<S_BITS, false>(ldsA, wbits, ps.tw_fwd[k], M);
+    if (TWO) tile_forward<S_BITS, false>(ldsB, wbits, ps.tw_fwd[k], M);
+    for (int x = tid; x < TW; x += HGX_NT) {
+      const u64 a = ldsA[HGX_PAD(x)];
+      const u64 b = TWO ? ldsB[HGX_PAD(x)] : a;
+      ldsA[HGX_PAD(x)] = mont_mul(mont_mul(a, b, M), rp.scale[k], M);
+    }
+    __syncthreads();
+    tile_inverse<S_BITS, false>(ldsA, wbits, ps.tw_inv[k], M);
+    u64 *dst = J.res0 + (u64)k * rp.L + gbase;
+    for (int x = tid; x < TW; x += HGX_NT) dst[x] = ldsA[HGX_PAD(x)];
   }
 }
 

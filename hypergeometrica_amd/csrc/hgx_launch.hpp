@@ -23,6 +23,8 @@ bool hgx_launch_row_inv_scaled(int sbits, dim3 grid, hipStream_t s, const PrimeS
 void hgx_launch_carry_map(dim3 grid, hipStream_t s, const JobRef &jr, int bpj_bits, u64 nruns, int rl);
 void hgx_launch_carry_blocks(dim3 grid, hipStream_t s, const JobRef &jr, u64 nblocks);
 void hgx_launch_carry_apply(dim3 grid, hipStream_t s, const JobRef &jr, int bpj_bits, u64 nruns, int rl);
+// the three carry steps in one launch when a job has at most 1024 runs
+void hgx_launch_carry_small(dim3 grid, hipStream_t s, const JobRef &jr, u64 nruns, int rl);
 void hgx_launch_splitmix64(dim3 grid, hipStream_t s, u64 *out, u64 n, u64 seed, int force_top);
 void hgx_launch_addsub(bool sub, hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out, u64 n,
                        uint8_t *agg, uint8_t *blkin, u64 *carry_out);

@@ -568,8 +568,8 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
       const u64 Rm = (u64)((((u128)1) << 64) % p);
       rp.scale[k] = to_mont(mulmod(Rm, invmod(L % p, p), p), p);
     }
-    rc = sqr ? launch_row<MODE_SQR, true>(n, dim3(njobs), s, rp, jr, 0)
-             : launch_row<MODE_MUL, true>(n, dim3(njobs), s, rp, jr, 0);
+    rc = sqr ? launch_row<MODE_SQR, true>(n, dim3(njobs, 3), s, rp, jr, 0)
+             : launch_row<MODE_MUL, true>(n, dim3(njobs, 3), s, rp, jr, 0);
     if (rc) return rc;
   } else {
     const int jt = n - 12;  // tiles per job
@@ -602,8 +602,11 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     HGX_TIMED("crt", s);
     hgx_launch_crt(dim3(njobs << cb), s, g_ctx.ps, jr, cb, L);
   }
-  // run-to-run carries between the CRT blocks of 2048 limbs
-  if (nruns > 1) {
+  // run-to-run carries between the CRT blocks of 1024 limbs
+  if (nruns > 1 && nruns <= 1024) {
+    HGX_TIMED("carry_small", s);
+    hgx_launch_carry_small(dim3(njobs), s, jr, nruns, (int)rl);
+  } else if (nruns > 1) {
     const u64 nblk = (nruns + HGX_CARRY_RUNS_PER_BLOCK - 1) / HGX_CARRY_RUNS_PER_BLOCK;
     const int bpj = log2_ceil(nblk);
     {
