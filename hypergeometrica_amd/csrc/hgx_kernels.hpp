@@ -145,6 +145,15 @@ __device__ __forceinline__ void round_regs(u64 *x, int goff, const u64 *__restri
       HGX_STAGE_FENCE();
     }
   } else {
+    // An output that the next stage of this round only multiplies by a
+    // nontrivial twiddle may stay in [0, 2p): Shoup's product takes any
+    // 64-bit multiplicand and returns a canonical residue.  Outputs that the
+    // next stage adds or subtracts, and every output of the round's last
+    // stage, are canonical.  The roles are fixed at compile time.
+    auto lazy_ok = [](int l, int q) {
+      const int nh = 1 << (R - l);  // the next stage's half
+      return l > 0 && (q & nh) != 0 && !(D == 1 && (q & (nh - 1)) == 0);
+    };
 #pragma unroll
     for (int l = R - 1; l >= 0; l--) {
       const int half = 1 << (R - 1 - l);
@@ -162,8 +171,8 @@ __device__ __forceinline__ void round_regs(u64 *x, int goff, const u64 *__restri
           const ulonglong2 w = *(const ulonglong2 *)(tw + 2 * (e * tstride));
           v = shoup_mul(x[j + half], w.x, w.y, M);
         }
-        x[j] = mod_add(u, v, M);
-        x[j + half] = mod_sub(u, v, M);
+        x[j] = lazy_ok(l, j) ? u + v : mod_add(u, v, M);
+        x[j + half] = lazy_ok(l, j + half) ? u + (M.p - v) : mod_sub(u, v, M);
       }
       HGX_STAGE_FENCE();
     }
