@@ -12,9 +12,12 @@ Also reported:
                 by the library around every kernel), against 8 TB/s;
   cpu_baseline  the oracle (C restatement of the reference's NTT multiply,
                 1 thread) timed on this host on a bounded sample;
-  latency       p50 / p99 of per-step wall time.
+  latency       p50 / p99 of per-step wall time;
+  legs          the 2^bits-limb square, 4096 batched products against the
+                per-call path, 2^16-limb per-call latency, the request server
+                under Poisson load, host-staged products and mpd-pi.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-<leg> ...]
 """
 from __future__ import annotations
 
@@ -159,6 +162,48 @@ def batched_leg(torch, h, _lib, count: int = 4096, reps: int = 3) -> dict:
             "limbs": tot_out, "batched_s": round(b, 5), "per_call_s": round(p, 5),
             "batched_limbs_per_s": tot_out / b, "per_call_limbs_per_s": tot_out / p,
             "speedup": round(p / b, 2), "outputs_identical": same}
+
+
+def square_leg(torch, h, _lib, bits: int = 26, steps: int = 10) -> dict:
+    """mpz-square (src/ntt-multiply.lisp:101-172) of one 2^BITS-limb operand
+    (seed 0x48475801, L = 2^(BITS+1)), operand and square in HBM: STEPS timed
+    calls after two warm-up calls, output limbs (2n) per second and the
+    per-kernel HIP-event durations of the square's launches.  The result is
+    checked against the general product of the operand with a copy of itself."""
+    L_ = _lib.lib()
+    n = 1 << bits
+    dev = torch.device("cuda:0")
+    A = torch.empty(n, dtype=torch.int64, device=dev)
+    A2 = torch.empty(n, dtype=torch.int64, device=dev)
+    O = torch.empty(2 * n, dtype=torch.int64, device=dev)
+    h.fill_splitmix64(A.data_ptr(), n, 0x48475801)
+    A2.copy_(A)
+
+    def sq():
+        _lib.check(L_.hgx_sqr_device(A.data_ptr(), n, O.data_ptr(), None))
+
+    for _ in range(2):
+        sq()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sq()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    _lib.timing_enable(True)
+    for _ in range(steps):
+        sq()
+    kt = _lib.timing_read()
+    _lib.timing_enable(False)
+    S = O.clone()
+    _lib.check(L_.hgx_mul_device(A.data_ptr(), n, A2.data_ptr(), n, O.data_ptr(), None))
+    torch.cuda.synchronize()
+    same = bool(torch.equal(S, O))
+    del A, A2, O, S
+    torch.cuda.empty_cache()
+    return {"workload": f"square of one 2^{bits}-limb operand, L=2^{bits + 1}, operand in HBM",
+            "ms": round(1e3 * dt, 3), "limbs_per_s": 2 * n / dt, "equals_general_product": same,
+            "kernels_ms": {k: round(t / c, 4) for k, (c, t) in sorted(kt.items())}}
 
 
 def latency_leg(torch, h, _lib, n: int = 1 << 16, reps: int = 200) -> dict:
@@ -378,6 +423,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-bits", type=int, default=22)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
+    ap.add_argument("--no-square", action="store_true", help="skip the 2^bits-limb square leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-server leg")
     ap.add_argument("--no-latency", action="store_true", help="skip the 2^16 x 2^16 per-call latency leg")
     ap.add_argument("--no-pi", action="store_true", help="skip the mpd-pi leg")
@@ -505,6 +551,11 @@ def main() -> int:
 
     del A, B, OUT
     torch.cuda.empty_cache()
+    square = None
+    if not args.no_square:
+        print("# square ...", file=sys.stderr, flush=True)
+        square = square_leg(torch, h, _lib, bits=args.bits)
+
     batched = None
     if not args.no_batch:
         print("# batched vs per-call ...", file=sys.stderr, flush=True)
@@ -563,6 +614,7 @@ def main() -> int:
         "multiply_roofline": {"alg_bytes": b_mul, "frac": round(b_mul / (ms_per_step * 1e-3) / HBM_PEAK, 4)},
         "latency_ms": {"p50": round(1e3 * p50, 3), "p99": round(1e3 * p99, 3)},
         "kernels": kernels,
+        "square": square,
         "batched": batched,
         "latency_2e16": latency,
         "serving": serving,
