@@ -250,6 +250,185 @@ int ensure(Workspace &w, size_t bytes) {
   return HGX_OK;
 }
 
+// ---- host transfers through pinned staging ----
+// A caller's limbs live in pageable memory (calloc or mmap, src/ram-vec.lisp:52-59,
+// src/disk-vec.lisp:34-35).  Large transfers go through kPinBufs pinned buffers
+// of kPinChunk bytes: worker threads copy chunk k + 1 between the caller's
+// memory and a staging buffer while the DMA engine moves chunk k.  Downloads
+// are deferred: a buffer's bytes reach the caller's memory when the buffer is
+// needed again or at pin_flush().  Every use holds g_ctx.mu.
+constexpr size_t kPinChunk = (size_t)32 << 20;
+constexpr int kPinBufs = 4;
+constexpr size_t kPinMin = (size_t)1 << 20;  // smaller transfers: one plain copy
+
+// memcpy split over a few worker threads; the workers are never joined (the
+// pool is leaked), so process exit does not wait for them
+struct CopyPool {
+  std::mutex mu;
+  std::condition_variable cv, done;
+  char *dst = nullptr;
+  const char *src = nullptr;
+  size_t bytes = 0;
+  int parts = 0, next = 0, left = 0;
+  int nthreads = 0;
+  void loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv.wait(lk, [this] { return next < parts; });
+      const int p = next++;
+      const size_t per = (bytes + parts - 1) / parts;
+      const size_t off = std::min(bytes, (size_t)p * per), len = std::min(per, bytes - off);
+      char *d = dst;
+      const char *s = src;
+      lk.unlock();
+      if (len) std::memcpy(d + off, s + off, len);
+      lk.lock();
+      if (--left == 0) done.notify_all();
+    }
+  }
+  void copy(void *d, const void *s, size_t n) {
+    if (nthreads == 0 || n < kPinMin) {
+      std::memcpy(d, s, n);
+      return;
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    dst = (char *)d;
+    src = (const char *)s;
+    bytes = n;
+    parts = nthreads;
+    next = 0;
+    left = nthreads;
+    cv.notify_all();
+    done.wait(lk, [this] { return left == 0; });
+    parts = 0;
+  }
+};
+
+CopyPool *copy_pool() {
+  static CopyPool *pool = [] {
+    CopyPool *p = new CopyPool;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int n = (int)std::min(8u, std::max(1u, hw / 2));
+    for (int i = 0; i < n; i++) std::thread([p] { p->loop(); }).detach();
+    p->nthreads = n;
+    return p;
+  }();
+  return pool;
+}
+
+struct PinStage {
+  char *buf[kPinBufs] = {};
+  hipEvent_t ev[kPinBufs] = {};
+  int state[kPinBufs] = {};  // 0 free, 1 upload in flight, 2 download to copy out
+  char *out_dst[kPinBufs] = {};
+  size_t out_len[kPinBufs] = {};
+  int next = 0;
+  bool ready = false;
+};
+PinStage g_pin;
+
+int pin_init() {
+  if (g_pin.ready) return HGX_OK;
+  for (int i = 0; i < kPinBufs; i++) {
+    HIPCHK(hipHostMalloc((void **)&g_pin.buf[i], kPinChunk, hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&g_pin.ev[i], hipEventDisableTiming));
+  }
+  (void)copy_pool();
+  g_pin.ready = true;
+  return HGX_OK;
+}
+
+// wait for buffer i's transfer; a finished download is copied to its caller
+int pin_settle(int i) {
+  if (g_pin.state[i] == 0) return HGX_OK;
+  HIPCHK(hipEventSynchronize(g_pin.ev[i]));
+  if (g_pin.state[i] == 2) copy_pool()->copy(g_pin.out_dst[i], g_pin.buf[i], g_pin.out_len[i]);
+  g_pin.state[i] = 0;
+  return HGX_OK;
+}
+
+// every deferred download has reached the caller's memory
+int pin_flush() {
+  if (!g_pin.ready) return HGX_OK;
+  for (int k = 0; k < kPinBufs; k++) {
+    const int i = (g_pin.next + k) % kPinBufs;  // oldest first
+    int rc = pin_settle(i);
+    if (rc) return rc;
+  }
+  return HGX_OK;
+}
+
+// host -> device on stream s; the host memory may be reused on return
+int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  if (bytes < kPinMin) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return HGX_OK;
+  }
+  int rc = pin_init();
+  if (rc) return rc;
+  for (size_t off = 0; off < bytes; off += kPinChunk) {
+    const int i = g_pin.next;
+    g_pin.next = (i + 1) % kPinBufs;
+    rc = pin_settle(i);
+    if (rc) return rc;
+    const size_t len = std::min(kPinChunk, bytes - off);
+    copy_pool()->copy(g_pin.buf[i], (const char *)src + off, len);
+    HIPCHK(hipMemcpyAsync((char *)dst + off, g_pin.buf[i], len, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(g_pin.ev[i], s));
+    g_pin.state[i] = 1;
+  }
+  return HGX_OK;
+}
+
+// device -> host on stream s, after the work queued before it; the bytes are
+// in the host memory after pin_flush() (or a later transfer reusing the
+// buffers)
+int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  if (bytes < kPinMin) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    return HGX_OK;
+  }
+  int rc = pin_init();
+  if (rc) return rc;
+  for (size_t off = 0; off < bytes; off += kPinChunk) {
+    const int i = g_pin.next;
+    g_pin.next = (i + 1) % kPinBufs;
+    rc = pin_settle(i);
+    if (rc) return rc;
+    const size_t len = std::min(kPinChunk, bytes - off);
+    HIPCHK(hipMemcpyAsync(g_pin.buf[i], (const char *)src + off, len, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(g_pin.ev[i], s));
+    g_pin.state[i] = 2;
+    g_pin.out_dst[i] = (char *)dst + off;
+    g_pin.out_len[i] = len;
+  }
+  return HGX_OK;
+}
+
+// An API call that queues downloads flushes them before it returns success;
+// on any other way out the scope drops what is still pending, so no later
+// call writes into memory its caller may have released.
+struct PinScope {
+  ~PinScope() {
+    if (!g_pin.ready) return;
+    for (int i = 0; i < kPinBufs; i++) {
+      if (g_pin.state[i] == 0) continue;
+      (void)hipEventSynchronize(g_pin.ev[i]);
+      g_pin.state[i] = 0;
+    }
+  }
+};
+
+void pin_free() {
+  if (!g_pin.ready) return;
+  for (int i = 0; i < kPinBufs; i++) {
+    (void)hipEventSynchronize(g_pin.ev[i]);
+    (void)hipEventDestroy(g_pin.ev[i]);
+    (void)hipHostFree(g_pin.buf[i]);
+  }
+  g_pin = PinStage{};
+}
+
 // ---- plans ----
 // A length-2^n transform is NCOL column passes of 8 bits and a row pass of
 // n - 8 NCOL bits; a column pass needs blocks of at least one 4096-word tile,
@@ -904,6 +1083,7 @@ int mul_staged_locked(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out
   } split_off;
   u64 in_a = ~0ull, in_b = ~0ull;  // chunks now in the device buffers
   int cur = 0;
+  PinScope pin;
   for (u64 d = 0; d + 1 < ka + kb; d++) {
     u64 *Wc = W[cur], *Wp = W[cur ^ 1];
     bool first = true;
@@ -914,11 +1094,13 @@ int mul_staged_locked(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out
       const u64 mi = std::min<u64>(M, na - i * M), mj = std::min<u64>(M, nb - j * M);
       const bool sq = sqr && i == j;
       if (in_a != i) {
-        HIPCHK(hipMemcpyAsync(ca, a + i * M, mi * 8, hipMemcpyHostToDevice, s));
+        rc = h2d(ca, a + i * M, mi * 8, s);
+        if (rc) return rc;
         in_a = i;
       }
       if (!sq && in_b != j) {
-        HIPCHK(hipMemcpyAsync(cbf, b + j * M, mj * 8, hipMemcpyHostToDevice, s));
+        rc = h2d(cbf, b + j * M, mj * 8, s);
+        if (rc) return rc;
         in_b = j;
       }
       rc = mul_device_locked(ca, mi, sq ? ca : cbf, sq ? mi : mj, P, sq, s);
@@ -937,9 +1119,12 @@ int mul_staged_locked(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out
     HIPCHK(hipGetLastError());
     const u64 at = d * M;
     const u64 cnt = (d + 2 == ka + kb) ? nout - at : M;
-    HIPCHK(hipMemcpyAsync(out + at, Wc, cnt * 8, hipMemcpyDeviceToHost, s));
+    rc = d2h(out + at, Wc, cnt * 8, s);
+    if (rc) return rc;
     cur ^= 1;
   }
+  rc = pin_flush();
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s));
   return ws_release(s);
 }
@@ -1535,6 +1720,7 @@ void hgx_shutdown(void) {
   if (g_ctx.jobs_host) (void)hipHostFree(g_ctx.jobs_host);
   g_ctx.jobs_host = nullptr;
   g_ctx.jobs_host_cap = 0;
+  pin_free();
   (void)hipEventDestroy(g_ctx.jobs_copied);
   dev_free(g_ctx.tables, 3 * 2 * 2048 * 2 * sizeof(u64));
   g_ctx.tables = nullptr;
@@ -1587,13 +1773,21 @@ static int mul_host_locked(const uint64_t *a, size_t na, const uint64_t *b, size
   if (rc) return rc;
   char *base = (char *)g_ctx.stage.ptr;
   u64 *d_a = (u64 *)base, *d_b = sqr ? d_a : (u64 *)(base + bytes_a), *d_o = (u64 *)(base + bytes_a + bytes_b);
-  HIPCHK(hipMemcpyAsync(d_a, a, na * 8, hipMemcpyHostToDevice, s));
-  if (!sqr) HIPCHK(hipMemcpyAsync(d_b, b, nb * 8, hipMemcpyHostToDevice, s));
-  rc = mul_device_locked(d_a, na, d_b, sqr ? na : nb, d_o, sqr, s);
+  PinScope pin;
+  rc = h2d(d_a, a, na * 8, s);
+  if (!rc && !sqr) rc = h2d(d_b, b, nb * 8, s);
+  if (!rc) rc = mul_device_locked(d_a, na, d_b, sqr ? na : nb, d_o, sqr, s);
+  if (!rc) rc = d2h(out, d_o, nout * 8, s);
+  if (!rc) rc = pin_flush();
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(out, d_o, nout * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return HGX_OK;
+}
+
+// limb ranges [x, x + nx) and [y, y + ny) share memory
+static bool overlaps(const uint64_t *x, size_t nx, const uint64_t *y, size_t ny) {
+  const uintptr_t x0 = (uintptr_t)x, x1 = x0 + 8 * nx, y0 = (uintptr_t)y, y1 = y0 + 8 * ny;
+  return x0 < y1 && y0 < x1;
 }
 
 int hgx_mul(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out) {
@@ -1601,6 +1795,8 @@ int hgx_mul(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t
   int rc = check_ready();
   if (rc) return rc;
   if (!a || !b || !out || na == 0 || nb == 0) return fail(HGX_EINVAL, "null operand or zero length");
+  if (overlaps(out, na + nb, a, na) || overlaps(out, na + nb, b, nb))
+    return fail(HGX_EINVAL, "the product storage overlaps an operand");
   return mul_host_locked(a, na, b, nb, out, a == b && na == nb);
 }
 
@@ -1609,6 +1805,7 @@ int hgx_sqr(const uint64_t *a, size_t na, uint64_t *out) {
   int rc = check_ready();
   if (rc) return rc;
   if (!a || !out || na == 0) return fail(HGX_EINVAL, "null operand or zero length");
+  if (overlaps(out, 2 * na, a, na)) return fail(HGX_EINVAL, "the square's storage overlaps the operand");
   return mul_host_locked(a, na, a, na, out, true);
 }
 
@@ -1668,11 +1865,15 @@ int hgx_mul_batch(const hgx_mul_desc *descs, size_t count) {
     dd[k].out = dout + ot;
     ot += d.na + d.nb;
   }
-  HIPCHK(hipMemcpyAsync(din, hin.data(), in_words * 8, hipMemcpyHostToDevice, s));
+  PinScope pin;
+  rc = h2d(din, hin.data(), in_words * 8, s);
+  if (rc) return rc;
   rc = batch_device_locked(dd.data(), dd.size(), s);
   if (rc) return rc;
   std::vector<u64> hout(out_words);
-  HIPCHK(hipMemcpyAsync(hout.data(), dout, out_words * 8, hipMemcpyDeviceToHost, s));
+  rc = d2h(hout.data(), dout, out_words * 8, s);
+  if (!rc) rc = pin_flush();
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s));
   ot = 0;
   for (size_t k = 0; k < rest.size(); k++) {
@@ -1704,10 +1905,12 @@ int hgx_ntt_forward(uint64_t *data, size_t n, int mod_num) {
   rc = ensure(g_ctx.stage, align_up(n * 8));
   if (rc) return rc;
   u64 *d = (u64 *)g_ctx.stage.ptr;
-  HIPCHK(hipMemcpyAsync(d, data, n * 8, hipMemcpyHostToDevice, s));
-  rc = enqueue_transform(d, n, mod_num, false, s);
+  PinScope pin;
+  rc = h2d(d, data, n * 8, s);
+  if (!rc) rc = enqueue_transform(d, n, mod_num, false, s);
+  if (!rc) rc = d2h(data, d, n * 8, s);
+  if (!rc) rc = pin_flush();
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(data, d, n * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return ws_release(s);
 }
@@ -1723,10 +1926,12 @@ int hgx_ntt_reverse(uint64_t *data, size_t n, int mod_num) {
   rc = ensure(g_ctx.stage, align_up(n * 8));
   if (rc) return rc;
   u64 *d = (u64 *)g_ctx.stage.ptr;
-  HIPCHK(hipMemcpyAsync(d, data, n * 8, hipMemcpyHostToDevice, s));
-  rc = enqueue_transform(d, n, mod_num, true, s);
+  PinScope pin;
+  rc = h2d(d, data, n * 8, s);
+  if (!rc) rc = enqueue_transform(d, n, mod_num, true, s);
+  if (!rc) rc = d2h(data, d, n * 8, s);
+  if (!rc) rc = pin_flush();
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(data, d, n * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   return ws_release(s);
 }

@@ -65,12 +65,15 @@ int hgx_max_transform_bits(void);
 // Replaces MPZ-*/NTT, src/ntt-multiply.lisp:174-254 (magnitude part; the
 // sign is sign(x)*sign(y), applied by the caller as at :254).  Writes exactly
 // NA + NB limbs to OUT (the reference's L-limb result trimmed by
-// optimize-mpz, src/multiply.lisp:19, holds the same value).
+// optimize-mpz, src/multiply.lisp:19, holds the same value).  OUT is fresh
+// storage, as the reference's result is (:184): it must not overlap A or B
+// (HGX_EINVAL otherwise).
 int hgx_mul(const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
             uint64_t *out);
 
 // Exact square.  Replaces MPZ-SQUARE, src/ntt-multiply.lisp:101-172.
-// Writes exactly 2 * NA limbs.
+// Writes exactly 2 * NA limbs to fresh storage OUT (:108), which must not
+// overlap A.
 int hgx_sqr(const uint64_t *a, size_t na, uint64_t *out);
 
 // Many independent products in one call, grouped on the device by

@@ -137,6 +137,34 @@ def test_headline_size_checksum(hgx, oracle):
     assert got[-1] != 0 or got[-2] != 0
 
 
+def test_host_transfers_through_staging(hgx, oracle):
+    # operands, product and transform above the 32 MiB staging chunk: uploads
+    # overlap their DMA with the next chunk's copy, downloads are copied out
+    # when a buffer is reused or at the end of the call
+    na, nb = (1 << 22) + 12345, (1 << 22) - 777
+    say(f"staged transfers {na}x{nb}")
+    a = oracle.splitmix64(0x5170, na)
+    b = oracle.splitmix64(0x5171, nb)
+    assert np.array_equal(hgx.mul_limbs(a, b), _gmp.mul(a, b))
+    assert np.array_equal(hgx.sqr_limbs(a), _gmp.mul(a, a.copy()))
+    x = residues(oracle, 1, 1 << 23, 0x5172)
+    assert np.array_equal(hgx.ntt_reverse(hgx.ntt_forward(x, 1), 1), x)
+
+
+def test_overlapping_product_storage_rejected(hgx):
+    from hypergeometrica_amd import _lib
+    L = _lib.lib()
+    buf = np.zeros(64, dtype=np.uint64)
+    buf[:8] = 3
+    p = buf.ctypes.data
+    assert L.hgx_mul(p, 4, p + 32, 4, p + 16) == _lib.HGX_EINVAL
+    assert L.hgx_mul(p, 4, p + 32, 4, p + 40) == _lib.HGX_EINVAL
+    assert L.hgx_sqr(p, 4, p + 8) == _lib.HGX_EINVAL
+    assert L.hgx_mul(p, 4, p + 32, 4, p + 64) == 0
+    # (3 (1 + X + X^2 + X^3))^2 with X = 2^64
+    assert buf[8:16].tolist() == [9, 18, 27, 36, 27, 18, 9, 0]
+
+
 # ---------------------------------------------------------------- batch / serving
 
 def test_batch_equals_per_call(hgx, oracle):
