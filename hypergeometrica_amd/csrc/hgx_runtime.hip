@@ -259,7 +259,7 @@ int ensure(Workspace &w, size_t bytes) {
 // needed again or at pin_flush().  Every use holds g_ctx.mu.
 constexpr size_t kPinChunk = (size_t)32 << 20;
 constexpr int kPinBufs = 4;
-constexpr size_t kPinMin = (size_t)1 << 20;  // smaller transfers: one plain copy
+constexpr size_t kPinMin = (size_t)4 << 20;  // smaller transfers: one plain copy
 
 // memcpy split over a few worker threads; the workers are never joined (the
 // pool is leaked), so process exit does not wait for them
@@ -358,9 +358,23 @@ int pin_flush() {
   return HGX_OK;
 }
 
-// host -> device on stream s; the host memory may be reused on return
+// host memory the HIP runtime already has pinned (hipHostMalloc): the DMA
+// engine reads and writes it directly
+bool host_pinned(const void *p) {
+  hipPointerAttribute_t attr{};
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory is not an error here
+    return false;
+  }
+  return attr.type == hipMemoryTypeHost;
+}
+
+// host -> device on stream s.  Staged and small pageable transfers have read
+// the host memory on return; a pinned source is read by the DMA engine when
+// the stream reaches the copy, so the caller keeps it until the stream
+// synchronizes (every caller here does before it returns).
 int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
-  if (bytes < kPinMin) {
+  if (bytes < kPinMin || host_pinned(src)) {
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
     return HGX_OK;
   }
@@ -384,7 +398,7 @@ int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
 // in the host memory after pin_flush() (or a later transfer reusing the
 // buffers)
 int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
-  if (bytes < kPinMin) {
+  if (bytes < kPinMin || host_pinned(dst)) {
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
     return HGX_OK;
   }
