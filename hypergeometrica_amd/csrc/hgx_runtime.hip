@@ -931,9 +931,22 @@ int ensure_jobs_host(size_t count) {
   return HGX_OK;
 }
 
-// The split product into D_OUT on stream S; the caller holds the workspace.
-int split_product_locked(const u64 *d_a, size_t na, const u64 *d_b, size_t nb, u64 *d_out, const Split &sp,
-                         hipStream_t s) {
+// Device memory of a split product: the two images, the job table and the
+// chunks' work memory (MAIN), and the carry-scan scratch of the image sum.
+struct SplitBytes {
+  size_t main, scan;
+};
+SplitBytes split_bytes(size_t na, size_t nb, const Split &sp) {
+  const u64 n = na + nb;
+  return {2 * align_up(n * sizeof(u64)) + align_up(sp.k * sizeof(MulJob)) + sp.k * work_layout(sp.nc, false).total,
+          align_up(2 * hgx_addsub_blocks(n))};
+}
+
+// The split product into D_OUT on stream S, in the device memory at BASE
+// (split_bytes().main bytes) with scan scratch AGG.  JOBS (sp.k entries of
+// host memory) must stay unchanged until the stream has copied it.
+int split_product_at(const u64 *d_a, size_t na, const u64 *d_b, size_t nb, u64 *d_out, const Split &sp,
+                     hipStream_t s, char *base, uint8_t *agg, MulJob *jobs) {
   if (na < nb) {
     std::swap(d_a, d_b);
     std::swap(na, nb);
@@ -942,35 +955,43 @@ int split_product_locked(const u64 *d_a, size_t na, const u64 *d_b, size_t nb, u
   const WorkLayout wl = work_layout(sp.nc, false);
   const size_t img = align_up(n * sizeof(u64));
   const size_t tbl = align_up(sp.k * sizeof(MulJob));
-  const size_t bytes = 2 * img + tbl + sp.k * wl.total;
-  if (g_ctx.ws.bytes < bytes) HIPCHK(hipStreamSynchronize(s));  // this call's earlier work may use it
-  int rc = ensure(g_ctx.ws, bytes);
-  if (rc) return rc;
   const u64 nblk = hgx_addsub_blocks(n);
-  if (g_ctx.scan.bytes < 2 * nblk) HIPCHK(hipStreamSynchronize(s));
-  rc = ensure(g_ctx.scan, 2 * nblk);
-  if (rc) return rc;
-  char *base = (char *)g_ctx.ws.ptr;
   u64 *E = (u64 *)base, *O = (u64 *)(base + img);
   MulJob *table = (MulJob *)(base + 2 * img);
   char *work = base + 2 * img + tbl;
   hgx_launch_fill(s, E, 2 * img / sizeof(u64), 0);
-  rc = ensure_jobs_host(sp.k);
-  if (rc) return rc;
   for (u64 i = 0; i < sp.k; i++) {
     const u64 off = i * sp.m;
     const u64 mi = std::min<u64>(sp.m, na - off);
-    g_ctx.jobs_host[i] = make_job(d_a + off, mi, d_b, nb, ((i & 1) ? O : E) + off, sp.nc, false, work + i * wl.total);
+    jobs[i] = make_job(d_a + off, mi, d_b, nb, ((i & 1) ? O : E) + off, sp.nc, false, work + i * wl.total);
   }
-  HIPCHK(hipMemcpyAsync(table, g_ctx.jobs_host, sp.k * sizeof(MulJob), hipMemcpyHostToDevice, s));
-  HIPCHK(hipEventRecord(g_ctx.jobs_copied, s));
+  HIPCHK(hipMemcpyAsync(table, jobs, sp.k * sizeof(MulJob), hipMemcpyHostToDevice, s));
   JobRef jr{};
   jr.table = table;
-  rc = enqueue_group(jr, (unsigned)sp.k, sp.nc, false, s);
+  int rc = enqueue_group(jr, (unsigned)sp.k, sp.nc, false, s);
   if (rc) return rc;
-  uint8_t *agg = (uint8_t *)g_ctx.scan.ptr;
   hgx_launch_addsub(false, s, E, n, O, n, d_out, n, agg, agg + nblk, nullptr);
   HIPCHK(hipGetLastError());
+  return HGX_OK;
+}
+
+// The split product into D_OUT on stream S in the library workspace; the
+// caller holds the workspace.
+int split_product_locked(const u64 *d_a, size_t na, const u64 *d_b, size_t nb, u64 *d_out, const Split &sp,
+                         hipStream_t s) {
+  const SplitBytes sb = split_bytes(na, nb, sp);
+  if (g_ctx.ws.bytes < sb.main) HIPCHK(hipStreamSynchronize(s));  // this call's earlier work may use it
+  int rc = ensure(g_ctx.ws, sb.main);
+  if (rc) return rc;
+  if (g_ctx.scan.bytes < sb.scan) HIPCHK(hipStreamSynchronize(s));
+  rc = ensure(g_ctx.scan, sb.scan);
+  if (rc) return rc;
+  rc = ensure_jobs_host(sp.k);
+  if (rc) return rc;
+  rc = split_product_at(d_a, na, d_b, nb, d_out, sp, s, (char *)g_ctx.ws.ptr, (uint8_t *)g_ctx.scan.ptr,
+                        g_ctx.jobs_host);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(g_ctx.jobs_copied, s));
   return HGX_OK;
 }
 
@@ -1286,15 +1307,18 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
 //
 // hgx_submit queues a request and returns.  A server thread admits queued
 // requests in arrival order while their memory (operands, product, work
-// arrays) fits in a device arena carved from the HBM budget, groups each
-// admission round by transform length into batched launches, and spreads
-// rounds over several streams so rounds overlap on the device.  A request
+// arrays) fits in a device arena carved from the HBM budget (a round takes at
+// most a quarter of the arena), groups each admission round by transform
+// length into batched launches, and spreads rounds over several streams so
+// rounds overlap on the device.  A request
 // that arrives while others run joins the next round as soon as memory
 // frees up.  Completion is observed from events; the arena block is then
 // returned and the ticket completes.
 struct SrvReq {
   hgx_mul_desc d{};
   bool sqr = false;
+  bool split = false;  // unbalanced: chunk products in the request's own arena block
+  Split sp{};
   int n = 0;
   size_t bytes = 0, off = 0;
   size_t table_off = (size_t)-1, table_bytes = 0;  // job table of its group (last request of a group owns it)
@@ -1356,6 +1380,10 @@ void arena_free(size_t off, size_t bytes) {
 
 size_t request_bytes(const SrvReq &r) {
   const size_t in = align_up(r.d.na * 8) + (r.sqr ? 0 : align_up(r.d.nb * 8));
+  if (r.split) {
+    const SplitBytes sb = split_bytes(r.d.na, r.d.nb, r.sp);
+    return in + align_up((r.d.na + r.d.nb) * 8) + sb.main + sb.scan;
+  }
   return in + align_up((r.d.na + r.d.nb) * 8) + work_layout(r.n, r.sqr).total;
 }
 
@@ -1374,22 +1402,10 @@ hipEvent_t take_event() {
 void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   // unbalanced products (split into chunk products) after the grouped ones
-  std::vector<char> split(batch.size(), 0);
-  std::vector<Split> sps(batch.size());
-  for (size_t i = 0; i < batch.size(); i++)
-    split[i] = !batch[i]->sqr && g_ctx.split_on && choose_split(batch[i]->d.na, batch[i]->d.nb, sps[i]);
   std::vector<std::shared_ptr<SrvReq>> splits;
-  std::vector<Split> split_plans;
   {
     std::vector<std::shared_ptr<SrvReq>> grouped;
-    for (size_t i = 0; i < batch.size(); i++) {
-      if (split[i]) {
-        splits.push_back(batch[i]);
-        split_plans.push_back(sps[i]);
-      } else {
-        grouped.push_back(batch[i]);
-      }
-    }
+    for (auto &r : batch) (r->split ? splits : grouped).push_back(r);
     batch = grouped;
   }
   std::stable_sort(batch.begin(), batch.end(), [](const std::shared_ptr<SrvReq> &x, const std::shared_ptr<SrvReq> &y) {
@@ -1442,18 +1458,20 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
     g = h;
   }
   // the split products: operands in, chunk products and the image sum in the
-  // library workspace (ordered across streams by its event)
+  // request's own arena block, so split products on different streams overlap
   for (size_t i = 0; i < splits.size(); i++) {
     SrvReq &r = *splits[i];
     char *base = g_srv.arena + r.off;
     u64 *da = (u64 *)base;
     u64 *db = (u64 *)(base + align_up(r.d.na * 8));
     u64 *dout = (u64 *)(base + align_up(r.d.na * 8) + align_up(r.d.nb * 8));
+    char *work = (char *)dout + align_up((r.d.na + r.d.nb) * 8);
+    const SplitBytes sb = split_bytes(r.d.na, r.d.nb, r.sp);
     (void)hipMemcpyAsync(da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
-    int rc = ws_acquire(s);
-    if (!rc) rc = split_product_locked(da, r.d.na, db, r.d.nb, dout, split_plans[i], s);
-    if (!rc) rc = ws_release(s);
+    r.table_host.resize(r.sp.k);  // alive until the request completes
+    int rc = split_product_at(da, r.d.na, db, r.d.nb, dout, r.sp, s, work, (uint8_t *)(work + sb.main),
+                              r.table_host.data());
     if (rc) r.err = g_err;
     batch.push_back(splits[i]);
   }
@@ -1486,10 +1504,14 @@ void server_loop() {
         i++;
       }
     }
-    // admit in arrival order while memory lasts
+    // admit in arrival order while memory lasts, at most a quarter of the
+    // arena per round: under load several rounds are in flight on the
+    // streams, so one round's uploads and downloads overlap another's kernels
     std::vector<std::shared_ptr<SrvReq>> round;
+    size_t round_bytes = 0;
     while (!g_srv.queue.empty()) {
       auto r = g_srv.queue.front();
+      if (!round.empty() && round_bytes + r->bytes > g_srv.arena_bytes / kServeStreams) break;
       const size_t limit = g_ctx.stage_limit.load();
       if (r->bytes > g_srv.arena_bytes || (limit && r->bytes > limit)) {
         // larger than the arena: a host-staged product, here in the server
@@ -1516,6 +1538,7 @@ void server_loop() {
       r->state = 1;
       r->done_ev = take_event();
       round.push_back(r);
+      round_bytes += r->bytes;
       g_srv.queue.pop_front();
     }
     if (!round.empty()) {
@@ -2008,6 +2031,7 @@ int hgx_submit(const hgx_mul_desc *desc, hgx_ticket *ticket) {
   r->sqr = (d.a == d.b && d.na == d.nb);
   r->n = log2_ceil((u64)hgx_product_length(d.na, d.nb));
   if (r->n > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32");
+  r->split = !r->sqr && g_ctx.split_on && choose_split(d.na, d.nb, r->sp);
   r->bytes = request_bytes(*r);
   static std::atomic<hgx_ticket> next_ticket{1};
   const hgx_ticket t = next_ticket++;
