@@ -187,15 +187,25 @@ struct TimedLaunch {
 bool g_timing = false;
 std::vector<TimedLaunch> g_pending;
 std::map<std::string, std::pair<uint64_t, double>> g_kstats;
+std::vector<hipEvent_t> g_timing_events;  // drained events, reused (no creation on the launch path)
+
+hipEvent_t timing_event() {
+  if (!g_timing_events.empty()) {
+    hipEvent_t e = g_timing_events.back();
+    g_timing_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
 
 struct KTimer {
   hipStream_t s;
   bool on;
   KTimer(const char *name, hipStream_t st) : s(st), on(g_timing) {
     if (!on) return;
-    TimedLaunch t{name, nullptr, nullptr};
-    (void)hipEventCreate(&t.a);
-    (void)hipEventCreate(&t.b);
+    TimedLaunch t{name, timing_event(), timing_event()};
     (void)hipEventRecord(t.a, s);
     g_pending.push_back(t);
   }
@@ -212,8 +222,8 @@ void timing_drain() {
     auto &st = g_kstats[t.name];
     st.first += 1;
     st.second += ms;
-    (void)hipEventDestroy(t.a);
-    (void)hipEventDestroy(t.b);
+    g_timing_events.push_back(t.a);
+    g_timing_events.push_back(t.b);
   }
   g_pending.clear();
 }
@@ -2180,6 +2190,13 @@ int hgx_timing_enable(int on) {
   timing_drain();
   g_kstats.clear();
   g_timing = on != 0;
+  // events for the launches to come, created here rather than between them
+  if (g_timing)
+    while (g_timing_events.size() < 256) {
+      hipEvent_t e = nullptr;
+      HIPCHK(hipEventCreate(&e));
+      g_timing_events.push_back(e);
+    }
   return HGX_OK;
 }
 
