@@ -1750,6 +1750,11 @@ void hgx_shutdown(void) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   if (!g_ctx.ready) return;
   hipStreamSynchronize(g_ctx.stream);
+  timing_drain();
+  g_timing = false;
+  g_kstats.clear();
+  for (hipEvent_t e : g_timing_events) (void)hipEventDestroy(e);
+  g_timing_events.clear();
   for (auto &kv : g_ctx.plans) free_plan(*kv.second);
   g_ctx.plans.clear();
   for (auto &kv : g_ctx.graphs) (void)hipGraphExecDestroy(kv.second);
