@@ -151,6 +151,38 @@ def test_host_transfers_through_staging(hgx, oracle):
     assert np.array_equal(hgx.ntt_reverse(hgx.ntt_forward(x, 1), 1), x)
 
 
+def test_concurrent_host_calls_from_threads(hgx, oracle):
+    # the reference's :hypergeometrica-parallel build calls mpz-* from
+    # lparallel workers: eight threads call hgx_mul / hgx_sqr at once, with
+    # sizes on both sides of the 4 MiB staging threshold
+    import threading
+    shapes = [(3000, 2000), ((1 << 19) + 5, 1 << 19), (100000, 31072), (1 << 16, 1 << 16)]
+    say("concurrent host calls, 8 threads")
+    results, errors = {}, []
+
+    def worker(w):
+        try:
+            for r, (na, nb) in enumerate(shapes):
+                a = oracle.splitmix64(0x7000 + 16 * w + r, na)
+                b = oracle.splitmix64(0x7800 + 16 * w + r, nb)
+                if (w + r) % 3 == 0:
+                    results[(w, r)] = (hgx.sqr_limbs(a), _gmp.mul(a, a.copy()))
+                else:
+                    results[(w, r)] = (hgx.mul_limbs(a, b), _gmp.mul(a, b))
+        except Exception as e:  # reported below with the worker's index
+            errors.append((w, repr(e)))
+
+    threads = [threading.Thread(target=worker, args=(w,)) for w in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    assert len(results) == 8 * len(shapes)
+    for key, (got, want) in results.items():
+        assert np.array_equal(got, want), key
+
+
 def test_overlapping_product_storage_rejected(hgx):
     from hypergeometrica_amd import _lib
     L = _lib.lib()
