@@ -71,9 +71,11 @@ __device__ __forceinline__ u64 apply_twiddle_tt(u64 x, const ColTw &tw, int k, u
 // groups of 2^R elements spaced D = S >> (A+R) rows apart, one group per
 // thread iteration, values in registers.
 // ---------------------------------------------------------------------------
-template <int S_BITS>
+// RMAX: the largest radix exponent of a round (4: radix-16 rounds; 3:
+// radix-8 rounds, for tiles of 2048 words on 256 threads).
+template <int S_BITS, int RMAX = 4>
 struct RoundPlan {
-  static constexpr int NR = (S_BITS + 3) / 4;
+  static constexpr int NR = (S_BITS + RMAX - 1) / RMAX;
   static constexpr int bits(int i) { return S_BITS / NR + (i < S_BITS % NR ? 1 : 0); }
   static constexpr int start(int i) {
     int a = 0;
@@ -389,10 +391,10 @@ __global__ __launch_bounds__(HGX_NT) void k_fwd_col(PrimeSet ps, ColTw tw, JobRe
 
 // Round I of a row transform (rows of 2^S_BITS words, row t of the tile at
 // index t << S_BITS): group gid is row t, elements base + j*D.
-template <int S_BITS, int I>
+template <int S_BITS, int I, int RMAX = 4>
 struct RowRound {
-  static constexpr int A = RoundPlan<S_BITS>::start(I);
-  static constexpr int R = RoundPlan<S_BITS>::bits(I);
+  static constexpr int A = RoundPlan<S_BITS, RMAX>::start(I);
+  static constexpr int R = RoundPlan<S_BITS, RMAX>::bits(I);
   static constexpr int RR = 1 << R;
   static constexpr int D = (1 << S_BITS) >> (A + R);
   static constexpr int LD = Log2<D>::value;
@@ -408,10 +410,10 @@ struct RowRound {
 // One forward round on the same groups of two row tiles: both tiles' values
 // are read from LDS before either is transformed, so the second read's
 // latency hides behind the first tile's butterflies.
-template <int S_BITS, int I>
+template <int S_BITS, int I, int RMAX>
 __device__ __forceinline__ void row_round_two(u64 *ldsA, u64 *ldsB, int wbits, const u64 *__restrict__ tw,
                                               const Mod &M) {
-  using RI = RowRound<S_BITS, I>;
+  using RI = RowRound<S_BITS, I, RMAX>;
 #pragma unroll 1
   for (int gid = threadIdx.x; gid < (1 << (wbits + RI::GPC_BITS)); gid += HGX_NT) {
     int row, goff, base;
@@ -431,27 +433,29 @@ __device__ __forceinline__ void row_round_two(u64 *ldsA, u64 *ldsB, int wbits, c
 }
 
 // Forward rounds I .. I1 in LDS on one or two tiles, a barrier after each.
-template <int S_BITS, int I, int I1, bool TWO>
+template <int S_BITS, int I, int I1, bool TWO, int RMAX = 4>
 struct FwdRange {
   static __device__ __forceinline__ void run(u64 *ldsA, u64 *ldsB, int wbits, const u64 *tw, const Mod &M) {
+    using RP = RoundPlan<S_BITS, RMAX>;
     if constexpr (I <= I1) {
       if constexpr (TWO)
-        row_round_two<S_BITS, I>(ldsA, ldsB, wbits, tw, M);
+        row_round_two<S_BITS, I, RMAX>(ldsA, ldsB, wbits, tw, M);
       else
-        tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), false, false>(ldsA, wbits, tw, M);
+        tile_round<S_BITS, RP::start(I), RP::bits(I), false, false>(ldsA, wbits, tw, M);
       __syncthreads();
-      FwdRange<S_BITS, I + 1, I1, TWO>::run(ldsA, ldsB, wbits, tw, M);
+      FwdRange<S_BITS, I + 1, I1, TWO, RMAX>::run(ldsA, ldsB, wbits, tw, M);
     }
   }
 };
 // Inverse rounds I down to I0 in LDS, a barrier after each.
-template <int S_BITS, int I, int I0>
+template <int S_BITS, int I, int I0, int RMAX = 4>
 struct InvRange {
   static __device__ __forceinline__ void run(u64 *lds, int wbits, const u64 *tw, const Mod &M) {
+    using RP = RoundPlan<S_BITS, RMAX>;
     if constexpr (I >= I0) {
-      tile_round<S_BITS, RoundPlan<S_BITS>::start(I), RoundPlan<S_BITS>::bits(I), true, false>(lds, wbits, tw, M);
+      tile_round<S_BITS, RP::start(I), RP::bits(I), true, false>(lds, wbits, tw, M);
       __syncthreads();
-      InvRange<S_BITS, I - 1, I0>::run(lds, wbits, tw, M);
+      InvRange<S_BITS, I - 1, I0, RMAX>::run(lds, wbits, tw, M);
     }
   }
 };
@@ -461,12 +465,12 @@ struct InvRange {
 // rows.  Round 0 reads HBM and the last inverse round writes it; the last
 // forward round, the product and the first inverse round run per group in
 // registers, so the operand tiles meet in LDS only between rounds.
-template <int S_BITS, bool TWO>
+template <int S_BITS, bool TWO, int RMAX = 4>
 __device__ __forceinline__ void row_mul_tile(u64 *A, const u64 *Bp, u64 *ldsA, u64 *ldsB, int wbits, const u64 *twf,
                                              const u64 *twi, const Mod &M) {
-  constexpr int NR = RoundPlan<S_BITS>::NR;
-  using R0 = RowRound<S_BITS, 0>;
-  using RL = RowRound<S_BITS, NR - 1>;
+  constexpr int NR = RoundPlan<S_BITS, RMAX>::NR;
+  using R0 = RowRound<S_BITS, 0, RMAX>;
+  using RL = RowRound<S_BITS, NR - 1, RMAX>;
   const int tid = threadIdx.x;
   if constexpr (NR == 1) {
 #pragma unroll 1
@@ -512,7 +516,7 @@ __device__ __forceinline__ void row_mul_tile(u64 *A, const u64 *Bp, u64 *ldsA, u
       }
     }
     __syncthreads();
-    if constexpr (NR > 2) FwdRange<S_BITS, 1, NR - 2, TWO>::run(ldsA, ldsB, wbits, twf, M);
+    if constexpr (NR > 2) FwdRange<S_BITS, 1, NR - 2, TWO, RMAX>::run(ldsA, ldsB, wbits, twf, M);
     // last forward round, product, first inverse round
 #pragma unroll 1
     for (int gid = tid; gid < (1 << (wbits + RL::GPC_BITS)); gid += HGX_NT) {
@@ -534,7 +538,7 @@ __device__ __forceinline__ void row_mul_tile(u64 *A, const u64 *Bp, u64 *ldsA, u
       for (int j = 0; j < RL::RR; j++) ldsA[HGX_PAD(row + base + j * RL::D)] = a[j];
     }
     __syncthreads();
-    if constexpr (NR > 2) InvRange<S_BITS, NR - 2, 1>::run(ldsA, wbits, twi, M);
+    if constexpr (NR > 2) InvRange<S_BITS, NR - 2, 1, RMAX>::run(ldsA, wbits, twi, M);
     // inverse round 0 to HBM
 #pragma unroll 1
     for (int gid = tid; gid < (1 << (wbits + R0::GPC_BITS)); gid += HGX_NT) {
@@ -610,6 +614,28 @@ __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRe
     u64 *dst = J.res0 + (u64)k * rp.L + gbase;
     for (int x = tid; x < TW; x += HGX_NT) dst[x] = ldsA[HGX_PAD(x)];
   }
+}
+
+// Row pass of a product (TWO) or square on tiles of 2048 words (rows of
+// 2^S_BITS <= 2^11 words) with radix-8 rounds: twice the workgroups of
+// k_row's 4096-word tiles for the same rows, which lowers the latency of
+// products too small to fill the GPU (the runtime uses it when k_row would
+// launch fewer than two workgroups per CU).  Same residues as k_row.
+template <int S_BITS, bool TWO>
+__global__ __launch_bounds__(HGX_NT) void k_row_half(PrimeSet ps, RowParams rp, JobRef jr, int jt_bits) {
+  static_assert(S_BITS <= HGX_HALF_TILE_BITS, "whole rows per tile");
+  constexpr int TB = HGX_HALF_TILE_BITS;
+  constexpr int NLDS = (1 << TB) + (1 << TB) / 16;
+  __shared__ u64 ldsA[NLDS];
+  __shared__ u64 ldsB[TWO ? NLDS : 1];
+  const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
+  const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
+  const u64 gbase = tile << TB;
+  const int k = rp.kbase + blockIdx.y;
+  const Mod M = mod_of(ps, k);
+  u64 *A = J.res0 + (u64)blockIdx.y * rp.L + gbase;
+  const u64 *Bp = TWO ? (J.res1 + (u64)blockIdx.y * rp.L + gbase) : nullptr;
+  row_mul_tile<S_BITS, TWO, 3>(A, Bp, ldsA, ldsB, TB - S_BITS, ps.tw_fwd[k], ps.tw_inv[k], M);
 }
 
 // Transform-only inverse when L <= 4096 (ntt-reverse on one tile, with the

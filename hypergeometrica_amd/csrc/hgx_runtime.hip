@@ -31,6 +31,9 @@ namespace {
 
 constexpr int kMaxBits = 32;   // largest transform: 2^32 words per residue array
 constexpr int kRowMaxBits = 12;
+// a multiply row pass with fewer 4096-word workgroups than this (two per CU)
+// runs on 2048-word tiles instead
+constexpr u64 kHalfTileBelow = 512;
 
 thread_local std::string g_err;
 
@@ -791,9 +794,19 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     rp.L = L;
     rp.tw_bits = 12;
     rp.kbase = 0;
-    rc = sqr ? launch_row<MODE_SQR, false>(plan->srow, dim3(grid, 3), s, rp, jr, jt)
-             : launch_row<MODE_MUL, false>(plan->srow, dim3(grid, 3), s, rp, jr, jt);
-    if (rc) return rc;
+    if (plan->srow <= HGX_HALF_TILE_BITS && 3ull * grid < kHalfTileBelow) {
+      // too few 4096-word tiles to fill the GPU: 2048-word tiles, twice the
+      // workgroups, for a lower latency
+      HGX_TIMED(sqr ? "row_sqr_half" : "row_mul_half", s);
+      const int jh = n - HGX_HALF_TILE_BITS;
+      rp.tw_bits = HGX_HALF_TILE_BITS;
+      if (!hgx_launch_row_half(!sqr, plan->srow, dim3(njobs << jh, 3), s, g_ctx.ps, rp, jr, jh))
+        return fail(HGX_EINVAL, "row pass size out of range");
+    } else {
+      rc = sqr ? launch_row<MODE_SQR, false>(plan->srow, dim3(grid, 3), s, rp, jr, jt)
+               : launch_row<MODE_MUL, false>(plan->srow, dim3(grid, 3), s, rp, jr, jt);
+      if (rc) return rc;
+    }
     for (int i = plan->ncol - 1; i >= 0; i--) {
       HGX_TIMED("inv_col", s);
       hgx_launch_inv_col(dim3(grid, 3), s, g_ctx.ps, coltw(plan->col[i], i == 0 ? 2 : 1), jr, jt, L, plan->col[i].bbits,

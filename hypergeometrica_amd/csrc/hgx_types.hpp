@@ -9,6 +9,7 @@
 #define HGX_EPT 16  // elements of a full tile per thread
 #define HGX_PAD(x) ((x) + ((x) >> 4))
 #define HGX_TILE_LDS (HGX_TILE + HGX_TILE / 16)
+#define HGX_HALF_TILE_BITS 11  // tiles of the low-latency multiply row pass (rows of at most 2^11 words)
 #define HGX_RUN 4  // limbs per carry run: one run per thread of a CRT workgroup
 #define HGX_CRT_BLOCK_BITS 10
 #define HGX_CRT_BLOCK (1 << HGX_CRT_BLOCK_BITS)  // limbs per CRT workgroup (the unit of the global carry scan)
