@@ -1345,13 +1345,21 @@ struct SrvReq {
   int n = 0;
   size_t bytes = 0, off = 0;
   size_t table_off = (size_t)-1, table_bytes = 0;  // job table of its group (last request of a group owns it)
-  std::vector<MulJob> table_host;
+  // host copy of that table (or of a split product's chunk table), alive
+  // until the request completes: pinned, from the server's table pool, so
+  // its upload is asynchronous (a copy from pageable memory would hold the
+  // server thread until the stream reached it); table_spill when the pool
+  // is exhausted
+  MulJob *table_host = nullptr;
+  size_t host_off = (size_t)-1, host_bytes = 0;
+  std::vector<MulJob> table_spill;
   hipEvent_t done_ev = nullptr;
   int state = 0;  // 0 queued, 1 in flight, 2 done, <0 failed
   std::string err;
 };
 
 constexpr int kServeStreams = 4;
+constexpr size_t kServeTableBytes = 4 << 20;  // pinned pool of the server's host job tables
 
 struct Server {
   bool started = false, stop = false;
@@ -1364,6 +1372,8 @@ struct Server {
   char *arena = nullptr;
   size_t arena_bytes = 0;
   std::map<size_t, size_t> free_list;  // offset -> size
+  char *tables = nullptr;               // pinned host pool of job tables
+  std::map<size_t, size_t> table_free;  // offset -> size in it
   hipStream_t streams[kServeStreams]{};
   int next_stream = 0;
   std::vector<hipEvent_t> event_pool;
@@ -1371,34 +1381,54 @@ struct Server {
 
 Server g_srv;
 
-size_t arena_alloc(size_t bytes) {
+// First-fit ranges of a pool (offset -> size of each free range).
+size_t range_alloc(std::map<size_t, size_t> &fl, size_t bytes) {
   bytes = align_up(bytes);
-  for (auto it = g_srv.free_list.begin(); it != g_srv.free_list.end(); ++it) {
+  for (auto it = fl.begin(); it != fl.end(); ++it) {
     if (it->second >= bytes) {
       const size_t off = it->first, sz = it->second;
-      g_srv.free_list.erase(it);
-      if (sz > bytes) g_srv.free_list[off + bytes] = sz - bytes;
+      fl.erase(it);
+      if (sz > bytes) fl[off + bytes] = sz - bytes;
       return off;
     }
   }
   return (size_t)-1;
 }
 
-void arena_free(size_t off, size_t bytes) {
+void range_free(std::map<size_t, size_t> &fl, size_t off, size_t bytes) {
   bytes = align_up(bytes);
-  auto it = g_srv.free_list.emplace(off, bytes).first;
+  auto it = fl.emplace(off, bytes).first;
   auto nx = std::next(it);
-  if (nx != g_srv.free_list.end() && it->first + it->second == nx->first) {
+  if (nx != fl.end() && it->first + it->second == nx->first) {
     it->second += nx->second;
-    g_srv.free_list.erase(nx);
+    fl.erase(nx);
   }
-  if (it != g_srv.free_list.begin()) {
+  if (it != fl.begin()) {
     auto pv = std::prev(it);
     if (pv->first + pv->second == it->first) {
       pv->second += it->second;
-      g_srv.free_list.erase(it);
+      fl.erase(it);
     }
   }
+}
+
+size_t arena_alloc(size_t bytes) { return range_alloc(g_srv.free_list, bytes); }
+void arena_free(size_t off, size_t bytes) { range_free(g_srv.free_list, off, bytes); }
+
+// Host storage of COUNT jobs for request R, kept until it completes (caller
+// holds g_srv.mu).
+MulJob *host_table(SrvReq &r, size_t count) {
+  const size_t bytes = count * sizeof(MulJob);
+  const size_t off = g_srv.tables ? range_alloc(g_srv.table_free, bytes) : (size_t)-1;
+  if (off != (size_t)-1) {
+    r.host_off = off;
+    r.host_bytes = bytes;
+    r.table_host = (MulJob *)(g_srv.tables + off);
+  } else {
+    r.table_spill.resize(count);
+    r.table_host = r.table_spill.data();
+  }
+  return r.table_host;
 }
 
 size_t request_bytes(const SrvReq &r) {
@@ -1454,17 +1484,18 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
     while (h < batch.size() && batch[h]->n == batch[g]->n && batch[h]->sqr == batch[g]->sqr) h++;
     int rc = HGX_OK;
     size_t toff = (size_t)-1;
+    SrvReq &owner = *batch[h - 1];
     if (h - g > 1) {
       std::lock_guard<std::mutex> lk2(g_srv.mu);
       toff = arena_alloc((h - g) * sizeof(MulJob));
+      if (toff != (size_t)-1) host_table(owner, h - g);
     }
     if (toff != (size_t)-1) {
-      SrvReq &owner = *batch[h - 1];
       owner.table_off = toff;
       owner.table_bytes = (h - g) * sizeof(MulJob);
-      owner.table_host.assign(jobs.begin() + g, jobs.begin() + h);
+      std::copy(jobs.begin() + g, jobs.begin() + h, owner.table_host);
       MulJob *dtab = (MulJob *)(g_srv.arena + toff);
-      (void)hipMemcpyAsync(dtab, owner.table_host.data(), owner.table_bytes, hipMemcpyHostToDevice, s);
+      (void)hipMemcpyAsync(dtab, owner.table_host, owner.table_bytes, hipMemcpyHostToDevice, s);
       JobRef jr{};
       jr.table = dtab;
       rc = enqueue_group(jr, (unsigned)(h - g), batch[g]->n, batch[g]->sqr, s);
@@ -1492,9 +1523,12 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
     const SplitBytes sb = split_bytes(r.d.na, r.d.nb, r.sp);
     (void)hipMemcpyAsync(da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
-    r.table_host.resize(r.sp.k);  // alive until the request completes
-    int rc = split_product_at(da, r.d.na, db, r.d.nb, dout, r.sp, s, work, (uint8_t *)(work + sb.main),
-                              r.table_host.data());
+    MulJob *tab;
+    {
+      std::lock_guard<std::mutex> lk2(g_srv.mu);
+      tab = host_table(r, r.sp.k);
+    }
+    int rc = split_product_at(da, r.d.na, db, r.d.nb, dout, r.sp, s, work, (uint8_t *)(work + sb.main), tab);
     if (rc) r.err = g_err;
     batch.push_back(splits[i]);
   }
@@ -1517,6 +1551,7 @@ void server_loop() {
       if (hipEventQuery(r->done_ev) == hipSuccess) {
         arena_free(r->off, r->bytes);
         if (r->table_off != (size_t)-1) arena_free(r->table_off, r->table_bytes);
+        if (r->host_off != (size_t)-1) range_free(g_srv.table_free, r->host_off, r->host_bytes);
         g_srv.event_pool.push_back(r->done_ev);
         r->done_ev = nullptr;
         r->state = r->err.empty() ? 2 : HGX_EDEVICE;
@@ -1601,6 +1636,13 @@ int server_start_locked() {
   g_srv.arena_bytes = arena;
   g_srv.free_list.clear();
   g_srv.free_list[0] = arena;
+  g_srv.table_free.clear();
+  if (hipHostMalloc((void **)&g_srv.tables, kServeTableBytes, hipHostMallocDefault) == hipSuccess)
+    g_srv.table_free[0] = kServeTableBytes;
+  else {
+    g_srv.tables = nullptr;  // job tables then come from pageable memory
+    (void)hipGetLastError();
+  }
   for (int i = 0; i < kServeStreams; i++) HIPCHK(hipStreamCreateWithFlags(&g_srv.streams[i], hipStreamNonBlocking));
   g_srv.stop = false;
   g_srv.th = std::thread(server_loop);
@@ -1629,6 +1671,9 @@ void server_stop(bool at_exit) {
   dev_free(g_srv.arena, g_srv.arena_bytes);
   g_srv.arena = nullptr;
   g_srv.arena_bytes = 0;
+  if (g_srv.tables) (void)hipHostFree(g_srv.tables);
+  g_srv.tables = nullptr;
+  g_srv.table_free.clear();
   g_srv.queue.clear();
   g_srv.inflight.clear();
   g_srv.started = false;
