@@ -1354,7 +1354,8 @@ struct SrvReq {
   size_t host_off = (size_t)-1, host_bytes = 0;
   std::vector<MulJob> table_spill;
   hipEvent_t done_ev = nullptr;
-  int state = 0;  // 0 queued, 1 in flight, 2 done, <0 failed
+  int stream = -1;  // server stream of its round
+  int state = 0;    // 0 queued, 1 in flight, 2 done, <0 failed
   std::string err;
 };
 
@@ -1375,7 +1376,7 @@ struct Server {
   char *tables = nullptr;               // pinned host pool of job tables
   std::map<size_t, size_t> table_free;  // offset -> size in it
   hipStream_t streams[kServeStreams]{};
-  int next_stream = 0;
+  size_t pending[kServeStreams]{};  // arena bytes of the requests in flight on each stream
   std::vector<hipEvent_t> event_pool;
 };
 
@@ -1552,6 +1553,7 @@ void server_loop() {
         arena_free(r->off, r->bytes);
         if (r->table_off != (size_t)-1) arena_free(r->table_off, r->table_bytes);
         if (r->host_off != (size_t)-1) range_free(g_srv.table_free, r->host_off, r->host_bytes);
+        g_srv.pending[r->stream] -= r->bytes;
         g_srv.event_pool.push_back(r->done_ev);
         r->done_ev = nullptr;
         r->state = r->err.empty() ? 2 : HGX_EDEVICE;
@@ -1600,8 +1602,18 @@ void server_loop() {
       g_srv.queue.pop_front();
     }
     if (!round.empty()) {
-      for (auto &r : round) g_srv.inflight.push_back(r);
-      hipStream_t s = g_srv.streams[g_srv.next_stream++ % kServeStreams];
+      // the stream with the least work in flight (request sizes span four
+      // orders of magnitude: rotating through the streams would queue small
+      // requests behind a large one while other streams idle)
+      int si = 0;
+      for (int i = 1; i < kServeStreams; i++)
+        if (g_srv.pending[i] < g_srv.pending[si]) si = i;
+      for (auto &r : round) {
+        r->stream = si;
+        g_srv.pending[si] += r->bytes;
+        g_srv.inflight.push_back(r);
+      }
+      hipStream_t s = g_srv.streams[si];
       lk.unlock();
       launch_round(round, s);
       lk.lock();
@@ -1636,6 +1648,7 @@ int server_start_locked() {
   g_srv.arena_bytes = arena;
   g_srv.free_list.clear();
   g_srv.free_list[0] = arena;
+  for (auto &b : g_srv.pending) b = 0;
   g_srv.table_free.clear();
   if (hipHostMalloc((void **)&g_srv.tables, kServeTableBytes, hipHostMallocDefault) == hipSuccess)
     g_srv.table_free[0] = kServeTableBytes;
