@@ -70,6 +70,19 @@ def test_mul_matches_oracle(hgx, oracle, na, nb):
     assert np.array_equal(got, want[: na + nb])
 
 
+@pytest.mark.parametrize("bits", range(13, 21))
+def test_every_row_pass_length(hgx, oracle, bits):
+    # transform lengths 2^13 .. 2^20: one column pass and rows of 2^5 .. 2^12
+    # words; up to 2^19 a single product's row pass runs on 2048-word tiles,
+    # at 2^20 on 4096-word tiles
+    n = 1 << (bits - 1)
+    say(f"row pass L=2^{bits}")
+    a = oracle.splitmix64(0x524F5700 + bits, n)
+    b = oracle.splitmix64(0x524F5780 + bits, n - 5)
+    assert np.array_equal(hgx.mul_limbs(a, b), oracle.mpz_mul_ntt(a, b)[: 2 * n - 5])
+    assert np.array_equal(hgx.sqr_limbs(a), oracle.mpz_square(a)[: 2 * n])
+
+
 @pytest.mark.parametrize("n", [1, 2, 50, 101, 2048, 2049, 30000, 65536])
 def test_sqr_matches_oracle(hgx, oracle, n):
     say(f"sqr {n}")
