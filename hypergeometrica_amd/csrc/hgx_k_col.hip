@@ -296,10 +296,174 @@ __global__ void k_splitmix64(u64 *out, u64 n, u64 seed, int force_top) {
   out[i] = z;
 }
 
+// Column passes for launches too small to fill the GPU: the same 4096-word
+// tile (16 columns x 256 rows) and the same residues as k_fwd_col and
+// k_inv_col, on 512 threads that hold 8 rows each, so twice the waves share
+// a tile's work.  The 256-point transform runs as rounds of 8, 8 and 4:
+// stages 0-2 on rows g + 32 j (straight from or to HBM), 3-5 on rows
+// 32 (g / 4) + g % 4 + 4 j and 6-7 on rows 4 g' + j for g' = g, g + 32 (with
+// the twiddles w_B^(t q) and HBM on the other side), two LDS exchanges.
+// LDS slot of row r, column t: 17 r + t (padded as HGX_PAD).
+constexpr int kColNT8 = 512;
+
+template <bool FIRST, int FORM>
+__global__ __launch_bounds__(kColNT8) void k_fwd_col8(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
+                                                     int kbase) {
+  __shared__ u64 lds[HGX_TILE_LDS];
+  __shared__ ulonglong2 tts[FORM == TW_FACTORED ? 256 : 1];
+  const int tid = threadIdx.x;
+  const int t = tid & 15, g = tid >> 4;
+  const u32 bx = FIRST ? blockIdx.x / 3 : blockIdx.x;
+  const int ky = FIRST ? (int)(blockIdx.x - 3 * bx) : (int)blockIdx.y;
+  const MulJob J = load_job(jr, bx >> jt_bits);
+  const u64 tile = bx & ((1u << jt_bits) - 1);
+  const int tpb_bits = bbits - 12;
+  const u64 blk = tile >> tpb_bits;
+  const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
+  const u64 rs = 1ull << (bbits - 8);
+  const u64 cbase = (blk << bbits) + t0;
+  if constexpr (FORM == TW_FACTORED) {
+    if (tid < 256) tts[tid] = gat2(gmem(tw.TT[kbase + ky]), (u32)((t0 >> 4) << 8) + tid);
+  }
+  const int op = blockIdx.z;
+  u64 *res = op ? J.res1 : J.res0;
+  const int k = kbase + ky;
+  const Mod M = mod_of(ps, k);
+  gu64 *arr = gmem(res + (u64)ky * L + cbase);
+  const u32 o1 = t + (u32)rs * g;  // row g of column t; row g + 32 j at o1 + 32 rs j
+  u64 x[8];
+  if (FIRST) {
+    const u64 *limbs = op ? J.b : J.a;
+    const u64 n = op ? J.nb : J.na;
+    const gu64 *lb = gmem(limbs + cbase);
+    if ((blk << bbits) + (rs << 7) >= n) {
+      // rows 128.. (j >= 4) lie at or above the operand's length
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const u64 pos = cbase + 32 * rs * j + o1;
+        x[j] = (pos < n) ? gat(lb + 32 * rs * j, o1) : 0;
+      }
+      round_regs<8, 0, 3, false, true>(x, g, ps.tw_fwd[k], M);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const u64 pos = cbase + 32 * rs * j + o1;
+        x[j] = mod_reduce64((pos < n) ? gat(lb + 32 * rs * j, o1) : 0, M);
+      }
+      round_regs<8, 0, 3, false>(x, g, ps.tw_fwd[k], M);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; j++) x[j] = gat(arr + 32 * rs * j, o1);
+    round_regs<8, 0, 3, false>(x, g, ps.tw_fwd[k], M);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) lds[17 * (g + 32 * j) + t] = x[j];
+  __syncthreads();
+  const int b2 = ((g >> 2) << 5) + (g & 3);
+#pragma unroll
+  for (int j = 0; j < 8; j++) x[j] = lds[17 * (b2 + 4 * j) + t];
+  round_regs<8, 3, 3, false>(x, g & 3, ps.tw_fwd[k], M);
+#pragma unroll
+  for (int j = 0; j < 8; j++) lds[17 * (b2 + 4 * j) + t] = x[j];
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int gg = g + 32 * h;
+    u64 y[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) y[j] = lds[17 * (4 * gg + j) + t];
+    round_regs<8, 6, 2, false>(y, 0, ps.tw_fwd[k], M);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const u32 m = 4 * gg + j;
+      u64 v;
+      if constexpr (FORM == TW_FACTORED)
+        v = apply_twiddle_tt(y[j], tw, k, (u32)t0 + t, bitrev8(m), tts, M);
+      else
+        v = apply_twiddle<FORM>(y[j], tw, k, (u32)t0 + t, bitrev8(m), M);
+      gat(arr, (u32)rs * m + t) = v;
+    }
+  }
+}
+
+template <int FORM>
+__global__ __launch_bounds__(kColNT8) void k_inv_col8(PrimeSet ps, ColTw tw, JobRef jr, int jt_bits, u64 L, int bbits,
+                                                     int kbase) {
+  __shared__ u64 lds[HGX_TILE_LDS];
+  __shared__ ulonglong2 tts[FORM == TW_FACTORED ? 256 : 1];
+  const int tid = threadIdx.x;
+  const int t = tid & 15, g = tid >> 4;
+  const MulJob J = load_job(jr, blockIdx.x >> jt_bits);
+  const u64 tile = blockIdx.x & ((1u << jt_bits) - 1);
+  const int tpb_bits = bbits - 12;
+  const u64 blk = tile >> tpb_bits;
+  const u64 t0 = (tile & ((1ull << tpb_bits) - 1)) << 4;
+  const u64 rs = 1ull << (bbits - 8);
+  const int k = kbase + blockIdx.y;
+  const Mod M = mod_of(ps, k);
+  gu64 *arr = gmem(J.res0 + (u64)blockIdx.y * L + (blk << bbits) + t0);
+  if constexpr (FORM == TW_FACTORED) {
+    if (tid < 256) tts[tid] = gat2(gmem(tw.TT[k]), (u32)((t0 >> 4) << 8) + tid);
+    __syncthreads();
+  }
+  // stages 7-6 on rows 4 g' + j, twiddles w_B^-(t q) applied as they come in
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int gg = g + 32 * h;
+    u64 y[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const u32 m = 4 * gg + j;
+      const u64 v = gat(arr, (u32)rs * m + t);
+      if constexpr (FORM == TW_FACTORED)
+        y[j] = apply_twiddle_tt(v, tw, k, (u32)t0 + t, bitrev8(m), tts, M);
+      else
+        y[j] = apply_twiddle<FORM>(v, tw, k, (u32)t0 + t, bitrev8(m), M);
+    }
+    round_regs<8, 6, 2, true>(y, 0, ps.tw_inv[k], M);
+#pragma unroll
+    for (int j = 0; j < 4; j++) lds[17 * (4 * gg + j) + t] = y[j];
+  }
+  __syncthreads();
+  const int b2 = ((g >> 2) << 5) + (g & 3);
+  u64 x[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) x[j] = lds[17 * (b2 + 4 * j) + t];
+  round_regs<8, 3, 3, true>(x, g & 3, ps.tw_inv[k], M);
+#pragma unroll
+  for (int j = 0; j < 8; j++) lds[17 * (b2 + 4 * j) + t] = x[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 8; j++) x[j] = lds[17 * (g + 32 * j) + t];
+  round_regs<8, 0, 3, true>(x, g, ps.tw_inv[k], M);
+  const u32 o1 = t + (u32)rs * g;
+#pragma unroll
+  for (int j = 0; j < 8; j++) gat(arr + 32 * rs * j, o1) = x[j];
+}
+
+// a column launch with fewer workgroups than this (two per CU) runs the
+// 512-thread kernels above
+constexpr unsigned kColSmallBelow = 512;
+
+static bool col_small(dim3 grid, int bbits) { return bbits <= 28 && grid.x * grid.y * grid.z < kColSmallBelow; }
+
 static int tw_form(const ColTw &tw) { return tw.D[0] ? TW_DIRECT : (tw.DM[0] ? TW_DIRECT_MONT : TW_FACTORED); }
 
 void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr,
                         int jt_bits, u64 L, int bbits, int kbase) {
+  if (col_small(grid, bbits)) {
+    const dim3 b(kColNT8);
+    switch (tw_form(tw) + (first ? 3 : 0)) {
+      case TW_FACTORED: hipLaunchKernelGGL((k_fwd_col8<false, TW_FACTORED>), grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
+      case TW_DIRECT: hipLaunchKernelGGL((k_fwd_col8<false, TW_DIRECT>), grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
+      case TW_DIRECT_MONT: hipLaunchKernelGGL((k_fwd_col8<false, TW_DIRECT_MONT>), grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
+      case 3 + TW_FACTORED: hipLaunchKernelGGL((k_fwd_col8<true, TW_FACTORED>), grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
+      case 3 + TW_DIRECT: hipLaunchKernelGGL((k_fwd_col8<true, TW_DIRECT>), grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
+      default: hipLaunchKernelGGL((k_fwd_col8<true, TW_DIRECT_MONT>), grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
+    }
+    return;
+  }
   const bool big = bbits > 28;  // tile-factored twiddles only (direct forms stop at 2^28)
   switch (tw_form(tw) + (first ? 3 : 0)) {
     case TW_FACTORED:
@@ -337,6 +501,15 @@ void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps
 
 void hgx_launch_inv_col(dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr, int jt_bits,
                         u64 L, int bbits, int kbase) {
+  if (col_small(grid, bbits)) {
+    const dim3 b(kColNT8);
+    switch (tw_form(tw)) {
+      case TW_FACTORED: hipLaunchKernelGGL(k_inv_col8<TW_FACTORED>, grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
+      case TW_DIRECT: hipLaunchKernelGGL(k_inv_col8<TW_DIRECT>, grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
+      default: hipLaunchKernelGGL(k_inv_col8<TW_DIRECT_MONT>, grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
+    }
+    return;
+  }
   switch (tw_form(tw)) {
     case TW_FACTORED:
       if (bbits > 28)
