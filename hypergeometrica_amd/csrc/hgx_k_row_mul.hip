@@ -38,24 +38,34 @@ bool hgx_launch_row_mul(bool single, int sbits, dim3 grid, hipStream_t s, const 
   return false;
 }
 
-template <bool TWO>
+template <bool TWO, int TB>
 static bool launch_half(int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp, const JobRef &jr,
                         int jt_bits) {
   switch (sbits) {
-  case 4: hipLaunchKernelGGL((k_row_half<4, TWO>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
-  case 5: hipLaunchKernelGGL((k_row_half<5, TWO>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
-  case 6: hipLaunchKernelGGL((k_row_half<6, TWO>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
-  case 7: hipLaunchKernelGGL((k_row_half<7, TWO>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
-  case 8: hipLaunchKernelGGL((k_row_half<8, TWO>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
-  case 9: hipLaunchKernelGGL((k_row_half<9, TWO>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
-  case 10: hipLaunchKernelGGL((k_row_half<10, TWO>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
-  case 11: hipLaunchKernelGGL((k_row_half<11, TWO>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
+  case 4: hipLaunchKernelGGL((k_row_half<4, TWO, TB>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
+  case 5: hipLaunchKernelGGL((k_row_half<5, TWO, TB>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
+  case 6: hipLaunchKernelGGL((k_row_half<6, TWO, TB>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
+  case 7: hipLaunchKernelGGL((k_row_half<7, TWO, TB>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
+  case 8: hipLaunchKernelGGL((k_row_half<8, TWO, TB>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
+  case 9: hipLaunchKernelGGL((k_row_half<9, TWO, TB>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
+  case 10: hipLaunchKernelGGL((k_row_half<10, TWO, TB>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits); return true;
+  case 11:
+    if constexpr (TB >= 11) {
+      hipLaunchKernelGGL((k_row_half<11, TWO, TB>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jt_bits);
+      return true;
+    }
+    return false;
   }
   return false;
 }
 
-bool hgx_launch_row_half(bool two, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
-                         const JobRef &jr, int jt_bits) {
-  return two ? launch_half<true>(sbits, grid, s, ps, rp, jr, jt_bits)
-             : launch_half<false>(sbits, grid, s, ps, rp, jr, jt_bits);
+bool hgx_launch_row_half(bool two, int tb, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps,
+                         const RowParams &rp, const JobRef &jr, int jt_bits) {
+  if (tb == 11)
+    return two ? launch_half<true, 11>(sbits, grid, s, ps, rp, jr, jt_bits)
+               : launch_half<false, 11>(sbits, grid, s, ps, rp, jr, jt_bits);
+  if (tb == 10)
+    return two ? launch_half<true, 10>(sbits, grid, s, ps, rp, jr, jt_bits)
+               : launch_half<false, 10>(sbits, grid, s, ps, rp, jr, jt_bits);
+  return false;
 }

@@ -616,15 +616,15 @@ __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRe
   }
 }
 
-// Row pass of a product (TWO) or square on tiles of 2048 words (rows of
-// 2^S_BITS <= 2^11 words) with radix-8 rounds: twice the workgroups of
-// k_row's 4096-word tiles for the same rows, which lowers the latency of
-// products too small to fill the GPU (the runtime uses it when k_row would
-// launch fewer than two workgroups per CU).  Same residues as k_row.
-template <int S_BITS, bool TWO>
+// Row pass of a product (TWO) or square on tiles of 2^TB words (rows of
+// 2^S_BITS <= 2^TB words): 2048-word tiles with radix-8 rounds or 1024-word
+// tiles with radix-4 rounds, two or four times the workgroups of k_row's
+// 4096-word tiles for the same rows, which lowers the latency of products
+// too small to fill the GPU (the runtime uses them when k_row would launch
+// fewer than two workgroups per CU).  Same residues as k_row.
+template <int S_BITS, bool TWO, int TB>
 __global__ __launch_bounds__(HGX_NT) void k_row_half(PrimeSet ps, RowParams rp, JobRef jr, int jt_bits) {
-  static_assert(S_BITS <= HGX_HALF_TILE_BITS, "whole rows per tile");
-  constexpr int TB = HGX_HALF_TILE_BITS;
+  static_assert(S_BITS <= TB && (TB == 10 || TB == 11), "whole rows per tile, one group per thread per round");
   constexpr int NLDS = (1 << TB) + (1 << TB) / 16;
   __shared__ u64 ldsA[NLDS];
   __shared__ u64 ldsB[TWO ? NLDS : 1];
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(HGX_NT) void k_row_half(PrimeSet ps, RowParams rp, 
   const Mod M = mod_of(ps, k);
   u64 *A = J.res0 + (u64)blockIdx.y * rp.L + gbase;
   const u64 *Bp = TWO ? (J.res1 + (u64)blockIdx.y * rp.L + gbase) : nullptr;
-  row_mul_tile<S_BITS, TWO, 3>(A, Bp, ldsA, ldsB, TB - S_BITS, ps.tw_fwd[k], ps.tw_inv[k], M);
+  row_mul_tile<S_BITS, TWO, TB - 8>(A, Bp, ldsA, ldsB, TB - S_BITS, ps.tw_fwd[k], ps.tw_inv[k], M);
 }
 
 // Transform-only inverse when L <= 4096 (ntt-reverse on one tile, with the

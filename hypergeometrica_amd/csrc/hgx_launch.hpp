@@ -14,10 +14,10 @@ void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &
 // row passes; each returns false when SBITS is out of range
 bool hgx_launch_row_mul(bool single, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
                         const JobRef &jr, int jt_bits);
-// multiply row pass of a product (two) or square on 2048-word tiles, rows of
-// at most 2^11 words
-bool hgx_launch_row_half(bool two, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
-                         const JobRef &jr, int jt_bits);
+// multiply row pass of a product (two) or square on 2^tb-word tiles (tb 10
+// or 11), rows of at most 2^tb words
+bool hgx_launch_row_half(bool two, int tb, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps,
+                         const RowParams &rp, const JobRef &jr, int jt_bits);
 bool hgx_launch_row_sqr(bool single, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
                         const JobRef &jr, int jt_bits);
 bool hgx_launch_row_tr(int mode, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,

@@ -796,11 +796,14 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     rp.kbase = 0;
     if (plan->srow <= HGX_HALF_TILE_BITS && 3ull * grid < kHalfTileBelow) {
       // too few 4096-word tiles to fill the GPU: 2048-word tiles, twice the
-      // workgroups, for a lower latency
+      // workgroups (1024-word tiles, four times, when that is still few),
+      // for a lower latency
       HGX_TIMED(sqr ? "row_sqr_half" : "row_mul_half", s);
-      const int jh = n - HGX_HALF_TILE_BITS;
-      rp.tw_bits = HGX_HALF_TILE_BITS;
-      if (!hgx_launch_row_half(!sqr, plan->srow, dim3(njobs << jh, 3), s, g_ctx.ps, rp, jr, jh))
+      const int tb = (plan->srow < HGX_HALF_TILE_BITS && 6ull * grid < kHalfTileBelow) ? HGX_HALF_TILE_BITS - 1
+                                                                                        : HGX_HALF_TILE_BITS;
+      const int jh = n - tb;
+      rp.tw_bits = tb;
+      if (!hgx_launch_row_half(!sqr, tb, plan->srow, dim3(njobs << jh, 3), s, g_ctx.ps, rp, jr, jh))
         return fail(HGX_EINVAL, "row pass size out of range");
     } else {
       rc = sqr ? launch_row<MODE_SQR, false>(plan->srow, dim3(grid, 3), s, rp, jr, jt)
