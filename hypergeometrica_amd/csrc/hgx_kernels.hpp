@@ -32,6 +32,9 @@
 //                     sum of every 16-limb run, with each run's overflow
 //   k_carry_*         resolves the run-to-run carries (block scan)
 //   For L <= 4096 one k_row<..., SINGLE> does all the transform work.
+//   Launches too small to fill the GPU (fewer than 512 of these workgroups)
+//   run k_fwd_col8 / k_inv_col8 (512 threads per column tile) and k_row_half
+//   (2048- or 1024-word row tiles) instead, with the same results.
 #pragma once
 #include "hgx_types.hpp"
 
