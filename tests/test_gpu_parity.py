@@ -468,12 +468,14 @@ def _plan(L, bits):
     return n.value, d.value, m.value
 
 
-@pytest.mark.parametrize("bits", [20, 21, 28])
+@pytest.mark.parametrize("bits", [16, 20, 21, 28])
 def test_every_pass_structure_gives_the_same_results(hgx, oracle, bits):
     # each column-pass count, direct-table limit and direct Montgomery flag:
     # products against GMP (or the mod 2^61-1 residue at 2^28), squares, and
     # transforms against the oracle; the default is restored after.  At 2^28
-    # the direct-table limits change only the 2^20 pass: one is enough.
+    # the direct-table limits change only the 2^20 pass: one is enough.  At
+    # 2^16 every launch is small: the 512-thread column kernels run with each
+    # twiddle form.
     from hypergeometrica_amd import _lib
     L = _lib.lib()
     default = _plan(L, bits)
