@@ -31,8 +31,9 @@ namespace {
 
 constexpr int kMaxBits = 32;   // largest transform: 2^32 words per residue array
 constexpr int kRowMaxBits = 12;
-// a multiply row pass with fewer 4096-word workgroups than this (two per CU)
-// runs on 2048-word tiles instead
+// a multiply row pass with fewer 4096-word workgroups than this (two per CU
+// of the 256) runs on 2048-word tiles instead, and on 1024-word tiles when
+// the 2048-word ones are still fewer
 constexpr u64 kHalfTileBelow = 512;
 
 thread_local std::string g_err;
