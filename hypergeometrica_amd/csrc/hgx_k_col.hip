@@ -57,17 +57,19 @@ __global__ __launch_bounds__(HGX_NT) void k_inv_col(PrimeSet ps, ColTw tw, JobRe
   for (int j = 0; j < 16; j++) gat(arr + 16 * rs * j, o1) = x[j];
 }
 
-// CRT and carry sums.  A workgroup takes a block of min(L, 1024) consecutive
-// coefficients: the threads reconstruct them from the three residue arrays
-// (coalesced loads, Garner form) into LDS; then thread t sums its run of 4
-// coefficients exactly in registers (4 digits + a 2-word overflow); the
+// CRT and carry sums.  A workgroup takes a block of min(L, 2^cbits)
+// consecutive coefficients (1024, or 512 when blocks of 1024 would be too few
+// to fill the GPU): the threads reconstruct them from the three residue
+// arrays (coalesced loads, Garner form) into LDS; then thread t sums its run
+// of 4 coefficients exactly in registers (4 digits + a 2-word overflow); the
 // overflows are carried from run to run inside the block (kill / propagate /
-// generate scan over the 256 runs), so the block leaves exact digits plus one
-// 2-word overflow and an all-ones flag for limbs 2.. of the block.
+// generate scan over the block's 256 or 128 runs), so the block leaves exact
+// digits plus one 2-word overflow and an all-ones flag for limbs 2.. of the
+// block.
 // add-big-digit (src/ntt-multiply.lisp:10-51) adds the same values at the
 // same positions.  grid = jobs << bpj_bits
 static_assert(HGX_CRT_BLOCK == HGX_NT * HGX_RUN, "one run per thread");
-__global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_bits, u64 L) {
+__global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_bits, int cbits, u64 L) {
   constexpr int PADN = HGX_CRT_BLOCK + HGX_CRT_BLOCK / 16;
   __shared__ u64 w0[PADN];
   __shared__ u64 w1[PADN];
@@ -77,11 +79,11 @@ __global__ __launch_bounds__(HGX_NT) void k_crt(PrimeSet ps, JobRef jr, int bpj_
   const int tid = threadIdx.x;
   const MulJob J = load_job(jr, blockIdx.x >> bpj_bits);
   const u64 blk = blockIdx.x & ((1u << bpj_bits) - 1);
-  const int P = L < HGX_CRT_BLOCK ? (int)L : HGX_CRT_BLOCK;
+  const int P = L < (1ull << cbits) ? (int)L : (1 << cbits);  // cbits <= HGX_CRT_BLOCK_BITS
   const int rl = P < HGX_RUN ? P : HGX_RUN;
   const int nruns = P / rl;
   const bool act = tid < nruns;
-  const u64 base = blk * HGX_CRT_BLOCK;
+  const u64 base = blk << cbits;
   const gu64 *r0 = gmem(J.res0 + base), *r1 = gmem(J.res0 + L + base), *r2 = gmem(J.res0 + 2 * L + base);
   for (int x = tid; x < P; x += HGX_NT) {
     const U192 cf = garner3(gat(r0, x), gat(r1, x), gat(r2, x), ps);
@@ -547,8 +549,8 @@ void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT
   hipLaunchKernelGGL(k_direct_mont, dim3(4096), dim3(256), 0, s, DM, TS, TT, tbits, M, rm, rmq);
 }
 
-void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &jr, int bpj_bits, u64 L) {
-  hipLaunchKernelGGL(k_crt, grid, dim3(HGX_NT), 0, s, ps, jr, bpj_bits, L);
+void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &jr, int bpj_bits, int cbits, u64 L) {
+  hipLaunchKernelGGL(k_crt, grid, dim3(HGX_NT), 0, s, ps, jr, bpj_bits, cbits, L);
 }
 
 void hgx_launch_carry_map(dim3 grid, hipStream_t s, const JobRef &jr, int bpj_bits, u64 nruns, int rl) {

@@ -10,7 +10,8 @@ void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps
                         int jt_bits, u64 L, int bbits, int kbase);
 void hgx_launch_inv_col(dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr, int jt_bits,
                         u64 L, int bbits, int kbase);
-void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &jr, int bpj_bits, u64 L);
+// blocks of 2^cbits limbs (cbits 9 or 10)
+void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &jr, int bpj_bits, int cbits, u64 L);
 // row passes; each returns false when SBITS is out of range
 bool hgx_launch_row_mul(bool single, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
                         const JobRef &jr, int jt_bits);

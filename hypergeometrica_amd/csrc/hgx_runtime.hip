@@ -35,6 +35,18 @@ constexpr int kRowMaxBits = 12;
 // of the 256) runs on 2048-word tiles instead, and on 1024-word tiles when
 // the 2048-word ones are still fewer
 constexpr u64 kHalfTileBelow = 512;
+// transforms up to this length can run the CRT on blocks of 512 limbs (twice
+// the workgroups) when blocks of 1024 would be fewer than kHalfTileBelow; their
+// carry arrays are sized for 512-limb blocks
+constexpr int kCrtHalfMaxBits = 18;
+
+// log2 of the CRT block (and carry run) for NJOBS products of length 2^n
+int crt_block_bits(int n, unsigned njobs) {
+  if (n <= kCrtHalfMaxBits && n > HGX_CRT_BLOCK_BITS &&
+      ((u64)njobs << (n - HGX_CRT_BLOCK_BITS)) < kHalfTileBelow)
+    return HGX_CRT_BLOCK_BITS - 1;
+  return HGX_CRT_BLOCK_BITS;
+}
 
 thread_local std::string g_err;
 
@@ -708,7 +720,8 @@ struct WorkLayout {
 };
 WorkLayout work_layout(int n, bool sqr) {
   const u64 L = 1ull << n;
-  const u64 rl = std::min<u64>(L, HGX_CRT_BLOCK);
+  // carry runs of the smallest CRT block the length may use
+  const u64 rl = std::min<u64>(L, 1ull << (n <= kCrtHalfMaxBits ? HGX_CRT_BLOCK_BITS - 1 : HGX_CRT_BLOCK_BITS));
   const u64 nruns = L / rl;
   const u64 nblk = (nruns + HGX_CARRY_RUNS_PER_BLOCK - 1) / HGX_CARRY_RUNS_PER_BLOCK;
   WorkLayout w{};
@@ -760,7 +773,8 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
   int rc = build_plan(n, &plan);
   if (rc) return rc;
   const u64 L = 1ull << n;
-  const u64 rl = std::min<u64>(L, HGX_CRT_BLOCK);  // carry runs are whole CRT blocks
+  const int cbits = crt_block_bits(n, njobs);
+  const u64 rl = std::min<u64>(L, 1ull << cbits);  // carry runs are whole CRT blocks
   const u64 nruns = L / rl;
   const int nops = sqr ? 1 : 2;
 
@@ -818,11 +832,11 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     }
   }
   {
-    const int cb = n > HGX_CRT_BLOCK_BITS ? n - HGX_CRT_BLOCK_BITS : 0;  // CRT blocks per job (log2)
+    const int cb = n > cbits ? n - cbits : 0;  // CRT blocks per job (log2)
     HGX_TIMED("crt", s);
-    hgx_launch_crt(dim3(njobs << cb), s, g_ctx.ps, jr, cb, L);
+    hgx_launch_crt(dim3(njobs << cb), s, g_ctx.ps, jr, cb, cbits, L);
   }
-  // run-to-run carries between the CRT blocks of 1024 limbs
+  // run-to-run carries between the CRT blocks
   if (nruns > 1 && nruns <= 1024) {
     HGX_TIMED("carry_small", s);
     hgx_launch_carry_small(dim3(njobs), s, jr, nruns, (int)rl);
