@@ -273,10 +273,10 @@ __global__ __launch_bounds__(HGX_NT) void k_carry_apply(JobRef jr, int bpj_bits,
   }
 }
 
-// All three steps in one workgroup of 1024 threads per job when a job has at
-// most 1024 runs (L <= 2^20): thread r takes run r's map, one scan over the
-// workgroup gives every run its incoming carry, and the thread adds it.
-// grid = jobs
+// All three steps in one workgroup per job when a job has at most 1024 runs
+// (L <= 2^20): thread r takes run r's map, one scan over the workgroup gives
+// every run its incoming carry, and the thread adds it.  grid = jobs,
+// blockDim = the runs rounded up to a power of two, at least 64.
 __global__ __launch_bounds__(1024) void k_carry_small(JobRef jr, u64 nruns, int rl) {
   __shared__ u32 sh[1024];
   const u64 r = threadIdx.x;
@@ -566,7 +566,11 @@ void hgx_launch_carry_apply(dim3 grid, hipStream_t s, const JobRef &jr, int bpj_
 }
 
 void hgx_launch_carry_small(dim3 grid, hipStream_t s, const JobRef &jr, u64 nruns, int rl) {
-  hipLaunchKernelGGL(k_carry_small, grid, dim3(1024), 0, s, jr, nruns, rl);
+  // one thread per run (nruns <= 1024), whole waves: fewer scan steps for
+  // fewer runs
+  unsigned nt = 64;
+  while (nt < nruns) nt <<= 1;
+  hipLaunchKernelGGL(k_carry_small, grid, dim3(nt), 0, s, jr, nruns, rl);
 }
 
 void hgx_launch_splitmix64(dim3 grid, hipStream_t s, u64 *out, u64 n, u64 seed, int force_top) {
