@@ -32,6 +32,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from hypergeometrica_amd import workload as W  # noqa: E402
+
 HBM_PEAK = 8.0e12
 VALU_PEAK = 1024 * 2.4e9 / 4  # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz, a wave64 VALU instruction per 4 cycles
 METRIC = "NTT-multiply limbs/s batched vs per-call on GPU; %HBM roofline; p99 latency"
@@ -103,21 +105,13 @@ def cpu_baseline(sample_bits: int = 22) -> dict:
     }
 
 
-def log_uniform_sizes(seed: int, count: int, lo_bits: float, hi_bits: float) -> list[int]:
-    """n = floor(2^(lo + (hi - lo) u)), u = SplitMix64 output / 2^64."""
-    from oracle.oracle import splitmix64_py
-    us = splitmix64_py(seed, count)
-    return [max(1, int(2 ** (lo_bits + (hi_bits - lo_bits) * (u / 2**64)))) for u in us]
-
-
 def batched_leg(torch, h, _lib, count: int = 4096, reps: int = 3) -> dict:
     """Binary-splitting level: COUNT independent products, operand sizes
     log-uniform in [2^8, 2^16] limbs (seed 0x48475803), operands in HBM.
     Batched (hgx_mul_batch_device, grouped by transform length) against the
     per-call path (hgx_mul_device per product); outputs compared."""
     import ctypes
-    sizes = log_uniform_sizes(0x48475803, 2 * count, 8, 16)
-    na, nb = sizes[0::2], sizes[1::2]
+    na, nb = W.batch_shapes(count, W.BATCH_SEED)
     tot_in = sum(na) + sum(nb)
     tot_out = tot_in
     dev = torch.device("cuda:0")
@@ -293,85 +287,227 @@ def pi_leg(bits: int = 1 << 20) -> dict:
             "products": g.products, "max_product_limbs": g.max_product_limbs}
 
 
-def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 0.5, slots: int = 32, seed: int = 0x48475804,
-                warm: bool = True) -> dict:
-    """Request server under load: Poisson arrivals at RATE/s for SECONDS,
-    operand sizes log-uniform in [2^10, 2^24] limbs (SEED), operands and
-    products in pinned host memory (operands are slices of two random
-    2^24-limb buffers; SLOTS output buffers bound the requests in flight).
-    The root tables of every transform length in the range are built before
-    the stream starts (one request per length), as a server keeps them
-    resident.  Latency = completion seen by the client - scheduled arrival."""
+def pcie_probe(torch, mib: int = 1024, reps: int = 3) -> dict:
+    """Host<->device copy rates of pinned memory on this box: each direction
+    alone and both at once (two streams), best of REPS."""
+    dev = torch.device("cuda:0")
+    n = mib << 17
+    H1 = torch.empty(n, dtype=torch.int64, pin_memory=True)
+    H2 = torch.empty(n, dtype=torch.int64, pin_memory=True)
+    D1 = torch.empty(n, dtype=torch.int64, device=dev)
+    D2 = torch.empty(n, dtype=torch.int64, device=dev)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+
+    def timed(fn):
+        best = float("inf")
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return 8 * n / best / 1e9
+
+    def both():
+        with torch.cuda.stream(s1):
+            D1.copy_(H1, non_blocking=True)
+        with torch.cuda.stream(s2):
+            H2.copy_(D2, non_blocking=True)
+
+    up = timed(lambda: D1.copy_(H1, non_blocking=True))
+    down = timed(lambda: H2.copy_(D2, non_blocking=True))
+    bi = timed(both)
+    del H1, H2, D1, D2
+    torch.cuda.empty_cache()
+    torch._C._host_emptyCache()
+    # a served request moves 8 bytes per output limb up (its operands) and 8
+    # down (its product), both directions at once
+    return {"h2d_GBps": round(up, 1), "d2h_GBps": round(down, 1), "both_ways_GBps_each": round(bi, 1),
+            "bound_limbs_per_s": bi * 1e9 / 8}
+
+
+class _Stream:
+    """Pinned host buffers of the serving legs: two random 2^24-limb operand
+    buffers (requests take slices of them) with their prefix residues mod
+    2^61 - 1, and SLOTS product buffers."""
+
+    def __init__(self, torch, slots: int, seed: int):
+        import numpy as np
+
+        from tests import _residue as RES
+        self.nmax = 1 << 24
+        self.A = torch.empty(self.nmax, dtype=torch.int64, pin_memory=True)
+        self.B = torch.empty(self.nmax, dtype=torch.int64, pin_memory=True)
+        self.a = self.A.numpy().view(np.uint64)
+        self.b = self.B.numpy().view(np.uint64)
+        self.a[:] = W.splitmix64(seed + 3, self.nmax)
+        self.b[:] = W.splitmix64(seed + 4, self.nmax)
+        self.PA = RES.prefix_residues(self.a)
+        self.PB = RES.prefix_residues(self.b)
+        self.outs = [torch.empty(2 * self.nmax, dtype=torch.int64, pin_memory=True) for _ in range(slots)]
+
+    def release(self, torch):
+        del self.A, self.B, self.outs, self.a, self.b
+        torch._C._host_emptyCache()
+
+
+def serve_stream(torch, _lib, st: _Stream, rate: float, seconds: float, seed: int, checkers: int = 12,
+                 gmp_sample: int = 0, check: bool = True) -> dict:
+    """Poisson arrivals at RATE/s for SECONDS through hgx_submit, operand
+    sizes log-uniform in [2^10, 2^24] limbs (SEED), operands slices of the
+    pinned buffers of ST, products into its slots.  Latency = completion seen
+    by the client - scheduled arrival.  Every product is checked: its residue
+    mod 2^61 - 1 (a checker thread pool, which frees the slot afterwards)
+    against the product of the operands' residues; GMP_SAMPLE of them are
+    also compared limb for limb with GMP after the stream."""
     import ctypes
-    import math
+    import queue
+    from concurrent.futures import ThreadPoolExecutor
 
     import numpy as np
-    from oracle.oracle import splitmix64_py
+
+    from tests import _gmp
+    from tests import _residue as RES
     L_ = _lib.lib()
     count = max(1, int(rate * seconds))
-    sizes = log_uniform_sizes(seed, 2 * count, 10, 24)
-    us = splitmix64_py(seed + 1, count)
-    gaps = [-math.log(1.0 - (u + 0.5) / 2**64) / rate for u in us]
-    nmax = 1 << 24
-    rng = np.random.default_rng(seed + 2)
-    A = torch.empty(nmax, dtype=torch.int64, pin_memory=True)
-    B = torch.empty(nmax, dtype=torch.int64, pin_memory=True)
-    A.numpy()[:] = rng.integers(-2**63, 2**63, size=nmax, dtype=np.int64)
-    B.numpy()[:] = rng.integers(-2**63, 2**63, size=nmax, dtype=np.int64)
-    outs = [torch.empty(2 * nmax, dtype=torch.int64, pin_memory=True) for _ in range(slots)]
-    if warm:
+    na_l, nb_l = W.serve_shapes(count, seed)
+    gaps = W.poisson_gaps(seed + 1, count, rate)
+    offs = [int(v) for v in W.splitmix64(seed + 2, count)]
+    free = queue.SimpleQueue()
+    for k in range(len(st.outs)):
+        free.put(k)
+    sample_every = max(1, count // gmp_sample) if gmp_sample else 0
+    kept = {}
+    mism = []
+    checked = [0]
+    pool = ThreadPoolExecutor(max_workers=checkers)
+
+    def check_one(i, slot, n, want):
+        try:
+            if not check:
+                return
+            got = RES.residue_ptr(st.outs[slot].data_ptr(), n)
+            if sample_every and i % sample_every == 0:
+                kept[i] = st.outs[slot].numpy()[:n].view(np.uint64).copy()
+            if got != want:
+                mism.append(i)
+            checked[0] += 1
+        finally:
+            free.put(slot)
+
+    inflight = {}
+    lat, arr_t, limbs = [], [], 0
+
+    def reap():
+        nonlocal limbs
+        # hgx_poll releases a ticket when it reports completion; a failed
+        # request raises here
+        done = [t for t in inflight if _lib.check(L_.hgx_poll(ctypes.c_uint64(t))) == 1]
+        now = time.perf_counter()
+        for t in done:
+            i, arr, slot, n, want = inflight.pop(t)
+            lat.append(now - arr)
+            arr_t.append(arr)
+            limbs += n
+            pool.submit(check_one, i, slot, n, want)
+        return bool(done)
+
+    stats0 = _lib.server_stats()
+    t0 = time.perf_counter()
+    sched = t0
+    for i in range(count):
+        sched += gaps[i]
+        na, nb = na_l[i], nb_l[i]
+        while time.perf_counter() < sched:
+            reap()
+        while True:
+            try:
+                slot = free.get_nowait()
+                break
+            except queue.Empty:
+                reap()
+        oa = (offs[i] >> 8) % (st.nmax - na + 1)
+        ob = (offs[i] >> 32) % (st.nmax - nb + 1)
+        want = RES.mulmod(RES.slice_residue(st.PA, oa, na), RES.slice_residue(st.PB, ob, nb))
+        d = _lib.MulDesc(st.A.data_ptr() + 8 * oa, na, st.B.data_ptr() + 8 * ob, nb, st.outs[slot].data_ptr())
+        t = ctypes.c_uint64(0)
+        _lib.check(L_.hgx_submit(ctypes.byref(d), ctypes.byref(t)))
+        inflight[t.value] = (i, sched, slot, na + nb, want)
+    while inflight:
+        reap()
+    wall = time.perf_counter() - t0
+    pool.shutdown(wait=True)
+    stats1 = _lib.server_stats()
+    gmp_bad = 0
+    for i, got in kept.items():
+        na, nb = na_l[i], nb_l[i]
+        oa = (offs[i] >> 8) % (st.nmax - na + 1)
+        ob = (offs[i] >> 32) % (st.nmax - nb + 1)
+        if not np.array_equal(got, _gmp.mul(st.a[oa:oa + na], st.b[ob:ob + nb])):
+            gmp_bad += 1
+    order = sorted(range(len(lat)), key=lambda k: arr_t[k])
+    half = len(order) // 2
+
+    def q(xs, f):
+        xs = sorted(xs)
+        return round(1e3 * xs[min(len(xs) - 1, int(f * len(xs)))], 3) if xs else None
+
+    first = [lat[k] for k in order[:half]]
+    second = [lat[k] for k in order[half:]]
+    p99a, p99b = q(first, 0.99), q(second, 0.99)
+    launched = stats1["launch_s"] - stats0["launch_s"]
+    return {"rate_per_s": rate, "seconds": seconds, "requests": count, "limbs": limbs,
+            "offered_limbs_per_s": (sum(na_l) + sum(nb_l)) / seconds, "wall_s": round(wall, 4),
+            "limbs_per_s": limbs / wall, "p50_ms": q(lat, 0.5), "p99_ms": q(lat, 0.99),
+            "p99_first_half_ms": p99a, "p99_second_half_ms": p99b,
+            "bounded": bool(p99a and p99b and p99b <= 1.5 * p99a and wall <= seconds + 1.0),
+            "outputs_checked": checked[0], "mismatches": len(mism), "gmp_checked": len(kept),
+            "gmp_mismatches": gmp_bad,
+            "server_rounds": stats1["rounds"] - stats0["rounds"],
+            "server_thread_launch_frac": round(launched / wall, 4)}
+
+
+def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 10.0, slots: int = 32,
+                seed: int = W.SERVE_SEED, search=(1500.0, 1200.0, 1000.0, 800.0, 600.0, 400.0),
+                search_seconds: float = 4.0) -> dict:
+    """Request server under load (the serving configuration): Poisson
+    arrivals at RATE/s for SECONDS, then streams of SEARCH_SECONDS at
+    decreasing rates until one keeps its p99 bounded (the second half's p99
+    at most 1.5x the first half's, and the stream drained within a second of
+    its end).  The root tables of every transform length in the range are
+    built first (one request per length), as a server keeps them resident.
+    Every served product is checked (serve_stream)."""
+    import ctypes
+    L_ = _lib.lib()
+    st = _Stream(torch, slots, seed)
+    try:
         for bits in range(10, 25):
-            d = _lib.MulDesc(A.data_ptr(), 1 << bits, B.data_ptr(), 1 << bits, outs[0].data_ptr())
+            d = _lib.MulDesc(st.A.data_ptr(), 1 << bits, st.B.data_ptr(), 1 << bits, st.outs[0].data_ptr())
             t = ctypes.c_uint64(0)
             _lib.check(L_.hgx_submit(ctypes.byref(d), ctypes.byref(t)))
             _lib.check(L_.hgx_wait(t))
-    free = list(range(slots))
-    inflight = {}
-    lat, limbs = [], 0
-    t0 = time.perf_counter()
-    sched = t0
-
-    def reap(block: bool):
-        nonlocal limbs
-        while True:
-            done = [t for t in inflight if L_.hgx_poll(ctypes.c_uint64(t)) == 1]
-            now = time.perf_counter()
-            for t in done:
-                arr, slot, n = inflight.pop(t)
-                _lib.check(L_.hgx_wait(ctypes.c_uint64(t)))
-                lat.append(now - arr)
-                limbs += n
-                free.append(slot)
-            if done or not block or not inflight:
-                return
-
-    for i in range(count):
-        sched += gaps[i]
-        na, nb = sizes[2 * i], sizes[2 * i + 1]
-        while time.perf_counter() < sched:
-            reap(False)
-        while not free:
-            reap(True)
-        slot = free.pop()
-        oa = (us[i] >> 8) % (nmax - na + 1)
-        ob = (us[i] >> 32) % (nmax - nb + 1)
-        d = _lib.MulDesc(A.data_ptr() + 8 * oa, na, B.data_ptr() + 8 * ob, nb, outs[slot].data_ptr())
-        t = ctypes.c_uint64(0)
-        _lib.check(L_.hgx_submit(ctypes.byref(d), ctypes.byref(t)))
-        inflight[t.value] = (sched, slot, na + nb)
-    while inflight:
-        reap(True)
-    wall = time.perf_counter() - t0
-    lat.sort()
-    offered = sum(sizes) / seconds
-    return {"workload": f"Poisson arrivals {rate:.0f}/s for {seconds} s ({count} requests, seed {seed:#x}), operand "
-                        f"limbs log-uniform in [2^10, 2^24], pinned host buffers, {slots} output slots, root tables "
-                        f"built before the stream",
-            "requests": count, "limbs": limbs, "offered_limbs_per_s": offered, "wall_s": round(wall, 4),
-            "limbs_per_s": limbs / wall,
-            "p50_ms": round(1e3 * lat[len(lat) // 2], 3),
-            "p99_ms": round(1e3 * lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
-            "mean_request_limbs": limbs / count}
+        main = serve_stream(torch, _lib, st, rate, seconds, seed, gmp_sample=6)
+        sustained = None
+        tried = []
+        for r in search:
+            res = serve_stream(torch, _lib, st, r, search_seconds, seed + 0x100 + int(r))
+            tried.append({k: res[k] for k in ("rate_per_s", "limbs_per_s", "p50_ms", "p99_ms", "p99_first_half_ms",
+                                               "p99_second_half_ms", "bounded", "outputs_checked", "mismatches")})
+            if res["bounded"]:
+                sustained = res
+                break
+        stats = _lib.server_stats()
+    finally:
+        st.release(torch)
+    main["workload"] = (f"Poisson arrivals {rate:.0f}/s for {seconds} s (seed {seed:#x}), operand limbs "
+                        f"log-uniform in [2^10, 2^24], pinned host buffers, {slots} product slots, root tables "
+                        f"built before the stream, every product checked mod 2^61-1 and {main['gmp_checked']} "
+                        f"against GMP")
+    main["sustained"] = sustained
+    main["rate_search"] = tried
+    main["server"] = {"lanes": stats["lanes"], "split_downloads": stats["split_downloads"],
+                      "staged": stats["staged"], "failed": stats["failed"]}
+    return main
 
 
 def staged_leg(torch, h, _lib, bits: int = 26, limits=(4 << 30, 1 << 30), reps: int = 2) -> dict:
@@ -569,9 +705,8 @@ def main() -> int:
     if not args.no_serve:
         print("# serving ...", file=sys.stderr, flush=True)
         serving = serving_leg(torch, h, _lib)
-        print("# serving below saturation ...", file=sys.stderr, flush=True)
-        serving["below_saturation"] = serving_leg(torch, h, _lib, rate=250.0, seconds=2.0, seed=0x48475814,
-                                                  warm=False)
+        serving["pcie"] = pcie_probe(torch)
+        serving["frac_of_pcie_bound"] = round(serving["limbs_per_s"] / serving["pcie"]["bound_limbs_per_s"], 4)
 
     staged = None
     if not args.no_staged:
@@ -622,6 +757,12 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    # release the library and torch's cached pinned host memory while the HIP
+    # runtime is fully up, before the interpreter's and the runtime's own
+    # teardown
+    _lib.shutdown()
+    torch.cuda.synchronize()
+    torch._C._host_emptyCache()
     return 0
 
 

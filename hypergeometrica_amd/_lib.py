@@ -23,6 +23,7 @@ EXPORTS = (
     "hgx_ntt_reverse", "hgx_submit", "hgx_poll", "hgx_wait", "hgx_fill_splitmix64",
     "hgx_timing_enable", "hgx_timing_read", "hgx_add_device", "hgx_sub_device", "hgx_graphs_enable",
     "hgx_plan_set", "hgx_plan_get", "hgx_autotune", "hgx_split_enable", "hgx_staging_limit",
+    "hgx_server_stats",
 )
 
 
@@ -35,6 +36,13 @@ class HgxError(RuntimeError):
 class MulDesc(ctypes.Structure):
     _fields_ = [("a", ctypes.c_void_p), ("na", ctypes.c_size_t), ("b", ctypes.c_void_p),
                 ("nb", ctypes.c_size_t), ("out", ctypes.c_void_p)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
+                ("staged", ctypes.c_uint64), ("queued", ctypes.c_uint64), ("in_flight", ctypes.c_uint64),
+                ("uptime_s", ctypes.c_double), ("launch_s", ctypes.c_double), ("lanes", ctypes.c_int),
+                ("split_downloads", ctypes.c_int)]
 
 
 class KernelTime(ctypes.Structure):
@@ -87,6 +95,7 @@ def load() -> ctypes.CDLL:
         L.hgx_fill_splitmix64.argtypes = [_vp, _sz, _u64, ctypes.c_int, _vp]
         L.hgx_timing_enable.argtypes = [ctypes.c_int]
         L.hgx_timing_read.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int]
+        L.hgx_server_stats.argtypes = [ctypes.POINTER(ServerStats)]
         _lib = L
     return _lib
 
@@ -111,6 +120,11 @@ def _shutdown_at_exit() -> None:
         _lib.hgx_shutdown()
 
 
+def shutdown() -> None:
+    """hgx_shutdown now (the exit hook then has nothing left to do)."""
+    _shutdown_at_exit()
+
+
 def lib(device: int = 0, budget: int = 0) -> ctypes.CDLL:
     """The library, initialised on DEVICE (once per process)."""
     global _initialised, _exit_hook
@@ -128,6 +142,13 @@ def lib(device: int = 0, budget: int = 0) -> ctypes.CDLL:
 
 def timing_enable(on: bool = True) -> None:
     check(lib().hgx_timing_enable(1 if on else 0))
+
+
+def server_stats() -> dict:
+    """The request server's counters (hgx_server_stats) as a dict."""
+    st = ServerStats()
+    check(load().hgx_server_stats(ctypes.byref(st)))
+    return {name: getattr(st, name) for name, _ in ServerStats._fields_}
 
 
 def timing_read() -> dict:

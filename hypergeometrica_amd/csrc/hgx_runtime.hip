@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cmath>
 #include <cstdio>
@@ -258,7 +259,7 @@ int dev_alloc(void **p, size_t bytes) {
 
 void dev_free(void *p, size_t bytes) {
   if (!p) return;
-  hipFree(p);
+  (void)hipFree(p);
   g_ctx.allocated -= bytes;
 }
 
@@ -279,17 +280,21 @@ int ensure(Workspace &w, size_t bytes) {
 // ---- host transfers through pinned staging ----
 // A caller's limbs live in pageable memory (calloc or mmap, src/ram-vec.lisp:52-59,
 // src/disk-vec.lisp:34-35).  Large transfers go through kPinBufs pinned buffers
-// of kPinChunk bytes: worker threads copy chunk k + 1 between the caller's
-// memory and a staging buffer while the DMA engine moves chunk k.  Downloads
-// are deferred: a buffer's bytes reach the caller's memory when the buffer is
-// needed again or at pin_flush().  Every use holds g_ctx.mu.
+// of kPinChunk bytes of a PinStage: worker threads copy chunk k + 1 between the
+// caller's memory and a staging buffer while the DMA engine moves chunk k.
+// Downloads are deferred: a buffer's bytes reach the caller's memory when the
+// buffer is needed again or at pin_flush().  A PinStage has one user at a time:
+// the library's own (g_pin) is used under g_ctx.mu, the request server's
+// stager thread owns another.
 constexpr size_t kPinChunk = (size_t)32 << 20;
 constexpr int kPinBufs = 4;
 constexpr size_t kPinMin = (size_t)4 << 20;  // smaller transfers: one plain copy
 
 // memcpy split over a few worker threads; the workers are never joined (the
-// pool is leaked), so process exit does not wait for them
+// pool is leaked), so process exit does not wait for them.  Copies from
+// different threads take turns (USE).
 struct CopyPool {
+  std::mutex use;
   std::mutex mu;
   std::condition_variable cv, done;
   char *dst = nullptr;
@@ -317,6 +322,7 @@ struct CopyPool {
       std::memcpy(d, s, n);
       return;
     }
+    std::lock_guard<std::mutex> turn(use);
     std::unique_lock<std::mutex> lk(mu);
     dst = (char *)d;
     src = (const char *)s;
@@ -353,36 +359,39 @@ struct PinStage {
 };
 PinStage g_pin;
 
-int pin_init() {
-  if (g_pin.ready) return HGX_OK;
+int pin_init(PinStage &P) {
+  if (P.ready) return HGX_OK;
   for (int i = 0; i < kPinBufs; i++) {
-    HIPCHK(hipHostMalloc((void **)&g_pin.buf[i], kPinChunk, hipHostMallocDefault));
-    HIPCHK(hipEventCreateWithFlags(&g_pin.ev[i], hipEventDisableTiming));
+    HIPCHK(hipHostMalloc((void **)&P.buf[i], kPinChunk, hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&P.ev[i], hipEventDisableTiming));
   }
   (void)copy_pool();
-  g_pin.ready = true;
+  P.ready = true;
   return HGX_OK;
 }
 
 // wait for buffer i's transfer; a finished download is copied to its caller
-int pin_settle(int i) {
-  if (g_pin.state[i] == 0) return HGX_OK;
-  HIPCHK(hipEventSynchronize(g_pin.ev[i]));
-  if (g_pin.state[i] == 2) copy_pool()->copy(g_pin.out_dst[i], g_pin.buf[i], g_pin.out_len[i]);
-  g_pin.state[i] = 0;
+int pin_settle(PinStage &P, int i) {
+  if (P.state[i] == 0) return HGX_OK;
+  const hipError_t e = hipEventSynchronize(P.ev[i]);
+  const int was = P.state[i];
+  P.state[i] = 0;
+  if (e != hipSuccess) return fail(HGX_EDEVICE, std::string("staging transfer failed: ") + hipGetErrorString(e));
+  if (was == 2) copy_pool()->copy(P.out_dst[i], P.buf[i], P.out_len[i]);
   return HGX_OK;
 }
 
 // every deferred download has reached the caller's memory
-int pin_flush() {
-  if (!g_pin.ready) return HGX_OK;
+int pin_flush(PinStage &P) {
+  if (!P.ready) return HGX_OK;
   for (int k = 0; k < kPinBufs; k++) {
-    const int i = (g_pin.next + k) % kPinBufs;  // oldest first
-    int rc = pin_settle(i);
+    const int i = (P.next + k) % kPinBufs;  // oldest first
+    int rc = pin_settle(P, i);
     if (rc) return rc;
   }
   return HGX_OK;
 }
+int pin_flush() { return pin_flush(g_pin); }
 
 // host memory the HIP runtime already has pinned (hipHostMalloc): the DMA
 // engine reads and writes it directly
@@ -399,74 +408,78 @@ bool host_pinned(const void *p) {
 // the host memory on return; a pinned source is read by the DMA engine when
 // the stream reaches the copy, so the caller keeps it until the stream
 // synchronizes (every caller here does before it returns).
-int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
+int h2d(PinStage &P, void *dst, const void *src, size_t bytes, hipStream_t s) {
   if (bytes < kPinMin || host_pinned(src)) {
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
     return HGX_OK;
   }
-  int rc = pin_init();
+  int rc = pin_init(P);
   if (rc) return rc;
   for (size_t off = 0; off < bytes; off += kPinChunk) {
-    const int i = g_pin.next;
-    g_pin.next = (i + 1) % kPinBufs;
-    rc = pin_settle(i);
+    const int i = P.next;
+    P.next = (i + 1) % kPinBufs;
+    rc = pin_settle(P, i);
     if (rc) return rc;
     const size_t len = std::min(kPinChunk, bytes - off);
-    copy_pool()->copy(g_pin.buf[i], (const char *)src + off, len);
-    HIPCHK(hipMemcpyAsync((char *)dst + off, g_pin.buf[i], len, hipMemcpyHostToDevice, s));
-    HIPCHK(hipEventRecord(g_pin.ev[i], s));
-    g_pin.state[i] = 1;
+    copy_pool()->copy(P.buf[i], (const char *)src + off, len);
+    HIPCHK(hipMemcpyAsync((char *)dst + off, P.buf[i], len, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(P.ev[i], s));
+    P.state[i] = 1;
   }
   return HGX_OK;
 }
+int h2d(void *dst, const void *src, size_t bytes, hipStream_t s) { return h2d(g_pin, dst, src, bytes, s); }
 
 // device -> host on stream s, after the work queued before it; the bytes are
 // in the host memory after pin_flush() (or a later transfer reusing the
 // buffers)
-int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
+int d2h(PinStage &P, void *dst, const void *src, size_t bytes, hipStream_t s) {
   if (bytes < kPinMin || host_pinned(dst)) {
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
     return HGX_OK;
   }
-  int rc = pin_init();
+  int rc = pin_init(P);
   if (rc) return rc;
   for (size_t off = 0; off < bytes; off += kPinChunk) {
-    const int i = g_pin.next;
-    g_pin.next = (i + 1) % kPinBufs;
-    rc = pin_settle(i);
+    const int i = P.next;
+    P.next = (i + 1) % kPinBufs;
+    rc = pin_settle(P, i);
     if (rc) return rc;
     const size_t len = std::min(kPinChunk, bytes - off);
-    HIPCHK(hipMemcpyAsync(g_pin.buf[i], (const char *)src + off, len, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipEventRecord(g_pin.ev[i], s));
-    g_pin.state[i] = 2;
-    g_pin.out_dst[i] = (char *)dst + off;
-    g_pin.out_len[i] = len;
+    HIPCHK(hipMemcpyAsync(P.buf[i], (const char *)src + off, len, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(P.ev[i], s));
+    P.state[i] = 2;
+    P.out_dst[i] = (char *)dst + off;
+    P.out_len[i] = len;
   }
   return HGX_OK;
 }
+int d2h(void *dst, const void *src, size_t bytes, hipStream_t s) { return d2h(g_pin, dst, src, bytes, s); }
 
 // An API call that queues downloads flushes them before it returns success;
 // on any other way out the scope drops what is still pending, so no later
 // call writes into memory its caller may have released.
 struct PinScope {
+  PinStage &P;
+  explicit PinScope(PinStage &p = g_pin) : P(p) {}
   ~PinScope() {
-    if (!g_pin.ready) return;
+    if (!P.ready) return;
     for (int i = 0; i < kPinBufs; i++) {
-      if (g_pin.state[i] == 0) continue;
-      (void)hipEventSynchronize(g_pin.ev[i]);
-      g_pin.state[i] = 0;
+      if (P.state[i] == 0) continue;
+      (void)hipEventSynchronize(P.ev[i]);
+      P.state[i] = 0;
     }
   }
 };
 
-void pin_free() {
-  if (!g_pin.ready) return;
+void pin_free(PinStage &P) {
+  if (!P.ready) return;
   for (int i = 0; i < kPinBufs; i++) {
-    (void)hipEventSynchronize(g_pin.ev[i]);
-    (void)hipEventDestroy(g_pin.ev[i]);
-    (void)hipHostFree(g_pin.buf[i]);
+    (void)hipEventSynchronize(P.ev[i]);
+    (void)hipEventDestroy(P.ev[i]);
+    (void)hipHostFree(P.buf[i]);
   }
-  g_pin = PinStage{};
+  P = PinStage{};
 }
 
 // ---- plans ----
@@ -1097,24 +1110,63 @@ size_t plan_bytes_needed(int n) {
   return align_up(words * sizeof(u64)) + big;
 }
 
-// device bytes of one product computed whole from host memory
-size_t inhbm_bytes(size_t na, size_t nb, bool sqr) {
+// device bytes of the work memory of one product (its transform's, or for an
+// unbalanced product that mul_device_locked splits, the chunk products' two
+// images, job table and work memory and the scan scratch of the image sum),
+// root tables still to be built included
+size_t work_bytes(size_t na, size_t nb, bool sqr) {
+  Split sp;
+  if (!sqr && g_ctx.split_on && choose_split(na, nb, sp)) {
+    const SplitBytes sb = split_bytes(na, nb, sp);
+    return sb.main + sb.scan + plan_bytes_needed(sp.nc);
+  }
   const int n = log2_ceil((u64)hgx_product_length(na, nb));
-  return align_up(na * 8) + (sqr ? 0 : align_up(nb * 8)) + align_up((na + nb) * 8) + work_layout(n, sqr).total +
-         plan_bytes_needed(n);
+  return work_layout(n, sqr).total + plan_bytes_needed(n);
 }
 
-// device bytes of the staged product with M-limb chunks
-size_t staged_bytes(u64 M) {
-  const int n = log2_ceil(2 * M);
-  return 2 * align_up(M * 8) + align_up(2 * M * 8) + 2 * align_up((2 * M + 2) * 8) + work_layout(n, false).total +
-         align_up(2 * hgx_addsub_blocks(2 * M + 2)) + plan_bytes_needed(n);
+// device bytes of one product computed whole from host memory: operands,
+// product and work memory
+size_t inhbm_bytes(size_t na, size_t nb, bool sqr) {
+  return align_up(na * 8) + (sqr ? 0 : align_up(nb * 8)) + align_up((na + nb) * 8) + work_bytes(na, nb, sqr);
 }
+
+// Device memory of a staged product with M-limb chunks: two operand chunks,
+// the chunk product, two result windows, the work memory of one transform of
+// length 2M and the scan scratch of the window additions.
+struct StagedLayout {
+  int n;
+  size_t ca, cb, P, W0, W1, work, scan, total;
+};
+StagedLayout staged_layout(u64 M) {
+  StagedLayout l{};
+  l.n = log2_ceil(2 * M);
+  size_t at = 0;
+  l.ca = at;
+  at += align_up(M * 8);
+  l.cb = at;
+  at += align_up(M * 8);
+  l.P = at;
+  at += align_up(2 * M * 8);
+  l.W0 = at;
+  at += align_up((2 * M + 2) * 8);
+  l.W1 = at;
+  at += align_up((2 * M + 2) * 8);
+  l.work = at;
+  at += work_layout(l.n, false).total;
+  l.scan = at;
+  at += align_up(2 * hgx_addsub_blocks(2 * M + 2));
+  l.total = at;
+  return l;
+}
+
+// device bytes of the staged product with M-limb chunks, root tables included
+size_t staged_bytes(u64 M) { return staged_layout(M).total + plan_bytes_needed(log2_ceil(2 * M)); }
 
 // device memory a host-memory product may use: the budget not held by other
-// allocations (the workspaces it reuses count as free), capped by the limit
-size_t memory_for_product() {
-  const size_t reusable = g_ctx.ws.bytes + g_ctx.stage.bytes + g_ctx.scan.bytes;
+// allocations, capped by the staging limit.  With REUSE the library's
+// workspaces, which the product reuses, count as free.
+size_t memory_for_product(bool reuse = true) {
+  const size_t reusable = reuse ? g_ctx.ws.bytes + g_ctx.stage.bytes + g_ctx.scan.bytes : 0;
   const size_t held = g_ctx.allocated - reusable;
   size_t avail = g_ctx.budget > held ? g_ctx.budget - held : 0;
   const size_t limit = g_ctx.stage_limit.load();
@@ -1126,40 +1178,34 @@ bool needs_staging(size_t na, size_t nb, bool sqr) {
   return inhbm_bytes(na, nb, sqr) + kStageSlack > memory_for_product();
 }
 
-int mul_staged_locked(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, bool sqr) {
+// the largest power-of-two chunk (at most the longer operand rounded up)
+// whose staged working set fits AVAIL bytes; 0 when none does
+u64 staged_chunk(size_t na, size_t avail) {
+  u64 M = 1ull << log2_ceil(na);
+  while (M > kStageMinChunk && staged_bytes(M) + kStageSlack > avail) M >>= 1;
+  return staged_bytes(M) + kStageSlack > avail ? 0 : M;
+}
+
+// The staged product with M-limb chunks on stream S, through the pinned
+// staging PIN, in the device memory at BASE (staged_layout(M).total bytes).
+// g_ctx.mu is taken only around the kernel launches of each chunk product, so
+// a caller that does not hold it (the request server's stager thread) leaves
+// the library free between them.  Returns once the product is in OUT.
+int mul_staged_at(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, bool sqr, u64 M, hipStream_t s,
+                  PinStage &pin, char *base) {
   if (na < nb) {
     std::swap(a, b);
     std::swap(na, nb);
   }
-  const size_t avail = memory_for_product();
-  u64 M = 1ull << log2_ceil(na);
-  while (M > kStageMinChunk && staged_bytes(M) + kStageSlack > avail) M >>= 1;
-  if (staged_bytes(M) + kStageSlack > avail)
-    return fail(HGX_ENOMEM, "HBM budget too small even for a host-staged product: " + std::to_string(avail) +
-                                " bytes available, " + std::to_string(staged_bytes(M) + kStageSlack) + " needed");
+  const StagedLayout lay = staged_layout(M);
   const u64 ka = (na + M - 1) / M, kb = (nb + M - 1) / M, nout = na + nb, wlen = 2 * M + 2;
-  hipStream_t s = g_ctx.stream;
-  int rc = ws_acquire(s);
-  if (rc) return rc;
-  const size_t cb = align_up(M * 8), pb = align_up(2 * M * 8), wb = align_up(wlen * 8);
-  rc = ensure(g_ctx.stage, 2 * cb + pb + 2 * wb);
-  if (rc) return rc;
-  const u64 nblk = hgx_addsub_blocks(wlen);
-  rc = ensure(g_ctx.scan, 2 * nblk);
-  if (rc) return rc;
-  char *base = (char *)g_ctx.stage.ptr;
-  u64 *ca = (u64 *)base, *cbf = (u64 *)(base + cb), *P = (u64 *)(base + 2 * cb);
-  u64 *W[2] = {(u64 *)(base + 2 * cb + pb), (u64 *)(base + 2 * cb + pb + wb)};
-  uint8_t *agg = (uint8_t *)g_ctx.scan.ptr, *blkin = agg + nblk;
-  // chunk products are balanced: no split (its images would not fit the estimate)
-  struct SplitOff {
-    bool was = g_ctx.split_on;
-    SplitOff() { g_ctx.split_on = false; }
-    ~SplitOff() { g_ctx.split_on = was; }
-  } split_off;
+  u64 *ca = (u64 *)(base + lay.ca), *cbf = (u64 *)(base + lay.cb), *P = (u64 *)(base + lay.P);
+  u64 *W[2] = {(u64 *)(base + lay.W0), (u64 *)(base + lay.W1)};
+  uint8_t *agg = (uint8_t *)(base + lay.scan), *blkin = agg + hgx_addsub_blocks(wlen);
+  char *work = base + lay.work;
   u64 in_a = ~0ull, in_b = ~0ull;  // chunks now in the device buffers
-  int cur = 0;
-  PinScope pin;
+  int cur = 0, rc = HGX_OK;
+  PinScope scope(pin);
   for (u64 d = 0; d + 1 < ka + kb; d++) {
     u64 *Wc = W[cur], *Wp = W[cur ^ 1];
     bool first = true;
@@ -1170,39 +1216,58 @@ int mul_staged_locked(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out
       const u64 mi = std::min<u64>(M, na - i * M), mj = std::min<u64>(M, nb - j * M);
       const bool sq = sqr && i == j;
       if (in_a != i) {
-        rc = h2d(ca, a + i * M, mi * 8, s);
+        rc = h2d(pin, ca, a + i * M, mi * 8, s);
         if (rc) return rc;
         in_a = i;
       }
       if (!sq && in_b != j) {
-        rc = h2d(cbf, b + j * M, mj * 8, s);
+        rc = h2d(pin, cbf, b + j * M, mj * 8, s);
         if (rc) return rc;
         in_b = j;
       }
-      rc = mul_device_locked(ca, mi, sq ? ca : cbf, sq ? mi : mj, P, sq, s);
+      std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+      // the chunk product (balanced: never split) in this product's own work memory
+      const u64 mo = sq ? mi : mj;
+      const int n = log2_ceil(mi + mo);
+      JobRef jr{};
+      jr.one = make_job(ca, mi, sq ? ca : cbf, mo, P, n, sq, work);
+      rc = enqueue_group(jr, 1, n, sq, s);
       if (rc) return rc;
       for (int t = 0; t < ((sqr && !sq) ? 2 : 1); t++) {
         HGX_TIMED("stage_add", s);
         if (first) {  // W = (previous window from limb M) + P
-          hgx_launch_addsub(false, s, d ? Wp + M : nullptr, d ? M + 2 : 0, P, mi + (sq ? mi : mj), Wc, wlen, agg,
-                            blkin, nullptr);
+          hgx_launch_addsub(false, s, d ? Wp + M : nullptr, d ? M + 2 : 0, P, mi + mo, Wc, wlen, agg, blkin, nullptr);
           first = false;
         } else {
-          hgx_launch_addsub(false, s, Wc, wlen, P, mi + (sq ? mi : mj), Wc, wlen, agg, blkin, nullptr);
+          hgx_launch_addsub(false, s, Wc, wlen, P, mi + mo, Wc, wlen, agg, blkin, nullptr);
         }
       }
+      HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipGetLastError());
     const u64 at = d * M;
     const u64 cnt = (d + 2 == ka + kb) ? nout - at : M;
-    rc = d2h(out + at, Wc, cnt * 8, s);
+    rc = d2h(pin, out + at, Wc, cnt * 8, s);
     if (rc) return rc;
     cur ^= 1;
   }
-  rc = pin_flush();
+  rc = pin_flush(pin);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(s));
-  return ws_release(s);
+  return HGX_OK;
+}
+
+// A staged product for the library's synchronous calls (the caller holds
+// g_ctx.mu): the library stream, its pinned staging and its staging workspace.
+int mul_staged_locked(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, bool sqr) {
+  const size_t avail = memory_for_product();
+  const u64 M = staged_chunk(std::max(na, nb), avail);
+  if (!M)
+    return fail(HGX_ENOMEM, "HBM budget too small even for a host-staged product: " + std::to_string(avail) +
+                                " bytes available, " + std::to_string(staged_bytes(kStageMinChunk) + kStageSlack) +
+                                " needed");
+  int rc = ensure(g_ctx.stage, staged_layout(M).total);
+  if (rc) return rc;
+  return mul_staged_at(a, na, b, nb, out, sqr, M, g_ctx.stream, g_pin, (char *)g_ctx.stage.ptr);
 }
 
 // Replay the captured launch sequence for KEY on stream S, capturing it on
@@ -1349,12 +1414,19 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
 // hgx_submit queues a request and returns.  A server thread admits queued
 // requests in arrival order while their memory (operands, product, work
 // arrays) fits in a device arena carved from the HBM budget (a round takes at
-// most a 1/kServeStreams share of the arena), groups each admission round by
-// transform length into batched launches, and puts each round on the stream
-// with the least work in flight, so rounds overlap on the device.  A request
-// that arrives while others run joins the next round as soon as memory
-// frees up.  Completion is observed from events; the arena block is then
-// returned and the ticket completes.
+// most a 1/lanes share of the arena), groups each admission round by
+// transform length into batched launches, and puts each round on the lane
+// with the least work in flight, so rounds overlap on the device.  A lane is
+// one stream for a round's uploads and kernels and, with split downloads, a
+// second stream for its downloads, so that the next round's uploads on the
+// lane do not queue behind this round's downloads.  A request that arrives
+// while others run joins the next round as soon as memory frees up.
+// Requests larger than the arena go to a stager thread, which computes them
+// host-staged on a stream of its own and holds the library only while it
+// launches each chunk product, so rounds keep being admitted meanwhile.
+// Completion is observed from events; the arena block is then returned and
+// the ticket completes.  A ticket is released by hgx_wait, or by the hgx_poll
+// call that reports its completion.
 struct SrvReq {
   hgx_mul_desc d{};
   bool sqr = false;
@@ -1372,24 +1444,34 @@ struct SrvReq {
   size_t host_off = (size_t)-1, host_bytes = 0;
   std::vector<MulJob> table_spill;
   hipEvent_t done_ev = nullptr;
-  int stream = -1;  // server stream of its round
-  int state = 0;    // 0 queued, 1 in flight, 2 done, <0 failed
+  bool ev_ok = true;  // done_ev was recorded after the request's last work
+  int lane = -1;      // lane of its round
+  int state = 0;      // 0 queued, 1 in flight, 2 done, <0 failed
   std::string err;
 };
 
-// Each stream runs its rounds' uploads, kernels and downloads in order, so the
-// stream count bounds the requests in flight per stage; six measured fastest
-// (3.5-3.8e9 limbs/s on a saturated mixed-size stream, against 2.8-3.0e9
-// with four and 3.3-3.6e9 with eight).
-constexpr int kServeStreams = 6;
+// Six lanes measured fastest with one stream per lane (3.5-3.8e9 limbs/s on a
+// saturated mixed-size stream, against 2.8-3.0e9 with four and 3.3-3.6e9 with
+// eight).  HGX_SERVE_LANES (1..16) and HGX_SERVE_SPLIT_D2H (0/1) choose others.
+constexpr int kMaxLanes = 16;
+constexpr int kDefaultLanes = 6;
+constexpr int kDefaultSplitD2H = 0;
 constexpr size_t kServeTableBytes = 4 << 20;  // pinned pool of the server's host job tables
+
+struct Lane {
+  hipStream_t s = nullptr;  // uploads and kernels
+  hipStream_t d = nullptr;  // downloads (the same stream unless split)
+  hipEvent_t k = nullptr;   // end of the latest round's kernels on s, for split downloads
+  size_t pending = 0;       // arena bytes of the requests in flight on the lane
+};
 
 struct Server {
   bool started = false, stop = false;
-  std::thread th;
+  std::thread th, stager;
   std::mutex mu;
-  std::condition_variable cv;
-  std::deque<std::shared_ptr<SrvReq>> queue;
+  std::condition_variable cv;        // request state changes and arrivals
+  std::condition_variable stage_cv;  // requests for the stager
+  std::deque<std::shared_ptr<SrvReq>> queue, staged;
   std::vector<std::shared_ptr<SrvReq>> inflight;
   std::unordered_map<hgx_ticket, std::shared_ptr<SrvReq>> tickets;
   char *arena = nullptr;
@@ -1397,9 +1479,19 @@ struct Server {
   std::map<size_t, size_t> free_list;  // offset -> size
   char *tables = nullptr;               // pinned host pool of job tables
   std::map<size_t, size_t> table_free;  // offset -> size in it
-  hipStream_t streams[kServeStreams]{};
-  size_t pending[kServeStreams]{};  // arena bytes of the requests in flight on each stream
+  int nlanes = kDefaultLanes;
+  bool split_d2h = kDefaultSplitD2H;
+  Lane lanes[kMaxLanes];
+  hipStream_t stage_stream = nullptr;  // the stager's
+  PinStage stage_pin;                  // the stager's pinned staging
   std::vector<hipEvent_t> event_pool;
+  // statistics since the server started
+  std::chrono::steady_clock::time_point t0;
+  uint64_t completed = 0, failed = 0, rounds = 0, staged_done = 0;
+  double launch_s = 0;  // server thread time spent issuing rounds
+  // failure injection for tests: the FAIL_COPY-th copy the server issues is
+  // reported failed without being issued (HGX_SERVE_FAIL_COPY; 0: none)
+  uint64_t fail_copy = 0, copies = 0;
 };
 
 Server g_srv;
@@ -1463,6 +1555,7 @@ size_t request_bytes(const SrvReq &r) {
   return in + align_up((r.d.na + r.d.nb) * 8) + work_layout(r.n, r.sqr).total;
 }
 
+// a completion event from the pool (null when none can be created)
 hipEvent_t take_event() {
   if (!g_srv.event_pool.empty()) {
     hipEvent_t e = g_srv.event_pool.back();
@@ -1470,12 +1563,32 @@ hipEvent_t take_event() {
     return e;
   }
   hipEvent_t e = nullptr;
-  (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+  if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
   return e;
 }
 
-// Launch one admission round on stream s (called without g_srv.mu held).
-void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
+// A copy of the server thread; on failure R carries the error and false is
+// returned.
+bool srv_copy(SrvReq &r, void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  const uint64_t k = ++g_srv.copies;
+  const hipError_t e = (k == g_srv.fail_copy) ? hipErrorInvalidValue : hipMemcpyAsync(dst, src, bytes, kind, s);
+  if (e == hipSuccess) return true;
+  (void)hipGetLastError();
+  if (r.err.empty())
+    r.err = std::string(kind == hipMemcpyHostToDevice ? "upload" : "download") + " of a served request failed: " +
+            hipGetErrorString(e);
+  return false;
+}
+
+// Launch one admission round on lane LI (called without g_srv.mu held).
+// Every failure lands in the error of the requests it concerns; a request
+// with an error is not downloaded, and completes as failed.
+void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, int li) {
+  Lane &ln = g_srv.lanes[li];
+  const hipStream_t s = ln.s;
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   // unbalanced products (split into chunk products) after the grouped ones
   std::vector<std::shared_ptr<SrvReq>> splits;
@@ -1496,11 +1609,12 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
     u64 *db = r.sqr ? da : (u64 *)(base + align_up(r.d.na * 8));
     u64 *dout = (u64 *)(base + align_up(r.d.na * 8) + (r.sqr ? 0 : align_up(r.d.nb * 8)));
     char *work = (char *)dout + align_up((r.d.na + r.d.nb) * 8);
-    (void)hipMemcpyAsync(da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s);
-    if (!r.sqr) (void)hipMemcpyAsync(db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
+    if (srv_copy(r, da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s) && !r.sqr)
+      srv_copy(r, db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
     jobs[i] = make_job(da, r.d.na, db, r.d.nb, dout, r.n, r.sqr, work);
   }
-  // one launch sequence per (length, square) group
+  // one launch sequence per (length, square) group; every job's memory lies
+  // in the arena, so a request whose upload failed only computes garbage
   size_t g = 0;
   while (g < batch.size()) {
     size_t h = g;
@@ -1518,10 +1632,14 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
       owner.table_bytes = (h - g) * sizeof(MulJob);
       std::copy(jobs.begin() + g, jobs.begin() + h, owner.table_host);
       MulJob *dtab = (MulJob *)(g_srv.arena + toff);
-      (void)hipMemcpyAsync(dtab, owner.table_host, owner.table_bytes, hipMemcpyHostToDevice, s);
-      JobRef jr{};
-      jr.table = dtab;
-      rc = enqueue_group(jr, (unsigned)(h - g), batch[g]->n, batch[g]->sqr, s);
+      SrvReq probe;
+      if (!srv_copy(probe, dtab, owner.table_host, owner.table_bytes, hipMemcpyHostToDevice, s)) {
+        rc = fail(HGX_EDEVICE, probe.err);
+      } else {
+        JobRef jr{};
+        jr.table = dtab;
+        rc = enqueue_group(jr, (unsigned)(h - g), batch[g]->n, batch[g]->sqr, s);
+      }
     } else {
       for (size_t i = g; i < h && rc == HGX_OK; i++) {
         JobRef jr{};
@@ -1530,12 +1648,13 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
       }
     }
     if (rc) {
-      for (size_t i = g; i < h; i++) batch[i]->err = g_err;
+      for (size_t i = g; i < h; i++)
+        if (batch[i]->err.empty()) batch[i]->err = g_err;
     }
     g = h;
   }
   // the split products: operands in, chunk products and the image sum in the
-  // request's own arena block, so split products on different streams overlap
+  // request's own arena block, so split products on different lanes overlap
   for (size_t i = 0; i < splits.size(); i++) {
     SrvReq &r = *splits[i];
     char *base = g_srv.arena + r.off;
@@ -1544,101 +1663,141 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, hipStream_t s) {
     u64 *dout = (u64 *)(base + align_up(r.d.na * 8) + align_up(r.d.nb * 8));
     char *work = (char *)dout + align_up((r.d.na + r.d.nb) * 8);
     const SplitBytes sb = split_bytes(r.d.na, r.d.nb, r.sp);
-    (void)hipMemcpyAsync(da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
+    if (srv_copy(r, da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s))
+      srv_copy(r, db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
     MulJob *tab;
     {
       std::lock_guard<std::mutex> lk2(g_srv.mu);
       tab = host_table(r, r.sp.k);
     }
     int rc = split_product_at(da, r.d.na, db, r.d.nb, dout, r.sp, s, work, (uint8_t *)(work + sb.main), tab);
-    if (rc) r.err = g_err;
+    if (rc && r.err.empty()) r.err = g_err;
     batch.push_back(splits[i]);
   }
-  // products out, completion events (the product follows the operands in
-  // every request's arena block)
+  // products out and completion events (the product follows the operands in
+  // every request's arena block), on the download stream after the kernels
+  hipStream_t ds = s;
+  if (ln.d != ln.s) {
+    hipError_t e = hipEventRecord(ln.k, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ln.d, ln.k, 0);
+    if (e == hipSuccess)
+      ds = ln.d;
+    else
+      (void)hipGetLastError();  // downloads stay on the upload stream, in order
+  }
   for (size_t i = 0; i < batch.size(); i++) {
     SrvReq &r = *batch[i];
     const u64 *dout = (const u64 *)(g_srv.arena + r.off + align_up(r.d.na * 8) + (r.sqr ? 0 : align_up(r.d.nb * 8)));
-    (void)hipMemcpyAsync(r.d.out, dout, (r.d.na + r.d.nb) * 8, hipMemcpyDeviceToHost, s);
-    (void)hipEventRecord(r.done_ev, s);
+    if (r.err.empty()) srv_copy(r, r.d.out, dout, (r.d.na + r.d.nb) * 8, hipMemcpyDeviceToHost, ds);
+    const hipError_t e = hipEventRecord(r.done_ev, ds);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      r.ev_ok = false;
+      if (r.err.empty()) r.err = std::string("recording the completion event failed: ") + hipGetErrorString(e);
+    }
   }
+}
+
+// Return a finished request's memory and complete its ticket (g_srv.mu held).
+void finish_locked(SrvReq &r) {
+  arena_free(r.off, r.bytes);
+  if (r.table_off != (size_t)-1) arena_free(r.table_off, r.table_bytes);
+  if (r.host_off != (size_t)-1) range_free(g_srv.table_free, r.host_off, r.host_bytes);
+  g_srv.lanes[r.lane].pending -= r.bytes;
+  if (r.done_ev) g_srv.event_pool.push_back(r.done_ev);
+  r.done_ev = nullptr;
+  r.state = r.err.empty() ? 2 : HGX_EDEVICE;
+  (r.state == 2 ? g_srv.completed : g_srv.failed)++;
+}
+
+// Retire the requests whose work has finished (g_srv.mu held).  A request
+// whose completion event was not recorded or reports an error waits for its
+// lane's streams before its memory is returned, and fails.
+void retire_locked() {
+  bool any = false;
+  for (size_t i = 0; i < g_srv.inflight.size();) {
+    SrvReq &r = *g_srv.inflight[i];
+    const hipError_t q = r.ev_ok ? hipEventQuery(r.done_ev) : hipErrorInvalidHandle;
+    if (q == hipErrorNotReady) {
+      i++;
+      continue;
+    }
+    if (q != hipSuccess) {
+      (void)hipGetLastError();
+      if (r.err.empty()) r.err = std::string("completion event failed: ") + hipGetErrorString(q);
+      const Lane &ln = g_srv.lanes[r.lane];
+      (void)hipStreamSynchronize(ln.s);
+      if (ln.d != ln.s) (void)hipStreamSynchronize(ln.d);
+      (void)hipGetLastError();
+    }
+    finish_locked(r);
+    g_srv.inflight[i] = g_srv.inflight.back();
+    g_srv.inflight.pop_back();
+    any = true;
+  }
+  if (any) g_srv.cv.notify_all();
 }
 
 void server_loop() {
   std::unique_lock<std::mutex> lk(g_srv.mu);
   while (!g_srv.stop) {
-    // retire finished requests
-    for (size_t i = 0; i < g_srv.inflight.size();) {
-      auto &r = g_srv.inflight[i];
-      if (hipEventQuery(r->done_ev) == hipSuccess) {
-        arena_free(r->off, r->bytes);
-        if (r->table_off != (size_t)-1) arena_free(r->table_off, r->table_bytes);
-        if (r->host_off != (size_t)-1) range_free(g_srv.table_free, r->host_off, r->host_bytes);
-        g_srv.pending[r->stream] -= r->bytes;
-        g_srv.event_pool.push_back(r->done_ev);
-        r->done_ev = nullptr;
-        r->state = r->err.empty() ? 2 : HGX_EDEVICE;
-        g_srv.inflight[i] = g_srv.inflight.back();
-        g_srv.inflight.pop_back();
-        g_srv.cv.notify_all();
-      } else {
-        i++;
-      }
-    }
-    // admit in arrival order while memory lasts, at most a 1/kServeStreams
-    // share of the arena per round: under load several rounds are in flight on the
-    // streams, so one round's uploads and downloads overlap another's kernels
+    retire_locked();
+    // admit in arrival order while memory lasts, at most a 1/lanes share of
+    // the arena per round: under load several rounds are in flight on the
+    // lanes, so one round's uploads and downloads overlap another's kernels
     std::vector<std::shared_ptr<SrvReq>> round;
     size_t round_bytes = 0;
+    const size_t share = g_srv.arena_bytes / g_srv.nlanes;
     while (!g_srv.queue.empty()) {
       auto r = g_srv.queue.front();
-      if (!round.empty() && round_bytes + r->bytes > g_srv.arena_bytes / kServeStreams) break;
       const size_t limit = g_ctx.stage_limit.load();
       if (r->bytes > g_srv.arena_bytes || (limit && r->bytes > limit)) {
-        // larger than the arena: a host-staged product, here in the server
-        // thread (rounds already in flight keep running on their streams)
+        // larger than the arena: host-staged by the stager thread while the
+        // rounds go on
         g_srv.queue.pop_front();
         r->state = 1;
-        lk.unlock();
-        int rc;
-        std::string err;
-        {
-          std::lock_guard<std::recursive_mutex> lk2(g_ctx.mu);
-          rc = mul_staged_locked(r->d.a, r->d.na, r->d.b, r->d.nb, r->d.out, r->sqr);
-          if (rc) err = g_err;
-        }
-        lk.lock();
-        r->err = err;
-        r->state = rc ? rc : 2;
+        g_srv.staged.push_back(r);
+        g_srv.stage_cv.notify_one();
+        continue;
+      }
+      if (!round.empty() && round_bytes + r->bytes > share) break;
+      const size_t off = arena_alloc(r->bytes);
+      if (off == (size_t)-1) break;
+      hipEvent_t ev = take_event();
+      g_srv.queue.pop_front();
+      if (!ev) {
+        arena_free(off, r->bytes);
+        r->err = "no completion event could be created";
+        r->state = HGX_EDEVICE;
+        g_srv.failed++;
         g_srv.cv.notify_all();
         continue;
       }
-      const size_t off = arena_alloc(r->bytes);
-      if (off == (size_t)-1) break;
       r->off = off;
       r->state = 1;
-      r->done_ev = take_event();
+      r->done_ev = ev;
       round.push_back(r);
       round_bytes += r->bytes;
-      g_srv.queue.pop_front();
     }
     if (!round.empty()) {
-      // the stream with the least work in flight (request sizes span four
-      // orders of magnitude: rotating through the streams would queue small
-      // requests behind a large one while other streams idle)
-      int si = 0;
-      for (int i = 1; i < kServeStreams; i++)
-        if (g_srv.pending[i] < g_srv.pending[si]) si = i;
+      // the lane with the least work in flight (request sizes span four
+      // orders of magnitude: rotating through the lanes would queue small
+      // requests behind a large one while other lanes idle)
+      int li = 0;
+      for (int i = 1; i < g_srv.nlanes; i++)
+        if (g_srv.lanes[i].pending < g_srv.lanes[li].pending) li = i;
       for (auto &r : round) {
-        r->stream = si;
-        g_srv.pending[si] += r->bytes;
+        r->lane = li;
+        g_srv.lanes[li].pending += r->bytes;
         g_srv.inflight.push_back(r);
       }
-      hipStream_t s = g_srv.streams[si];
       lk.unlock();
-      launch_round(round, s);
+      const auto t = std::chrono::steady_clock::now();
+      launch_round(round, li);
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
       lk.lock();
+      g_srv.launch_s += dt;
+      g_srv.rounds++;
       continue;
     }
     if (g_srv.inflight.empty())
@@ -1648,19 +1807,99 @@ void server_loop() {
   }
 }
 
+// A request larger than the arena, host-staged on the stager's stream in
+// device memory of its own (allocated for the request, freed after it).
+int run_staged_request(const SrvReq &r) {
+  char *base = nullptr;
+  size_t bytes = 0;
+  u64 M = 0;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+    const size_t avail = memory_for_product(false);
+    M = staged_chunk(std::max(r.d.na, r.d.nb), avail);
+    if (!M)
+      return fail(HGX_ENOMEM, "HBM budget too small for a host-staged served product: " + std::to_string(avail) +
+                                  " bytes available");
+    Plan *plan = nullptr;
+    int rc = build_plan(staged_layout(M).n, &plan);  // root tables before the work memory
+    if (rc) return rc;
+    bytes = staged_layout(M).total;
+    rc = dev_alloc((void **)&base, bytes);
+    if (rc) return rc;
+  }
+  const int rc = mul_staged_at(r.d.a, r.d.na, r.d.b, r.d.nb, r.d.out, r.sqr, M, g_srv.stage_stream, g_srv.stage_pin,
+                               base);
+  (void)hipStreamSynchronize(g_srv.stage_stream);  // also after a failure: nothing may still use BASE
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  dev_free(base, bytes);
+  return rc;
+}
+
+void stager_loop() {
+  std::unique_lock<std::mutex> lk(g_srv.mu);
+  for (;;) {
+    g_srv.stage_cv.wait(lk, [] { return g_srv.stop || !g_srv.staged.empty(); });
+    if (g_srv.stop) return;
+    auto r = g_srv.staged.front();
+    g_srv.staged.pop_front();
+    lk.unlock();
+    const int rc = run_staged_request(*r);
+    const std::string err = rc ? g_err : std::string();
+    lk.lock();
+    r->err = err;
+    r->state = rc ? rc : 2;
+    (rc ? g_srv.failed : g_srv.staged_done)++;
+    if (!rc) g_srv.completed++;
+    g_srv.cv.notify_all();
+  }
+}
+
 void server_stop(bool at_exit = false);
 // At process exit the HIP runtime (and any tool loaded into the process) may
-// already be finalized, so only the server thread is stopped and joined; the
-// device memory, streams and events go with the process.
+// already be finalized, so only the server threads are stopped and joined;
+// the device memory, streams and events go with the process.
 void server_stop_at_exit() { server_stop(true); }
+
+// streams, events and memory of the server (HIP calls; not at process exit)
+void server_release() {
+  for (int i = 0; i < kMaxLanes; i++) {
+    Lane &ln = g_srv.lanes[i];
+    if (ln.d && ln.d != ln.s) (void)hipStreamDestroy(ln.d);
+    if (ln.s) (void)hipStreamDestroy(ln.s);
+    if (ln.k) (void)hipEventDestroy(ln.k);
+    ln = Lane{};
+  }
+  if (g_srv.stage_stream) (void)hipStreamDestroy(g_srv.stage_stream);
+  g_srv.stage_stream = nullptr;
+  pin_free(g_srv.stage_pin);
+  for (auto e : g_srv.event_pool) (void)hipEventDestroy(e);
+  g_srv.event_pool.clear();
+  if (g_srv.arena) dev_free(g_srv.arena, g_srv.arena_bytes);
+  g_srv.arena = nullptr;
+  g_srv.arena_bytes = 0;
+  g_srv.free_list.clear();
+  if (g_srv.tables) (void)hipHostFree(g_srv.tables);
+  g_srv.tables = nullptr;
+  g_srv.table_free.clear();
+}
+
+long env_long(const char *name, long dflt) {
+  const char *e = getenv(name);
+  return (e && *e) ? strtol(e, nullptr, 10) : dflt;
+}
 
 int server_start_locked() {
   if (g_srv.started) return HGX_OK;
   static bool registered = false;
   if (!registered) {
-    std::atexit(server_stop_at_exit);  // join the server thread before static destruction
+    std::atexit(server_stop_at_exit);  // join the server threads before static destruction
     registered = true;
   }
+  const long lanes = env_long("HGX_SERVE_LANES", kDefaultLanes);
+  if (lanes < 1 || lanes > kMaxLanes) return fail(HGX_EINVAL, "HGX_SERVE_LANES must be 1..16");
+  g_srv.nlanes = (int)lanes;
+  g_srv.split_d2h = env_long("HGX_SERVE_SPLIT_D2H", kDefaultSplitD2H) != 0;
+  g_srv.fail_copy = (uint64_t)std::max(0L, env_long("HGX_SERVE_FAIL_COPY", 0));
   size_t arena = 0;
   if (const char *e = getenv("HGX_SERVE_ARENA_BYTES")) arena = strtoull(e, nullptr, 10);
   if (!arena) arena = (g_ctx.budget - g_ctx.allocated) / 2;
@@ -1670,48 +1909,86 @@ int server_start_locked() {
   g_srv.arena_bytes = arena;
   g_srv.free_list.clear();
   g_srv.free_list[0] = arena;
-  for (auto &b : g_srv.pending) b = 0;
   g_srv.table_free.clear();
-  if (hipHostMalloc((void **)&g_srv.tables, kServeTableBytes, hipHostMallocDefault) == hipSuccess)
+  if (hipHostMalloc((void **)&g_srv.tables, kServeTableBytes, hipHostMallocDefault) == hipSuccess) {
     g_srv.table_free[0] = kServeTableBytes;
-  else {
+  } else {
     g_srv.tables = nullptr;  // job tables then come from pageable memory
     (void)hipGetLastError();
   }
-  for (int i = 0; i < kServeStreams; i++) HIPCHK(hipStreamCreateWithFlags(&g_srv.streams[i], hipStreamNonBlocking));
-  g_srv.stop = false;
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < g_srv.nlanes && e == hipSuccess; i++) {
+    Lane &ln = g_srv.lanes[i];
+    e = hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking);
+    ln.d = ln.s;
+    if (e == hipSuccess && g_srv.split_d2h) {
+      e = hipStreamCreateWithFlags(&ln.d, hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&ln.k, hipEventDisableTiming);
+      if (e != hipSuccess) ln.d = ln.s;
+    }
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&g_srv.stage_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    server_release();
+    return fail(HGX_EDEVICE, std::string("creating the server's streams failed: ") + hipGetErrorString(e));
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_srv.mu);
+    g_srv.stop = false;
+    g_srv.completed = g_srv.failed = g_srv.rounds = g_srv.staged_done = g_srv.copies = 0;
+    g_srv.launch_s = 0;
+    g_srv.t0 = std::chrono::steady_clock::now();
+    g_srv.started = true;
+  }
   g_srv.th = std::thread(server_loop);
-  g_srv.started = true;
+  g_srv.stager = std::thread(stager_loop);
   return HGX_OK;
 }
 
+// Stop the server.  Rounds already launched finish and their requests
+// complete normally; requests not yet launched fail with HGX_ENOTINIT, so no
+// hgx_wait waits forever.
 void server_stop(bool at_exit) {
-  if (!g_srv.started) return;
   {
     std::lock_guard<std::mutex> lk(g_srv.mu);
+    if (!g_srv.started) return;
     g_srv.stop = true;
   }
   g_srv.cv.notify_all();
-  g_srv.th.join();
+  g_srv.stage_cv.notify_all();
+  if (g_srv.th.joinable()) g_srv.th.join();
+  if (g_srv.stager.joinable()) g_srv.stager.join();
   if (at_exit) {
+    std::lock_guard<std::mutex> lk(g_srv.mu);
     g_srv.started = false;
     return;
   }
-  for (int i = 0; i < kServeStreams; i++) {
-    (void)hipStreamSynchronize(g_srv.streams[i]);
-    (void)hipStreamDestroy(g_srv.streams[i]);
+  for (int i = 0; i < g_srv.nlanes; i++) {
+    (void)hipStreamSynchronize(g_srv.lanes[i].s);
+    if (g_srv.lanes[i].d != g_srv.lanes[i].s) (void)hipStreamSynchronize(g_srv.lanes[i].d);
   }
-  for (auto e : g_srv.event_pool) (void)hipEventDestroy(e);
-  g_srv.event_pool.clear();
-  dev_free(g_srv.arena, g_srv.arena_bytes);
-  g_srv.arena = nullptr;
-  g_srv.arena_bytes = 0;
-  if (g_srv.tables) (void)hipHostFree(g_srv.tables);
-  g_srv.tables = nullptr;
-  g_srv.table_free.clear();
-  g_srv.queue.clear();
-  g_srv.inflight.clear();
-  g_srv.started = false;
+  (void)hipGetLastError();
+  {
+    std::lock_guard<std::mutex> lk(g_srv.mu);
+    retire_locked();
+    for (auto &r : g_srv.inflight) {  // an event that never completes
+      if (r->err.empty()) r->err = "the request's completion was never observed";
+      finish_locked(*r);
+    }
+    g_srv.inflight.clear();
+    for (auto *q : {&g_srv.queue, &g_srv.staged}) {
+      for (auto &r : *q) {
+        r->err = "the request server stopped (hgx_shutdown) before the request ran";
+        r->state = HGX_ENOTINIT;
+        g_srv.failed++;
+      }
+      q->clear();
+    }
+    g_srv.started = false;
+    g_srv.cv.notify_all();
+  }
+  server_release();
 }
 
 // ---- pass-structure choices and the autotuner ----
@@ -1842,7 +2119,7 @@ void hgx_shutdown(void) {
   server_stop();
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   if (!g_ctx.ready) return;
-  hipStreamSynchronize(g_ctx.stream);
+  (void)hipStreamSynchronize(g_ctx.stream);
   timing_drain();
   g_timing = false;
   g_kstats.clear();
@@ -1865,12 +2142,12 @@ void hgx_shutdown(void) {
   if (g_ctx.jobs_host) (void)hipHostFree(g_ctx.jobs_host);
   g_ctx.jobs_host = nullptr;
   g_ctx.jobs_host_cap = 0;
-  pin_free();
+  pin_free(g_pin);
   (void)hipEventDestroy(g_ctx.jobs_copied);
   dev_free(g_ctx.tables, 3 * 2 * 2048 * 2 * sizeof(u64));
   g_ctx.tables = nullptr;
-  hipEventDestroy(g_ctx.ws_done);
-  hipStreamDestroy(g_ctx.stream);
+  (void)hipEventDestroy(g_ctx.ws_done);
+  (void)hipStreamDestroy(g_ctx.stream);
   g_ctx.ready = false;
 }
 
@@ -1967,9 +2244,10 @@ int hgx_mul_batch(const hgx_mul_desc *descs, size_t count) {
       return fail(HGX_ETOOBIG, "transform length above 2^32 in descriptor " + std::to_string(i));
   }
   // products too large for the device memory are staged one by one; the
-  // others are packed into one device buffer, run as a device batch, unpacked
+  // others are packed into device buffers, each pack small enough that its
+  // largest product's work memory still fits beside it, and each pack runs as
+  // a device batch (which chunks the work memory by the budget itself)
   std::vector<size_t> rest;
-  size_t in_words = 0, out_words = 0;
   for (size_t i = 0; i < count; i++) {
     const hgx_mul_desc &d = descs[i];
     const bool sqr = (d.a == d.b && d.na == d.nb);
@@ -1979,52 +2257,71 @@ int hgx_mul_batch(const hgx_mul_desc *descs, size_t count) {
       continue;
     }
     rest.push_back(i);
-    in_words += d.na + (sqr ? 0 : d.nb);
-    out_words += d.na + d.nb;
   }
   if (rest.empty()) return HGX_OK;
   hipStream_t s = g_ctx.stream;
   rc = ws_acquire(s);
   if (rc) return rc;
-  rc = ensure(g_ctx.pack, align_up((in_words + out_words) * 8));
-  if (rc) return rc;
-  std::vector<u64> hin(in_words);
-  std::vector<hgx_mul_desc> dd(rest.size());
-  u64 *din = (u64 *)g_ctx.pack.ptr, *dout = din + in_words;
-  size_t at = 0, ot = 0;
-  for (size_t k = 0; k < rest.size(); k++) {
-    const hgx_mul_desc &d = descs[rest[k]];
-    const bool sqr = (d.a == d.b && d.na == d.nb);
-    std::memcpy(&hin[at], d.a, d.na * 8);
-    dd[k].a = din + at;
-    dd[k].na = d.na;
-    at += d.na;
-    if (sqr) {
-      dd[k].b = dd[k].a;
-    } else {
-      std::memcpy(&hin[at], d.b, d.nb * 8);
-      dd[k].b = din + at;
-      at += d.nb;
+  for (size_t k0 = 0; k0 < rest.size();) {
+    const size_t reusable = g_ctx.ws.bytes + g_ctx.pack.bytes + g_ctx.jobs_dev.bytes + g_ctx.scan.bytes;
+    const size_t held = g_ctx.allocated - reusable;
+    const size_t avail = g_ctx.budget > held ? g_ctx.budget - held : 0;
+    size_t k1 = k0, in_words = 0, out_words = 0, max_work = 0;
+    while (k1 < rest.size()) {
+      const hgx_mul_desc &d = descs[rest[k1]];
+      const bool sqr = (d.a == d.b && d.na == d.nb);
+      const size_t iw = d.na + (sqr ? 0 : d.nb), ow = d.na + d.nb;
+      const size_t w = std::max(max_work, work_bytes(d.na, d.nb, sqr));
+      const size_t need = align_up((in_words + iw + out_words + ow) * 8) + w +
+                          align_up((k1 - k0 + 1) * sizeof(MulJob)) + kStageSlack;
+      if (k1 > k0 && need > avail) break;
+      in_words += iw;
+      out_words += ow;
+      max_work = w;
+      k1++;
     }
-    dd[k].nb = d.nb;
-    dd[k].out = dout + ot;
-    ot += d.na + d.nb;
-  }
-  PinScope pin;
-  rc = h2d(din, hin.data(), in_words * 8, s);
-  if (rc) return rc;
-  rc = batch_device_locked(dd.data(), dd.size(), s);
-  if (rc) return rc;
-  std::vector<u64> hout(out_words);
-  rc = d2h(hout.data(), dout, out_words * 8, s);
-  if (!rc) rc = pin_flush();
-  if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(s));
-  ot = 0;
-  for (size_t k = 0; k < rest.size(); k++) {
-    const hgx_mul_desc &d = descs[rest[k]];
-    std::memcpy(d.out, &hout[ot], (d.na + d.nb) * 8);
-    ot += d.na + d.nb;
+    rc = ensure(g_ctx.pack, align_up((in_words + out_words) * 8));
+    if (rc) return rc;
+    std::vector<u64> hin(in_words);
+    std::vector<hgx_mul_desc> dd(k1 - k0);
+    u64 *din = (u64 *)g_ctx.pack.ptr, *dout = din + in_words;
+    size_t at = 0, ot = 0;
+    for (size_t k = k0; k < k1; k++) {
+      const hgx_mul_desc &d = descs[rest[k]];
+      const bool sqr = (d.a == d.b && d.na == d.nb);
+      hgx_mul_desc &e = dd[k - k0];
+      std::memcpy(&hin[at], d.a, d.na * 8);
+      e.a = din + at;
+      e.na = d.na;
+      at += d.na;
+      if (sqr) {
+        e.b = e.a;
+      } else {
+        std::memcpy(&hin[at], d.b, d.nb * 8);
+        e.b = din + at;
+        at += d.nb;
+      }
+      e.nb = d.nb;
+      e.out = dout + ot;
+      ot += d.na + d.nb;
+    }
+    PinScope pin;
+    rc = h2d(din, hin.data(), in_words * 8, s);
+    if (rc) return rc;
+    rc = batch_device_locked(dd.data(), dd.size(), s);
+    if (rc) return rc;
+    std::vector<u64> hout(out_words);
+    rc = d2h(hout.data(), dout, out_words * 8, s);
+    if (!rc) rc = pin_flush();
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    ot = 0;
+    for (size_t k = k0; k < k1; k++) {
+      const hgx_mul_desc &d = descs[rest[k]];
+      std::memcpy(d.out, &hout[ot], (d.na + d.nb) * 8);
+      ot += d.na + d.nb;
+    }
+    k0 = k1;
   }
   return HGX_OK;
 }
@@ -2127,24 +2424,25 @@ int hgx_submit(const hgx_mul_desc *desc, hgx_ticket *ticket) {
   if (!desc || !ticket) return fail(HGX_EINVAL, "null descriptor or ticket");
   const hgx_mul_desc &d = *desc;
   if (!d.a || !d.b || !d.out || d.na == 0 || d.nb == 0) return fail(HGX_EINVAL, "null operand or zero length");
+  auto r = std::make_shared<SrvReq>();
+  r->d = d;
+  r->sqr = (d.a == d.b && d.na == d.nb);
+  r->n = log2_ceil((u64)hgx_product_length(d.na, d.nb));
+  if (r->n > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32");
   {
     std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
     int rc = check_ready();
     if (rc) return rc;
     rc = server_start_locked();
     if (rc) return rc;
+    r->split = !r->sqr && g_ctx.split_on && choose_split(d.na, d.nb, r->sp);
   }
-  auto r = std::make_shared<SrvReq>();
-  r->d = d;
-  r->sqr = (d.a == d.b && d.na == d.nb);
-  r->n = log2_ceil((u64)hgx_product_length(d.na, d.nb));
-  if (r->n > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32");
-  r->split = !r->sqr && g_ctx.split_on && choose_split(d.na, d.nb, r->sp);
   r->bytes = request_bytes(*r);
   static std::atomic<hgx_ticket> next_ticket{1};
   const hgx_ticket t = next_ticket++;
   {
     std::lock_guard<std::mutex> lk(g_srv.mu);
+    if (!g_srv.started || g_srv.stop) return fail(HGX_ENOTINIT, "the request server is stopping (hgx_shutdown)");
     g_srv.tickets[t] = r;
     g_srv.queue.push_back(r);
   }
@@ -2153,13 +2451,21 @@ int hgx_submit(const hgx_mul_desc *desc, hgx_ticket *ticket) {
   return HGX_OK;
 }
 
+// 1 (done) or the request's error releases the ticket
 int hgx_poll(hgx_ticket ticket) {
   std::lock_guard<std::mutex> lk(g_srv.mu);
   auto it = g_srv.tickets.find(ticket);
   if (it == g_srv.tickets.end()) return fail(HGX_EINVAL, "unknown ticket");
   const int st = it->second->state;
-  if (st == 2) return 1;
-  if (st < 0) return fail(st, it->second->err);
+  if (st == 2) {
+    g_srv.tickets.erase(it);
+    return 1;
+  }
+  if (st < 0) {
+    const std::string err = it->second->err;
+    g_srv.tickets.erase(it);
+    return fail(st, err);
+  }
   return 0;
 }
 
@@ -2171,6 +2477,24 @@ int hgx_wait(hgx_ticket ticket) {
   g_srv.cv.wait(lk, [&] { return r->state == 2 || r->state < 0; });
   g_srv.tickets.erase(ticket);
   if (r->state < 0) return fail(r->state, r->err);
+  return HGX_OK;
+}
+
+int hgx_server_stats(hgx_server_counters *out) {
+  if (!out) return fail(HGX_EINVAL, "null output");
+  std::lock_guard<std::mutex> lk(g_srv.mu);
+  std::memset(out, 0, sizeof *out);
+  if (!g_srv.started) return HGX_OK;
+  out->completed = g_srv.completed;
+  out->failed = g_srv.failed;
+  out->rounds = g_srv.rounds;
+  out->staged = g_srv.staged_done;
+  out->queued = g_srv.queue.size() + g_srv.staged.size();
+  out->in_flight = g_srv.inflight.size();
+  out->uptime_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - g_srv.t0).count();
+  out->launch_s = g_srv.launch_s;
+  out->lanes = g_srv.nlanes;
+  out->split_downloads = g_srv.split_d2h ? 1 : 0;
   return HGX_OK;
 }
 

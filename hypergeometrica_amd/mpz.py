@@ -120,9 +120,12 @@ def mul_batch(pairs) -> list[np.ndarray]:
 
 class Ticket:
     """A queued product on the library's request server (hgx_submit).
-    Holds the operand and output arrays until the product completes."""
+    Holds the operand and output arrays until the product completes: a
+    ticket dropped before completion waits for it first, so that the server
+    never writes into freed memory."""
 
     def __init__(self, a: np.ndarray, b: np.ndarray, sqr: bool = False):
+        self._done = True  # nothing to wait for unless the submit succeeds
         self.a = _u64(a)
         self.b = self.a if sqr else _u64(b)
         self.out = np.empty(self.a.size + self.b.size, dtype=np.uint64)
@@ -133,15 +136,29 @@ class Ticket:
         self._done = False
 
     def done(self) -> bool:
+        """True once the product is in OUT; raises the request's error.  The
+        poll that reports completion releases the ticket (hgx_poll)."""
         if self._done:
             return True
-        return check(_lib.lib().hgx_poll(self.ticket)) == 1
+        try:
+            rc = check(_lib.lib().hgx_poll(self.ticket))
+        except HgxError:
+            self._done = True
+            raise
+        if rc == 1:
+            self._done = True
+        return self._done
 
     def wait(self) -> np.ndarray:
         if not self._done:
-            check(_lib.lib().hgx_wait(self.ticket))
             self._done = True
+            check(_lib.lib().hgx_wait(self.ticket))
         return self.out
+
+    def __del__(self):
+        if not getattr(self, "_done", True) and _lib._lib is not None:
+            self._done = True
+            _lib._lib.hgx_wait(self.ticket)
 
 
 def submit(a, b=None) -> Ticket:

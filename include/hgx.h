@@ -109,10 +109,30 @@ int hgx_ntt_reverse(uint64_t *data, size_t n, int mod_num);
 // device memory (the HBM budget) frees up, and their work overlaps on the
 // device.  Host buffers must stay valid until the ticket completes.
 // hgx_poll returns 1 when done, 0 while pending, <0 on failure;
-// hgx_wait blocks and returns HGX_OK or the request's error.
+// hgx_wait blocks and returns HGX_OK or the request's error.  A ticket is
+// released by hgx_wait or by the hgx_poll call that reports its completion
+// (1 or an error); later calls with it return HGX_EINVAL.  hgx_shutdown lets
+// the work already on the device finish and fails the requests that have not
+// started with HGX_ENOTINIT.  A failed upload, download or kernel launch
+// fails only the requests it concerns.
 int hgx_submit(const hgx_mul_desc *desc, hgx_ticket *ticket);
 int hgx_poll(hgx_ticket ticket);
 int hgx_wait(hgx_ticket ticket);
+
+// Counters of the request server since it started (all zero before the
+// first hgx_submit): requests completed and failed, admission rounds
+// launched, requests computed host-staged (larger than the server's arena),
+// requests queued and on the device now, seconds since start, and seconds
+// the server thread spent issuing rounds (copies and kernel launches) -- the
+// host-side cost per request that a request loop resident on the device
+// would remove.  No counterpart in the reference.
+typedef struct {
+  uint64_t completed, failed, rounds, staged, queued, in_flight;
+  double uptime_s, launch_s;
+  int lanes, split_downloads;
+} hgx_server_counters;
+
+int hgx_server_stats(hgx_server_counters *out);
 
 // Magnitude sum and difference of device limb vectors, the additions
 // between the products of a Newton iteration kept in device memory
