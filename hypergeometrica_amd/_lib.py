@@ -23,7 +23,7 @@ EXPORTS = (
     "hgx_ntt_reverse", "hgx_submit", "hgx_poll", "hgx_wait", "hgx_fill_splitmix64",
     "hgx_timing_enable", "hgx_timing_read", "hgx_add_device", "hgx_sub_device", "hgx_graphs_enable",
     "hgx_plan_set", "hgx_plan_get", "hgx_autotune", "hgx_split_enable", "hgx_staging_limit",
-    "hgx_server_stats",
+    "hgx_server_stats", "hgx_mul_device_noalloc",
 )
 
 
@@ -79,6 +79,7 @@ def load() -> ctypes.CDLL:
         L.hgx_mul_batch.argtypes = [ctypes.POINTER(MulDesc), _sz]
         L.hgx_mul_device.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
         L.hgx_sqr_device.argtypes = [_vp, _sz, _vp, _vp]
+        L.hgx_mul_device_noalloc.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
         L.hgx_add_device.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
         L.hgx_plan_set.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _ip = ctypes.POINTER(ctypes.c_int)

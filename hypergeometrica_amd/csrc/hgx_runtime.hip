@@ -719,7 +719,7 @@ int launch_row_inv_scaled(int sbits, dim3 grid, hipStream_t s, const RowParams &
                                                                        : fail(HGX_EINVAL, "row pass size out of range");
 }
 
-size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 int log2_ceil(u64 x) {
   int n = 0;
@@ -731,7 +731,7 @@ int log2_ceil(u64 x) {
 struct WorkLayout {
   size_t res0, res1, ovf, flags, maps, agg, blkin, total;
 };
-WorkLayout work_layout(int n, bool sqr) {
+constexpr WorkLayout work_layout(int n, bool sqr) {
   const u64 L = 1ull << n;
   // carry runs of the smallest CRT block the length may use
   const u64 rl = std::min<u64>(L, 1ull << (n <= kCrtHalfMaxBits ? HGX_CRT_BLOCK_BITS - 1 : HGX_CRT_BLOCK_BITS));
@@ -757,6 +757,32 @@ WorkLayout work_layout(int n, bool sqr) {
   w.total = at;
   return w;
 }
+
+// Products without allocation (hgx_mul_device_noalloc) compute in a scratch
+// array the library holds on every device it runs on: the work memory of one
+// product of transform length at most 2^kNoAllocMaxBits (the largest layout
+// of those lengths is the general product's at the top length).
+constexpr int kNoAllocMaxBits = 20;
+constexpr size_t noalloc_bytes() {
+  size_t m = 0;
+  for (int n = 1; n <= kNoAllocMaxBits; n++)
+    for (int sq = 0; sq < 2; sq++) m = std::max(m, work_layout(n, sq != 0).total);
+  return m;
+}
+constexpr size_t kNoAllocBytes = noalloc_bytes();
+constexpr int kMaxDevices = 64;
+void *g_scratch[kMaxDevices] = {};  // the array's device address, per device (hipGetSymbolAddress)
+
+}  // namespace
+
+// Work memory of hgx_mul_device_noalloc: the 6 * 2^20 residue words and the
+// carry arrays of one product of transform length 2^20 (48.0 MiB), held by
+// every device the library runs on until the process ends.  Host code never
+// uses this name as an address: hgx_init looks up each device's copy with
+// hipGetSymbolAddress.
+static __device__ __align__(256) uint64_t scratch[kNoAllocBytes / 8];
+
+namespace {
 
 MulJob make_job(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, int n, bool sqr, char *base) {
   const WorkLayout wl = work_layout(n, sqr);
@@ -2067,6 +2093,15 @@ int hgx_init(int device, size_t hbm_budget_bytes) {
   if (device < 0 || device >= ndev) return fail(HGX_EINVAL, "no such device");
   HIPCHK(hipSetDevice(device));
   g_ctx.device = device;
+  if (device >= kMaxDevices) return fail(HGX_EINVAL, "device index above 63");
+  if (!g_scratch[device]) {
+    void *sp = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp)
+      return fail(HGX_EDEVICE, std::string("looking up the no-allocation scratch array failed: ") +
+                                   hipGetErrorString(e));
+    g_scratch[device] = sp;
+  }
   size_t freeb = 0, totalb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
   g_ctx.budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(0.9 * (double)freeb);
@@ -2180,6 +2215,34 @@ int hgx_sqr_device(const uint64_t *d_a, size_t na, uint64_t *d_out, void *stream
   if (!d_a || !d_out || na == 0) return fail(HGX_EINVAL, "null operand or zero length");
   hipStream_t s = stream ? (hipStream_t)stream : g_ctx.stream;
   return mul_device_locked(d_a, na, d_a, na, d_out, true, s);
+}
+
+int hgx_mul_device_noalloc(const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, uint64_t *d_out,
+                           void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!d_a || !d_b || !d_out || na == 0 || nb == 0) return fail(HGX_EINVAL, "null operand or zero length");
+  const int n = log2_ceil((u64)hgx_product_length(na, nb));
+  if (n > kNoAllocMaxBits)
+    return fail(HGX_ETOOBIG, "hgx_mul_device_noalloc takes products of transform length at most 2^" +
+                                 std::to_string(kNoAllocMaxBits) + " (na + nb <= " +
+                                 std::to_string(1ull << kNoAllocMaxBits) + "); this one needs 2^" + std::to_string(n));
+  if (!g_ctx.plans.count(plan_key(n, choice_for(n))))
+    return fail(HGX_ENOTINIT, "the root tables of transform length 2^" + std::to_string(n) +
+                                  " are not resident: run one hgx_mul_device product of that length first");
+  void *sp = g_scratch[g_ctx.device];
+  if (!sp) return fail(HGX_ENOTINIT, "no scratch array on this device");
+  const bool sqr = (d_a == d_b && na == nb);
+  hipStream_t s = stream ? (hipStream_t)stream : g_ctx.stream;
+  JobRef jr{};
+  jr.one = make_job(d_a, na, d_b, nb, d_out, n, sqr, (char *)sp);
+  // no per-kernel timing events here: creating one may allocate
+  const bool timing = g_timing;
+  g_timing = false;
+  rc = enqueue_group(jr, 1, n, sqr, s);
+  g_timing = timing;
+  return rc;
 }
 
 // host-memory product: stage operands into device memory, multiply, copy back

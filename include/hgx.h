@@ -93,6 +93,23 @@ int hgx_sqr_device(const uint64_t *d_a, size_t na, uint64_t *d_out,
 int hgx_mul_batch_device(const hgx_mul_desc *descs, size_t count,
                          void *stream);
 
+// hgx_mul_device without any allocation: no hipMalloc, hipFree or other
+// allocating call, for callers that cannot allocate (code under HIP graph
+// capture, tight serving loops, harnesses that forbid it).  No counterpart in
+// the reference.  The work memory is a scratch array of 48 MiB that the
+// library holds on every device it runs on from hgx_init until the process
+// ends; products of transform length at most 2^20 (NA + NB <= 2^20 limbs)
+// fit it, larger ones fail with HGX_ETOOBIG.  The root tables of the
+// product's transform length must already be resident (any earlier
+// hgx_mul_device product of that length builds them), else HGX_ENOTINIT.
+// Never split (one transform of length >= NA + NB).  Every call on a device
+// shares that device's scratch array, which keeps the last call's contents
+// (nothing in it needs to start at zero): calls that overlap on one device --
+// on different streams, or from different threads -- race; calls ordered on
+// one stream do not.  Same output as hgx_mul_device.
+int hgx_mul_device_noalloc(const uint64_t *d_a, size_t na, const uint64_t *d_b,
+                           size_t nb, uint64_t *d_out, void *stream);
+
 // Forward transform of N canonical residues (each < modulus MOD_NUM) in
 // place, output in bit-reversed order.  Same result as NTT-FORWARD,
 // src/number-theoretic-transform.lisp:40-90, with **scheme** and MOD_NUM.
