@@ -103,24 +103,58 @@ def base_case(series: Series, lower: int) -> Partial:
     return Partial(lower, lower + 1, p, series.q(lower), series.b(lower), p * series.a(lower))
 
 
-def _tree(lower: int, upper: int, depth: int, nodes: list):
-    """The reference's split tree; returns the node's height."""
-    if upper - lower == 1:
+FOLD_TERMS = 32
+"""Subtrees of at most this many terms are evaluated by a left fold of their
+base cases on the host.  Their partials are the same four integers as the
+recursion's (below), and their numbers stay below *ntt-multiply-threshold*
+limbs for every series here, so the reference multiplies them on the host
+too; every node above them is the reference's own node with its own seven
+products."""
+
+
+def fold_partial(series: Series, lower: int, upper: int) -> Partial:
+    """The partial over [lower, upper): base case `lower` combined with the
+    base cases `lower + 1, ...` one at a time.  P, Q and B are the products
+    of p, q and b over the interval, and R = sum over n of
+    a(n) * prod_{m != n} b(m) * prod_{m <= n} p(m) * prod_{m > n} q(m), which
+    COMBINE (src/binary-splitting.lisp:112-121) preserves for any split, so
+    the fold gives the recursion's integers (tested against it)."""
+    a, b, p, q = series.a, series.b, series.p, series.q
+    P = p(lower)
+    Q, B, R = q(lower), b(lower), P * a(lower)
+    for k in range(lower + 1, upper):
+        pk, qk, bk = p(k), q(k), b(k)
+        # COMBINE with the base case (p_k, q_k, b_k, p_k a_k) on the right
+        R = bk * qk * R + B * P * pk * a(k)
+        P *= pk
+        Q *= qk
+        B *= bk
+    return Partial(lower, upper, P, Q, B, R)
+
+
+def _tree(lower: int, upper: int, nodes: list, blocks: list, fold: int) -> int:
+    """The reference's split tree down to subtrees of at most FOLD terms
+    (recorded in BLOCKS); returns the node's height above them."""
+    if upper - lower <= fold:
+        blocks.append((lower, upper))
         return 0
     m = (lower + upper) // 2
-    h = 1 + max(_tree(lower, m, depth + 1, nodes), _tree(m, upper, depth + 1, nodes))
+    h = 1 + max(_tree(lower, m, nodes, blocks, fold), _tree(m, upper, nodes, blocks, fold))
     nodes.append((h, lower, m, upper))
     return h
 
 
-def binary_split(series: Series, lower: int, upper: int, mul_batch=None) -> Partial:
+def binary_split(series: Series, lower: int, upper: int, mul_batch=None, fold: int = FOLD_TERMS) -> Partial:
     """binary-split (src/binary-splitting.lisp:123-133), evaluated one tree
-    height at a time with batched products."""
+    height at a time with batched products, the subtrees of at most FOLD
+    terms by host folds (FOLD = 1: the whole tree, from base cases)."""
     assert upper > lower
     mul = mul_batch or mul_many
-    parts: dict[tuple[int, int], Partial] = {(k, k + 1): base_case(series, k) for k in range(lower, upper)}
     nodes: list = []
-    _tree(lower, upper, 0, nodes)
+    blocks: list = []
+    _tree(lower, upper, nodes, blocks, max(1, fold))
+    parts: dict[tuple[int, int], Partial] = {
+        (lo, up): base_case(series, lo) if up - lo == 1 else fold_partial(series, lo, up) for lo, up in blocks}
     heights: dict[int, list] = {}
     for h, lo, m, up in nodes:
         heights.setdefault(h, []).append((lo, m, up))

@@ -56,3 +56,30 @@ def test_residue_checker_matches_python_integers():
     y = x.copy()
     y[[5, 6]] = y[[6, 5]]
     assert x[5] == x[6] or R.residue(x) != R.residue(y)
+
+
+def test_pi_checkers_pinned_by_mpmath():
+    # BBP digit extraction, MPFR's pi and the MPD scaling agree with mpmath's
+    # pi; a value off in one bit fails them
+    import mpmath
+
+    from tests import _bbp, _mpfr
+    from tests import _pi_check as PC
+    prec = 20000
+    mpmath.mp.prec = prec + 256
+    # pi/10 as an MPD of 64-bit digits, truncated to prec bits
+    e = -(prec // 64)
+    mant = int(mpmath.floor(mpmath.pi / 10 * mpmath.mpf(2) ** (-64 * e)))
+    for d in PC.bbp_positions(prec) + [0, 1, 100]:
+        g = int(mpmath.floor(mpmath.frac(mpmath.pi * mpmath.mpf(16) ** d) * mpmath.mpf(2) ** 64))
+        assert _bbp.matches(d, g), d
+    dist = PC.bbp_check(mant, e, PC.bbp_positions(prec))
+    assert PC.bbp_ok(dist), dist
+    bad = mant ^ (1 << (prec - 4 * PC.bbp_positions(prec)[0] - 40))
+    assert not PC.bbp_ok(PC.bbp_check(bad, e, PC.bbp_positions(prec)))
+    if _mpfr.available():
+        assert PC.mpfr_leading_bits_match(mant, e, 16000)
+        assert not PC.mpfr_leading_bits_match(mant ^ (1 << (prec - 12000)), e, 16000)
+    from tests import _gmp
+    if _gmp.available():
+        assert PC.gmp_mul(-(3 ** 5000), 7 ** 3000) == -(3 ** 5000) * 7 ** 3000
