@@ -59,12 +59,18 @@ def kernel_bytes(name: str, L: int, na: int, nb: int, nops: int) -> float | None
     return None
 
 
-def cpu_baseline(sample_bits: int = 22) -> dict:
+def cpu_baseline(sample_bits: int = 22, headline_bits: int = 26) -> dict:
     """Time the oracle's mpz-*/ntt restatement on one 2^sample_bits x
-    2^sample_bits multiply: single-threaded (the reference's default build,
-    the reported value), and with the reference's parallel tasks (one thread
-    per operand and modulus, at most 6); GMP's mpz_mul on the same operands
-    is a sanity line, not the baseline.  All three products are compared."""
+    2^sample_bits multiply (L = 2^(sample_bits+1), about 11 s): single-threaded
+    (the reference's default build, the reported value), and with the
+    reference's parallel tasks (one thread per operand and modulus, at most
+    6); GMP's mpz_mul on the same operands is a sanity line, not the
+    baseline.  All three products are compared.  The reported value is the
+    rate at the headline shape (L = 2^(headline_bits+1)) at the sample's time
+    per butterfly: the radix-2 transforms dominate and cost L log2 L, so the
+    sample's rate is scaled by log2 L_sample / log2 L_headline (CRT and
+    pointwise work, linear in L, are charged the same factor: the CPU rate is
+    if anything understated)."""
     import numpy as np
 
     from oracle import oracle as O
@@ -90,76 +96,128 @@ def cpu_baseline(sample_bits: int = 22) -> dict:
     except (ImportError, OSError):
         gmp = None
     host = f"host {platform.processor() or platform.machine()}, {os.cpu_count()} cpus"
+    scale = (sample_bits + 1) / (headline_bits + 1)
     return {
-        "value": (2 * n) / dt,
+        "value": (2 * n) / dt * scale,
         "unit": "limbs/s",
         "cores": 1,
         "kind": "port",
         "sample": f"one {n}x{n}-limb multiply (L=2^{sample_bits + 1}) through the oracle's restatement of "
-                  f"mpz-*/ntt, 1 thread, {dt:.2f} s; {host}",
-        "parallel_tasks": {"value": (2 * n) / dt6, "unit": "limbs/s", "cores": 6, "seconds": round(dt6, 3),
+                  f"mpz-*/ntt, 1 thread, {dt:.2f} s ({(2 * n) / dt:.4g} limbs/s), scaled to L=2^{headline_bits + 1} "
+                  f"by {sample_bits + 1}/{headline_bits + 1} (time per radix-2 butterfly held); {host}",
+        "sample_value": (2 * n) / dt,
+        "parallel_tasks": {"value": (2 * n) / dt6 * scale, "unit": "limbs/s", "cores": 6, "seconds": round(dt6, 3),
                            "same_product": bool(np.array_equal(p6, p1)),
                            "sample": "same multiply, one thread per (operand, prime) forward task and per prime "
-                                     "pointwise and inverse task (with-parallel-work), CRT serial"},
+                                     "pointwise and inverse task (with-parallel-work), CRT serial; scaled alike"},
         "gmp_sanity": gmp,
     }
 
 
-def batched_leg(torch, h, _lib, count: int = 4096, reps: int = 3) -> dict:
+class Dev:
+    """LIMBS words of device memory from the library (hgx_device_alloc)."""
+
+    def __init__(self, _lib, limbs: int):
+        import ctypes
+        p = ctypes.c_void_p(0)
+        _lib.check(_lib.lib().hgx_device_alloc(limbs, ctypes.byref(p)))
+        self._lib, self.ptr, self.n = _lib, int(p.value), limbs
+
+    def free(self):
+        if self.ptr:
+            self._lib.check(self._lib.lib().hgx_device_free(self.ptr, None))
+            self.ptr = 0
+
+
+class Pinned:
+    """LIMBS words of pinned host memory from the library (hgx_host_alloc),
+    with a numpy view."""
+
+    def __init__(self, _lib, limbs: int):
+        import ctypes
+
+        import numpy as np
+        p = ctypes.c_void_p(0)
+        _lib.check(_lib.lib().hgx_host_alloc(8 * limbs, ctypes.byref(p)))
+        self._lib, self.ptr, self.n = _lib, int(p.value), limbs
+        self.arr = np.ctypeslib.as_array((ctypes.c_uint64 * limbs).from_address(self.ptr))
+
+    def free(self):
+        if self.ptr:
+            self.arr = None
+            self._lib.check(self._lib.lib().hgx_host_free(self.ptr))
+            self.ptr = 0
+
+
+def sync(_lib) -> None:
+    """Wait for the work queued on the library's stream."""
+    _lib.check(_lib.lib().hgx_synchronize(None))
+
+
+def dev_equal(_lib, a: int, b: int, n: int) -> bool:
+    """Whether N limbs at device addresses A and B are equal."""
+    import ctypes
+    c = ctypes.c_int(9)
+    _lib.check(_lib.lib().hgx_device_compare(a, n, b, n, ctypes.byref(c), None))
+    return c.value == 0
+
+
+def download(_lib, ptr: int, n: int):
+    import numpy as np
+    out = np.empty(n, dtype=np.uint64)
+    _lib.check(_lib.lib().hgx_download(out.ctypes.data, ptr, n, None))
+    return out
+
+
+def batched_leg(h, _lib, count: int = 4096, reps: int = 3) -> dict:
     """Binary-splitting level: COUNT independent products, operand sizes
     log-uniform in [2^8, 2^16] limbs (seed 0x48475803), operands in HBM.
     Batched (hgx_mul_batch_device, grouped by transform length) against the
     per-call path (hgx_mul_device per product); outputs compared."""
-    import ctypes
     na, nb = W.batch_shapes(count, W.BATCH_SEED)
-    tot_in = sum(na) + sum(nb)
-    tot_out = tot_in
-    dev = torch.device("cuda:0")
-    IN = torch.empty(tot_in, dtype=torch.int64, device=dev)
-    OUT1 = torch.zeros(tot_out, dtype=torch.int64, device=dev)
-    OUT2 = torch.zeros(tot_out, dtype=torch.int64, device=dev)
+    tot = sum(na) + sum(nb)
+    IN, OUT1, OUT2 = Dev(_lib, tot), Dev(_lib, tot), Dev(_lib, tot)
     descs1 = (_lib.MulDesc * count)()
     descs2 = (_lib.MulDesc * count)()
-    base, ob = IN.data_ptr(), 0
     at = 0
     for i in range(count):
-        a = base + 8 * at
-        h.fill_splitmix64(a, na[i], 0x48475803 + 2 * i + 1)
-        at += na[i]
-        b = base + 8 * at
-        h.fill_splitmix64(b, nb[i], 0x48475803 + 2 * i + 2)
-        at += nb[i]
-        descs1[i] = _lib.MulDesc(a, na[i], b, nb[i], OUT1.data_ptr() + 8 * ob)
-        descs2[i] = _lib.MulDesc(a, na[i], b, nb[i], OUT2.data_ptr() + 8 * ob)
-        ob += na[i] + nb[i]
+        a = IN.ptr + 8 * at
+        h.fill_splitmix64(a, na[i], W.BATCH_SEED + 2 * i + 1)
+        b = a + 8 * na[i]
+        h.fill_splitmix64(b, nb[i], W.BATCH_SEED + 2 * i + 2)
+        descs1[i] = _lib.MulDesc(a, na[i], b, nb[i], OUT1.ptr + 8 * at)
+        descs2[i] = _lib.MulDesc(a, na[i], b, nb[i], OUT2.ptr + 8 * at)
+        at += na[i] + nb[i]
     L_ = _lib.lib()
     # warm up both paths (plans, work memory)
     _lib.check(L_.hgx_mul_batch_device(descs1, count, None))
     for i in range(count):
         _lib.check(L_.hgx_mul_device(descs2[i].a, descs2[i].na, descs2[i].b, descs2[i].nb, descs2[i].out, None))
-    torch.cuda.synchronize()
+    sync(_lib)
     tb = []
     for _ in range(reps):
         t0 = time.perf_counter()
         _lib.check(L_.hgx_mul_batch_device(descs1, count, None))
-        torch.cuda.synchronize()
+        sync(_lib)
         tb.append(time.perf_counter() - t0)
     tp = []
     for _ in range(reps):
         t0 = time.perf_counter()
         for i in range(count):
             _lib.check(L_.hgx_mul_device(descs2[i].a, descs2[i].na, descs2[i].b, descs2[i].nb, descs2[i].out, None))
-        torch.cuda.synchronize()
+        sync(_lib)
         tp.append(time.perf_counter() - t0)
-    same = bool(torch.equal(OUT1, OUT2))
+    same = dev_equal(_lib, OUT1.ptr, OUT2.ptr, tot)
+    for d in (IN, OUT1, OUT2):
+        d.free()
     b, p = min(tb), min(tp)
     return {"workload": f"{count} products, operand limbs log-uniform in [2^8, 2^16], seed 0x48475803",
-            "limbs": tot_out, "batched_s": round(b, 5), "per_call_s": round(p, 5),
-            "batched_limbs_per_s": tot_out / b, "per_call_limbs_per_s": tot_out / p,
+            "limbs": tot, "batched_s": round(b, 5), "per_call_s": round(p, 5),
+            "batched_limbs_per_s": tot / b, "per_call_limbs_per_s": tot / p,
             "speedup": round(p / b, 2), "outputs_identical": same}
 
 
-def square_leg(torch, h, _lib, bits: int = 26, steps: int = 10) -> dict:
+def square_leg(h, _lib, bits: int = 26, steps: int = 10) -> dict:
     """mpz-square (src/ntt-multiply.lisp:101-172) of one 2^BITS-limb operand
     (seed 0x48475801, L = 2^(BITS+1)), operand and square in HBM: STEPS timed
     calls after two warm-up calls, output limbs (2n) per second and the
@@ -167,57 +225,49 @@ def square_leg(torch, h, _lib, bits: int = 26, steps: int = 10) -> dict:
     checked against the general product of the operand with a copy of itself."""
     L_ = _lib.lib()
     n = 1 << bits
-    dev = torch.device("cuda:0")
-    A = torch.empty(n, dtype=torch.int64, device=dev)
-    A2 = torch.empty(n, dtype=torch.int64, device=dev)
-    O = torch.empty(2 * n, dtype=torch.int64, device=dev)
-    h.fill_splitmix64(A.data_ptr(), n, 0x48475801)
-    A2.copy_(A)
+    A, A2, O, S = Dev(_lib, n), Dev(_lib, n), Dev(_lib, 2 * n), Dev(_lib, 2 * n)
+    h.fill_splitmix64(A.ptr, n, 0x48475801)
+    _lib.check(L_.hgx_device_copy(A2.ptr, A.ptr, n, None))
 
     def sq():
-        _lib.check(L_.hgx_sqr_device(A.data_ptr(), n, O.data_ptr(), None))
+        _lib.check(L_.hgx_sqr_device(A.ptr, n, O.ptr, None))
 
     for _ in range(2):
         sq()
-    torch.cuda.synchronize()
+    sync(_lib)
     t0 = time.perf_counter()
     for _ in range(steps):
         sq()
-    torch.cuda.synchronize()
+    sync(_lib)
     dt = (time.perf_counter() - t0) / steps
     _lib.timing_enable(True)
     for _ in range(steps):
         sq()
     kt = _lib.timing_read()
     _lib.timing_enable(False)
-    S = O.clone()
-    _lib.check(L_.hgx_mul_device(A.data_ptr(), n, A2.data_ptr(), n, O.data_ptr(), None))
-    torch.cuda.synchronize()
-    same = bool(torch.equal(S, O))
-    del A, A2, O, S
-    torch.cuda.empty_cache()
+    _lib.check(L_.hgx_mul_device(A.ptr, n, A2.ptr, n, S.ptr, None))
+    same = dev_equal(_lib, S.ptr, O.ptr, 2 * n)
+    for d in (A, A2, O, S):
+        d.free()
     return {"workload": f"square of one 2^{bits}-limb operand, L=2^{bits + 1}, operand in HBM",
             "ms": round(1e3 * dt, 3), "limbs_per_s": 2 * n / dt, "equals_general_product": same,
             "kernels_ms": {k: round(t / c, 4) for k, (c, t) in sorted(kt.items())}}
 
 
-def latency_leg(torch, h, _lib, n: int = 1 << 16, reps: int = 200) -> dict:
+def latency_leg(h, _lib, n: int = 1 << 16, reps: int = 200) -> dict:
     """One 2^16 x 2^16-limb product (L = 2^17, seeds 0x48475801/2), REPS
     calls: host-to-host through hgx_mul (pinned host buffers: limbs in, the
     product out) and device-resident through hgx_mul_device.  Per-call wall
     time p50 / p99 and limbs/s."""
-    import ctypes
+    import numpy as np
 
     L_ = _lib.lib()
-    dev = torch.device("cuda:0")
-    A = torch.empty(n, dtype=torch.int64, device=dev)
-    B = torch.empty(n, dtype=torch.int64, device=dev)
-    O = torch.empty(2 * n, dtype=torch.int64, device=dev)
-    h.fill_splitmix64(A.data_ptr(), n, 0x48475801)
-    h.fill_splitmix64(B.data_ptr(), n, 0x48475802)
-    hA = A.cpu().pin_memory()
-    hB = B.cpu().pin_memory()
-    hO = torch.empty(2 * n, dtype=torch.int64).pin_memory()
+    A, B, O = Dev(_lib, n), Dev(_lib, n), Dev(_lib, 2 * n)
+    h.fill_splitmix64(A.ptr, n, 0x48475801)
+    h.fill_splitmix64(B.ptr, n, 0x48475802)
+    hA, hB, hO = Pinned(_lib, n), Pinned(_lib, n), Pinned(_lib, 2 * n)
+    _lib.check(L_.hgx_download(hA.ptr, A.ptr, n, None))
+    _lib.check(L_.hgx_download(hB.ptr, B.ptr, n, None))
 
     def timed(fn):
         for _ in range(10):
@@ -231,16 +281,17 @@ def latency_leg(torch, h, _lib, n: int = 1 << 16, reps: int = 200) -> dict:
         return ts
 
     def host_call():
-        _lib.check(L_.hgx_mul(ctypes.c_void_p(hA.data_ptr()), n, ctypes.c_void_p(hB.data_ptr()), n,
-                              ctypes.c_void_p(hO.data_ptr())))
+        _lib.check(L_.hgx_mul(hA.ptr, n, hB.ptr, n, hO.ptr))
 
     def dev_call():
-        _lib.check(L_.hgx_mul_device(A.data_ptr(), n, B.data_ptr(), n, O.data_ptr(), None))
-        torch.cuda.synchronize()
+        _lib.check(L_.hgx_mul_device(A.ptr, n, B.ptr, n, O.ptr, None))
+        sync(_lib)
 
     th = timed(host_call)
     td = timed(dev_call)
-    same = bool(torch.equal(hO, O.cpu()))
+    same = bool(np.array_equal(hO.arr, download(_lib, O.ptr, 2 * n)))
+    for d in (A, B, O, hA, hB, hO):
+        d.free()
 
     def q(ts, f):
         return round(1e3 * ts[min(len(ts) - 1, int(f * len(ts)))], 4)
@@ -251,79 +302,100 @@ def latency_leg(torch, h, _lib, n: int = 1 << 16, reps: int = 200) -> dict:
             "outputs_identical": same}
 
 
-def pi_leg(bits: int = 1 << 20) -> dict:
+def pi_leg(bits: int = 1 << 28, identical_bits: int = 1 << 22, mpfr_bits: int = 1 << 24, host_iterates: bool = False,
+           threads: int = 16) -> dict:
     """mpd-pi (Chudnovsky binary splitting, Newton reciprocal and inverse
-    square root) to BITS bits with the truncating Newton steps, every product
-    above the 100-limb threshold on the GPU (binary-splitting levels batched):
-    with host-resident Newton iterates and with the iterates in HBM
-    (mpd_device), each the faster of two runs, against the same computation
-    with host products; results compared, and the leading 4096 bits checked
-    against mpmath's pi/10."""
-    import mpmath
-
+    square root, src/pi.lisp:53-74) to BITS bits with the reference's
+    mpd-truncate! applied after each Newton step (src/mpd.lisp:56-70; a
+    declared deviation: untruncated iterates grow without bound), every
+    product above the 100-limb threshold on the GPU, binary-splitting levels
+    batched and the Newton iterates in HBM (mpd_device); optionally also with
+    host-resident iterates.  Checked three ways: hexadecimal digits of pi at
+    positions near the end, the middle and the start by BBP digit extraction;
+    the leading MPFR_BITS bits against MPFR's pi; and, at IDENTICAL_BITS, the
+    GPU routings (host and HBM iterates) bit-identical to GMP-routed host
+    products."""
     from hypergeometrica_amd import mpd as M
     from hypergeometrica_amd import mpd_device as D
-    # each GPU variant runs twice and the faster run counts, so that first-use
-    # work (root tables, work memory) is not charged to one of them
-    tg = td = float("inf")
-    for _ in range(2):
+    from tests import _mpfr
+    from tests import _pi_check as PC
+    res = {"workload": f"mpd-pi to 2^{bits.bit_length() - 1} bits, truncating Newton, GPU products "
+                       f"(binary-splitting levels batched, Newton iterates in HBM)"}
+    t0 = time.perf_counter()
+    r = D.DeviceContext(truncate=True).mpd_pi(bits)
+    res["gpu_newton_in_hbm_s"] = round(time.perf_counter() - t0, 3)
+    if host_iterates:
         t0 = time.perf_counter()
         g = M.Context(truncate=True)
-        rg = g.mpd_pi(bits)
-        tg = min(tg, time.perf_counter() - t0)
-        t0 = time.perf_counter()
-        rd = D.DeviceContext(truncate=True).mpd_pi(bits)
-        td = min(td, time.perf_counter() - t0)
+        rh = g.mpd_pi(bits)
+        res["gpu_host_iterates_s"] = round(time.perf_counter() - t0, 3)
+        res["host_and_hbm_iterates_identical"] = rh == r
+        res["products"], res["max_product_limbs"] = g.products, g.max_product_limbs
+        del rh
+    pos = PC.bbp_positions(bits)
     t0 = time.perf_counter()
-    rc = M.Context(mul=lambda x, y: x * y, truncate=True).mpd_pi(bits)
-    tc = time.perf_counter() - t0
-    mpmath.mp.prec = 4200
-    ref = int(mpmath.floor(mpmath.pi / 10 * mpmath.mpf(2) ** 4096))
-    sh = -64 * rg.exponent - 4096
-    got = (rg.mantissa >> sh) if sh >= 0 else (rg.mantissa << -sh)
-    return {"workload": f"mpd-pi to 2^{bits.bit_length() - 1} bits, truncating Newton, GPU products vs host products",
-            "gpu_s": round(tg, 3), "gpu_newton_in_hbm_s": round(td, 3), "host_s": round(tc, 3),
-            "speedup": round(tc / tg, 2), "results_identical": rg == rc and rd == rc, "leading_4096_bits_match": abs(got - ref) <= 1,
-            "products": g.products, "max_product_limbs": g.max_product_limbs}
+    dist = PC.bbp_check(r.mantissa, r.exponent, pos, threads=threads)
+    res["bbp"] = {"hex_positions": pos, "distance_2^-64": [dist[d] for d in pos],
+                  "bound_2^-64": [8 * (d + 17) for d in pos], "match": PC.bbp_ok(dist),
+                  "seconds": round(time.perf_counter() - t0, 2)}
+    if _mpfr.available():
+        t0 = time.perf_counter()
+        res["mpfr"] = {"leading_bits": mpfr_bits,
+                       "match": PC.mpfr_leading_bits_match(r.mantissa, r.exponent, mpfr_bits),
+                       "seconds": round(time.perf_counter() - t0, 2)}
+    del r
+    t0 = time.perf_counter()
+    g = M.Context(truncate=True)
+    a = g.mpd_pi(identical_bits)
+    b = D.DeviceContext(truncate=True).mpd_pi(identical_bits)
+    c = M.Context(mul=PC.gmp_mul, truncate=True).mpd_pi(identical_bits)
+    res["routing"] = {"bits": identical_bits, "gpu_host_iterates_eq_gpu_hbm_eq_gmp": a == b == c,
+                      "products": g.products, "seconds": round(time.perf_counter() - t0, 2)}
+    return res
 
 
-def pcie_probe(torch, mib: int = 1024, reps: int = 3) -> dict:
+def pcie_probe(_lib, mib: int = 1024, reps: int = 3) -> dict:
     """Host<->device copy rates of pinned memory on this box: each direction
-    alone and both at once (two streams), best of REPS."""
-    dev = torch.device("cuda:0")
+    alone and both at once (two threads, one stream each), best of REPS."""
+    import ctypes
+    import threading
+    L_ = _lib.lib()
     n = mib << 17
-    H1 = torch.empty(n, dtype=torch.int64, pin_memory=True)
-    H2 = torch.empty(n, dtype=torch.int64, pin_memory=True)
-    D1 = torch.empty(n, dtype=torch.int64, device=dev)
-    D2 = torch.empty(n, dtype=torch.int64, device=dev)
-    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    H1, H2, D1, D2 = Pinned(_lib, n), Pinned(_lib, n), Dev(_lib, n), Dev(_lib, n)
+    s1, s2 = ctypes.c_void_p(0), ctypes.c_void_p(0)
+    _lib.check(L_.hgx_stream_create(ctypes.byref(s1)))
+    _lib.check(L_.hgx_stream_create(ctypes.byref(s2)))
+
+    def up():
+        _lib.check(L_.hgx_upload(D1.ptr, H1.ptr, n, s1))
+
+    def down():
+        _lib.check(L_.hgx_download(H2.ptr, D2.ptr, n, s2))
+
+    def both():
+        th = [threading.Thread(target=f) for f in (up, down)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
 
     def timed(fn):
         best = float("inf")
         for _ in range(reps):
-            torch.cuda.synchronize()
             t0 = time.perf_counter()
             fn()
-            torch.cuda.synchronize()
             best = min(best, time.perf_counter() - t0)
         return 8 * n / best / 1e9
 
-    def both():
-        with torch.cuda.stream(s1):
-            D1.copy_(H1, non_blocking=True)
-        with torch.cuda.stream(s2):
-            H2.copy_(D2, non_blocking=True)
-
-    up = timed(lambda: D1.copy_(H1, non_blocking=True))
-    down = timed(lambda: H2.copy_(D2, non_blocking=True))
-    bi = timed(both)
-    del H1, H2, D1, D2
-    torch.cuda.empty_cache()
-    torch._C._host_emptyCache()
+    r_up, r_down, r_bi = timed(up), timed(down), timed(both)
+    _lib.check(L_.hgx_stream_destroy(s1))
+    _lib.check(L_.hgx_stream_destroy(s2))
+    for d in (H1, H2, D1, D2):
+        d.free()
     # a served request moves 8 bytes per output limb up (its operands) and 8
     # down (its product), both directions at once
-    return {"h2d_GBps": round(up, 1), "d2h_GBps": round(down, 1), "both_ways_GBps_each": round(bi, 1),
-            "bound_limbs_per_s": bi * 1e9 / 8}
+    return {"h2d_GBps": round(r_up, 1), "d2h_GBps": round(r_down, 1), "both_ways_GBps_each": round(r_bi, 1),
+            "bound_limbs_per_s": r_bi * 1e9 / 8}
 
 
 class _Stream:
@@ -331,27 +403,24 @@ class _Stream:
     buffers (requests take slices of them) with their prefix residues mod
     2^61 - 1, and SLOTS product buffers."""
 
-    def __init__(self, torch, slots: int, seed: int):
-        import numpy as np
-
+    def __init__(self, _lib, slots: int, seed: int):
         from tests import _residue as RES
         self.nmax = 1 << 24
-        self.A = torch.empty(self.nmax, dtype=torch.int64, pin_memory=True)
-        self.B = torch.empty(self.nmax, dtype=torch.int64, pin_memory=True)
-        self.a = self.A.numpy().view(np.uint64)
-        self.b = self.B.numpy().view(np.uint64)
+        self.A, self.B = Pinned(_lib, self.nmax), Pinned(_lib, self.nmax)
+        self.a, self.b = self.A.arr, self.B.arr
         self.a[:] = W.splitmix64(seed + 3, self.nmax)
         self.b[:] = W.splitmix64(seed + 4, self.nmax)
         self.PA = RES.prefix_residues(self.a)
         self.PB = RES.prefix_residues(self.b)
-        self.outs = [torch.empty(2 * self.nmax, dtype=torch.int64, pin_memory=True) for _ in range(slots)]
+        self.outs = [Pinned(_lib, 2 * self.nmax) for _ in range(slots)]
 
-    def release(self, torch):
-        del self.A, self.B, self.outs, self.a, self.b
-        torch._C._host_emptyCache()
+    def release(self):
+        self.a = self.b = None
+        for d in [self.A, self.B] + self.outs:
+            d.free()
 
 
-def serve_stream(torch, _lib, st: _Stream, rate: float, seconds: float, seed: int, checkers: int = 12,
+def serve_stream(_lib, st: _Stream, rate: float, seconds: float, seed: int, checkers: int = 12,
                  gmp_sample: int = 0, check: bool = True) -> dict:
     """Poisson arrivals at RATE/s for SECONDS through hgx_submit, operand
     sizes log-uniform in [2^10, 2^24] limbs (SEED), operands slices of the
@@ -386,9 +455,9 @@ def serve_stream(torch, _lib, st: _Stream, rate: float, seconds: float, seed: in
         try:
             if not check:
                 return
-            got = RES.residue_ptr(st.outs[slot].data_ptr(), n)
+            got = RES.residue_ptr(st.outs[slot].ptr, n)
             if sample_every and i % sample_every == 0:
-                kept[i] = st.outs[slot].numpy()[:n].view(np.uint64).copy()
+                kept[i] = st.outs[slot].arr[:n].copy()
             if got != want:
                 mism.append(i)
             checked[0] += 1
@@ -429,7 +498,7 @@ def serve_stream(torch, _lib, st: _Stream, rate: float, seconds: float, seed: in
         oa = (offs[i] >> 8) % (st.nmax - na + 1)
         ob = (offs[i] >> 32) % (st.nmax - nb + 1)
         want = RES.mulmod(RES.slice_residue(st.PA, oa, na), RES.slice_residue(st.PB, ob, nb))
-        d = _lib.MulDesc(st.A.data_ptr() + 8 * oa, na, st.B.data_ptr() + 8 * ob, nb, st.outs[slot].data_ptr())
+        d = _lib.MulDesc(st.A.ptr + 8 * oa, na, st.B.ptr + 8 * ob, nb, st.outs[slot].ptr)
         t = ctypes.c_uint64(0)
         _lib.check(L_.hgx_submit(ctypes.byref(d), ctypes.byref(t)))
         inflight[t.value] = (i, sched, slot, na + nb, want)
@@ -467,7 +536,7 @@ def serve_stream(torch, _lib, st: _Stream, rate: float, seconds: float, seed: in
             "server_thread_launch_frac": round(launched / wall, 4)}
 
 
-def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 10.0, slots: int = 32,
+def serving_leg(_lib, rate: float = 2000.0, seconds: float = 10.0, slots: int = 32,
                 seed: int = W.SERVE_SEED, search=(1500.0, 1200.0, 1000.0, 800.0, 600.0, 400.0),
                 search_seconds: float = 4.0) -> dict:
     """Request server under load (the serving configuration): Poisson
@@ -479,18 +548,18 @@ def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 10.0, slo
     Every served product is checked (serve_stream)."""
     import ctypes
     L_ = _lib.lib()
-    st = _Stream(torch, slots, seed)
+    st = _Stream(_lib, slots, seed)
     try:
         for bits in range(10, 25):
-            d = _lib.MulDesc(st.A.data_ptr(), 1 << bits, st.B.data_ptr(), 1 << bits, st.outs[0].data_ptr())
+            d = _lib.MulDesc(st.A.ptr, 1 << bits, st.B.ptr, 1 << bits, st.outs[0].ptr)
             t = ctypes.c_uint64(0)
             _lib.check(L_.hgx_submit(ctypes.byref(d), ctypes.byref(t)))
             _lib.check(L_.hgx_wait(t))
-        main = serve_stream(torch, _lib, st, rate, seconds, seed, gmp_sample=6)
+        main = serve_stream(_lib, st, rate, seconds, seed, gmp_sample=6)
         sustained = None
         tried = []
         for r in search:
-            res = serve_stream(torch, _lib, st, r, search_seconds, seed + 0x100 + int(r))
+            res = serve_stream(_lib, st, r, search_seconds, seed + 0x100 + int(r))
             tried.append({k: res[k] for k in ("rate_per_s", "limbs_per_s", "p50_ms", "p99_ms", "p99_first_half_ms",
                                                "p99_second_half_ms", "bounded", "outputs_checked", "mismatches")})
             if res["bounded"]:
@@ -498,7 +567,7 @@ def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 10.0, slo
                 break
         stats = _lib.server_stats()
     finally:
-        st.release(torch)
+        st.release()
     main["workload"] = (f"Poisson arrivals {rate:.0f}/s for {seconds} s (seed {seed:#x}), operand limbs "
                         f"log-uniform in [2^10, 2^24], pinned host buffers, {slots} product slots, root tables "
                         f"built before the stream, every product checked mod 2^61-1 and {main['gmp_checked']} "
@@ -510,22 +579,20 @@ def serving_leg(torch, h, _lib, rate: float = 2000.0, seconds: float = 10.0, slo
     return main
 
 
-def staged_leg(torch, h, _lib, bits: int = 26, limits=(4 << 30, 1 << 30), reps: int = 2) -> dict:
+def staged_leg(h, _lib, bits: int = 26, limits=(4 << 30, 1 << 30), reps: int = 2) -> dict:
     """Host-memory product of two 2^bits-limb operands: whole in HBM (operands
     up and the product down over PCIe), then host-staged under device-memory
     limits (chunk products summed diagonal by diagonal); every result equal."""
     import numpy as np
     L_ = _lib.lib()
     n = 1 << bits
-    dev = torch.device("cuda:0")
-    A = torch.empty(n, dtype=torch.int64, device=dev)
-    B = torch.empty(n, dtype=torch.int64, device=dev)
-    h.fill_splitmix64(A.data_ptr(), n, 0x48475801)
-    h.fill_splitmix64(B.data_ptr(), n, 0x48475802)
-    a = A.cpu().numpy().view(np.uint64)
-    b = B.cpu().numpy().view(np.uint64)
-    del A, B
-    torch.cuda.empty_cache()
+    A, B = Dev(_lib, n), Dev(_lib, n)
+    h.fill_splitmix64(A.ptr, n, 0x48475801)
+    h.fill_splitmix64(B.ptr, n, 0x48475802)
+    a = download(_lib, A.ptr, n)
+    b = download(_lib, B.ptr, n)
+    A.free()
+    B.free()
 
     def run():
         t = []
@@ -562,10 +629,13 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-per-call leg")
     ap.add_argument("--no-square", action="store_true", help="skip the 2^bits-limb square leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-server leg")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the 2000/s serving stream")
+    ap.add_argument("--no-serve-search", action="store_true", help="skip the sustainable-rate search")
     ap.add_argument("--no-latency", action="store_true", help="skip the 2^16 x 2^16 per-call latency leg")
     ap.add_argument("--no-pi", action="store_true", help="skip the mpd-pi leg")
     ap.add_argument("--no-staged", action="store_true", help="skip the host-staged product leg")
-    ap.add_argument("--pi-bits", type=int, default=20, help="mpd-pi precision 2^bits")
+    ap.add_argument("--pi-bits", type=int, default=28, help="mpd-pi precision 2^bits")
+    ap.add_argument("--pi-host-iterates", action="store_true", help="also time mpd-pi with host-resident iterates")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default pass structures)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (reference kernel for HBM counters)")
@@ -575,30 +645,29 @@ def main() -> int:
               f"and no multi-GPU path is part of this build", file=sys.stderr)
         return 2
 
-    import torch
-
     import hypergeometrica_amd as h
     from hypergeometrica_amd import _lib
 
+    # every device buffer, stream and pinned host buffer comes from the
+    # library: the process runs on the one HIP runtime the library is built
+    # against
     L_ = _lib.lib(0)
     n = 1 << args.bits
     na = nb = n
-    dev = torch.device("cuda:0")
-    A = torch.empty(na, dtype=torch.int64, device=dev)
-    B = torch.empty(nb, dtype=torch.int64, device=dev)
-    OUT = torch.empty(na + nb, dtype=torch.int64, device=dev)
+    A, B, OUT = Dev(_lib, na), Dev(_lib, nb), Dev(_lib, na + nb)
     L_bits = (2 * n - 1).bit_length()
-    h.fill_splitmix64(A.data_ptr(), na, 0x48475801)
-    h.fill_splitmix64(B.data_ptr(), nb, 0x48475802)
-    torch.cuda.synchronize()
+    h.fill_splitmix64(A.ptr, na, 0x48475801)
+    h.fill_splitmix64(B.ptr, nb, 0x48475802)
+    sync(_lib)
     if args.calib_copy:
         words = args.calib_copy << 17
-        src = torch.ones(words, dtype=torch.int64, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        src, dst = Dev(_lib, words), Dev(_lib, words)
+        _lib.check(L_.hgx_device_fill(src.ptr, words, 1, None))
+        _lib.check(L_.hgx_device_copy(dst.ptr, src.ptr, words, None))
+        sync(_lib)
         print(f"# calibration copy: {words * 8} bytes read, {words * 8} bytes written", file=sys.stderr, flush=True)
-        del src, dst
+        src.free()
+        dst.free()
 
     # the autotuner picks the pass structure of every length the legs use
     tuned = {}
@@ -613,11 +682,11 @@ def main() -> int:
         print(f"# autotuned 2^13..2^{L_bits} in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     def step():
-        _lib.check(L_.hgx_mul_device(A.data_ptr(), na, B.data_ptr(), nb, OUT.data_ptr(), None))
+        _lib.check(L_.hgx_mul_device(A.ptr, na, B.ptr, nb, OUT.ptr, None))
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    sync(_lib)
     print(f"# warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
 
     # timed region: K steps, wall clock, synchronised on both sides; the
@@ -625,14 +694,14 @@ def main() -> int:
     # stream during these steps (pooled events, no measurable overhead)
     per_step = []
     _lib.timing_enable(True)
-    torch.cuda.synchronize()
+    sync(_lib)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
         step()
-        torch.cuda.synchronize()
+        sync(_lib)
         per_step.append(time.perf_counter() - s0)
-    torch.cuda.synchronize()
+    sync(_lib)
     wall = time.perf_counter() - t0
     kt = _lib.timing_read()
     _lib.timing_enable(False)
@@ -684,39 +753,40 @@ def main() -> int:
     p50 = per_step_sorted[len(per_step_sorted) // 2]
     p99 = per_step_sorted[min(len(per_step_sorted) - 1, int(0.99 * len(per_step_sorted)))]
 
-    del A, B, OUT
-    torch.cuda.empty_cache()
+    for d in (A, B, OUT):
+        d.free()
     square = None
     if not args.no_square:
         print("# square ...", file=sys.stderr, flush=True)
-        square = square_leg(torch, h, _lib, bits=args.bits)
+        square = square_leg(h, _lib, bits=args.bits)
 
     batched = None
     if not args.no_batch:
         print("# batched vs per-call ...", file=sys.stderr, flush=True)
-        batched = batched_leg(torch, h, _lib)
+        batched = batched_leg(h, _lib)
 
     latency = None
     if not args.no_latency:
         print("# per-call latency ...", file=sys.stderr, flush=True)
-        latency = latency_leg(torch, h, _lib)
+        latency = latency_leg(h, _lib)
 
     serving = None
     if not args.no_serve:
         print("# serving ...", file=sys.stderr, flush=True)
-        serving = serving_leg(torch, h, _lib)
-        serving["pcie"] = pcie_probe(torch)
+        serving = serving_leg(_lib, seconds=args.serve_seconds,
+                              search=() if args.no_serve_search else (1500.0, 1200.0, 1000.0, 800.0, 600.0, 400.0))
+        serving["pcie"] = pcie_probe(_lib)
         serving["frac_of_pcie_bound"] = round(serving["limbs_per_s"] / serving["pcie"]["bound_limbs_per_s"], 4)
 
     staged = None
     if not args.no_staged:
         print("# host-staged products ...", file=sys.stderr, flush=True)
-        staged = staged_leg(torch, h, _lib, bits=args.bits)
+        staged = staged_leg(h, _lib, bits=args.bits)
 
     pi = None
     if not args.no_pi:
         print("# mpd-pi ...", file=sys.stderr, flush=True)
-        pi = pi_leg(1 << args.pi_bits)
+        pi = pi_leg(1 << args.pi_bits, host_iterates=args.pi_host_iterates)
 
     cpu = None
     if not args.no_cpu:
@@ -757,12 +827,9 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    # release the library and torch's cached pinned host memory while the HIP
-    # runtime is fully up, before the interpreter's and the runtime's own
-    # teardown
+    # release the library while the HIP runtime is fully up, before the
+    # interpreter's and the runtime's own teardown
     _lib.shutdown()
-    torch.cuda.synchronize()
-    torch._C._host_emptyCache()
     return 0
 
 

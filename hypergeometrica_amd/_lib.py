@@ -24,6 +24,9 @@ EXPORTS = (
     "hgx_timing_enable", "hgx_timing_read", "hgx_add_device", "hgx_sub_device", "hgx_graphs_enable",
     "hgx_plan_set", "hgx_plan_get", "hgx_autotune", "hgx_split_enable", "hgx_staging_limit",
     "hgx_server_stats", "hgx_mul_device_noalloc",
+    "hgx_device_alloc", "hgx_device_free", "hgx_upload", "hgx_download", "hgx_device_copy", "hgx_device_fill",
+    "hgx_device_significant", "hgx_device_compare", "hgx_shl_device", "hgx_host_alloc", "hgx_host_free",
+    "hgx_synchronize", "hgx_stream_create", "hgx_stream_destroy",
 )
 
 
@@ -97,6 +100,20 @@ def load() -> ctypes.CDLL:
         L.hgx_timing_enable.argtypes = [ctypes.c_int]
         L.hgx_timing_read.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int]
         L.hgx_server_stats.argtypes = [ctypes.POINTER(ServerStats)]
+        L.hgx_device_alloc.argtypes = [_sz, ctypes.POINTER(_vp)]
+        L.hgx_device_free.argtypes = [_vp, _vp]
+        L.hgx_upload.argtypes = [_vp, _vp, _sz, _vp]
+        L.hgx_download.argtypes = [_vp, _vp, _sz, _vp]
+        L.hgx_device_copy.argtypes = [_vp, _vp, _sz, _vp]
+        L.hgx_device_fill.argtypes = [_vp, _sz, _u64, _vp]
+        L.hgx_device_significant.argtypes = [_vp, _sz, ctypes.POINTER(_sz), _vp]
+        L.hgx_device_compare.argtypes = [_vp, _sz, _vp, _sz, ctypes.POINTER(ctypes.c_int), _vp]
+        L.hgx_shl_device.argtypes = [_vp, _sz, ctypes.c_int, _vp, _vp]
+        L.hgx_host_alloc.argtypes = [_sz, ctypes.POINTER(_vp)]
+        L.hgx_host_free.argtypes = [_vp]
+        L.hgx_synchronize.argtypes = [_vp]
+        L.hgx_stream_create.argtypes = [ctypes.POINTER(_vp)]
+        L.hgx_stream_destroy.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -111,13 +128,20 @@ def check(rc: int) -> int:
 _exit_hook = False
 
 
+generation = 0
+"""Count of library shutdowns in this process: device blocks handed out
+before one (hgx_device_alloc) are gone after it, and their owners must not
+release them again."""
+
+
 def _shutdown_at_exit() -> None:
     """Release the library (request server, streams, device memory) while the
     HIP runtime is still fully up: Python's exit hooks run before the C
     runtime's and the loaded libraries' finalizers."""
-    global _initialised
+    global _initialised, generation
     if _initialised and _lib is not None:
         _initialised = False
+        generation += 1
         _lib.hgx_shutdown()
 
 

@@ -108,7 +108,78 @@ __global__ __launch_bounds__(HGX_NT) void k_as_apply(const u64 *a, u64 na, const
   }
 }
 
+// ---- scans of device limb vectors (sizes and comparisons read back) ----
+constexpr unsigned kScanThreads = 256;
+constexpr unsigned kScanMaxBlocks = 16384;
+
+unsigned scan_blocks(u64 n) {
+  const u64 b = (n + kScanThreads - 1) / kScanThreads;
+  return (unsigned)(b < kScanMaxBlocks ? (b ? b : 1) : kScanMaxBlocks);
+}
+
+// the largest of the workgroup's values V, by thread 0
+__device__ __forceinline__ u64 block_max(u64 v, u64 *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (unsigned s = kScanThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s && sh[threadIdx.x + s] > sh[threadIdx.x]) sh[threadIdx.x] = sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// *res = max(*res, 1 + the largest i < n with a[i] != b[i]) (b zero-extended
+// beyond nb; b = null: compares with zero); *res starts at 0
+__global__ __launch_bounds__(kScanThreads) void k_top_diff(const u64 *a, u64 na, const u64 *b, u64 nb, u64 n,
+                                                           unsigned long long *res) {
+  __shared__ u64 sh[kScanThreads];
+  u64 best = 0;
+  const u64 stride = (u64)gridDim.x * kScanThreads;
+  for (u64 i = (u64)blockIdx.x * kScanThreads + threadIdx.x; i < n; i += stride) {
+    const u64 x = i < na ? a[i] : 0, y = (b && i < nb) ? b[i] : 0;
+    if (x != y) best = i + 1;
+  }
+  const u64 m = block_max(best, sh);
+  if (threadIdx.x == 0 && m) atomicMax(res, (unsigned long long)m);
+}
+
+// res[1] = sign of a - b (magnitudes) from res[0] = 1 + the top differing limb
+__global__ void k_cmp_sign(const u64 *a, u64 na, const u64 *b, u64 nb, unsigned long long *res) {
+  const u64 t = res[0];
+  if (t == 0) {
+    res[1] = 0;
+    return;
+  }
+  const u64 x = t - 1 < na ? a[t - 1] : 0, y = t - 1 < nb ? b[t - 1] : 0;
+  res[1] = x > y ? 1ull : ~0ull;
+}
+
+// out[i] = bits [64 i - s, 64 i - s + 64) of a, i <= n: a * 2^s in n + 1 limbs
+__global__ __launch_bounds__(256) void k_shl(const u64 *a, u64 n, int s, u64 *out) {
+  const u64 stride = (u64)gridDim.x * 256;
+  for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i <= n; i += stride) {
+    const u64 lo = i ? a[i - 1] : 0, hi = i < n ? a[i] : 0;
+    out[i] = s ? (hi << s) | (lo >> (64 - s)) : (i < n ? hi : 0);
+  }
+}
+
 }  // namespace
+
+void hgx_launch_top_diff(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *res) {
+  const u64 n = na > nb ? na : nb;
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, s, res, (u64)2, (u64)0);
+  if (n)
+    hipLaunchKernelGGL(k_top_diff, dim3(scan_blocks(n)), dim3(kScanThreads), 0, s, a, na, b, nb, n,
+                       (unsigned long long *)res);
+}
+
+void hgx_launch_cmp_sign(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *res) {
+  hipLaunchKernelGGL(k_cmp_sign, dim3(1), dim3(1), 0, s, a, na, b, nb, (unsigned long long *)res);
+}
+
+void hgx_launch_shl(hipStream_t s, const u64 *a, u64 n, int bits, u64 *out) {
+  hipLaunchKernelGGL(k_shl, dim3(scan_blocks(n + 1)), dim3(256), 0, s, a, n, bits, out);
+}
 
 void hgx_launch_fill(hipStream_t s, u64 *p, u64 n, u64 v) {
   if (n) hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);

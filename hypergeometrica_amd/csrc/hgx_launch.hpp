@@ -34,6 +34,13 @@ void hgx_launch_splitmix64(dim3 grid, hipStream_t s, u64 *out, u64 n, u64 seed, 
 void hgx_launch_addsub(bool sub, hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *out, u64 n,
                        uint8_t *agg, uint8_t *blkin, u64 *carry_out);
 void hgx_launch_fill(hipStream_t s, u64 *p, u64 n, u64 v);
+// res[0] (zeroed first) = 1 + the top index where a and b differ (b null or
+// shorter: zero-extended); 0 when equal.  cmp_sign then sets res[1] to the
+// sign of a - b (1, 0 or 2^64 - 1).
+void hgx_launch_top_diff(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *res);
+void hgx_launch_cmp_sign(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *res);
+// out[0..n] = a * 2^bits, 0 <= bits < 64 (out must not overlap a)
+void hgx_launch_shl(hipStream_t s, const u64 *a, u64 n, int bits, u64 *out);
 u64 hgx_addsub_blocks(u64 n);
 void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M, u64 rm,
                             u64 rmq);

@@ -2076,6 +2076,135 @@ void tune_cache_save() {
   fclose(f);
 }
 
+// ---- device limb vectors for callers without an allocator of their own ----
+// A caching allocator: a freed block is kept by size class and handed out
+// again once the stream work queued before its release has finished (an
+// event recorded at the release).  Blocks count against the HBM budget;
+// cached ones are returned to the device when an allocation would exceed it.
+struct DevBlock {
+  void *p;
+  size_t bytes;
+  hipEvent_t ev;
+};
+struct DevPool {
+  std::multimap<size_t, DevBlock> cached;  // class size -> released block
+  std::unordered_map<void *, size_t> live;  // handed-out block -> class size
+  u64 *res = nullptr;                        // device words of the scans' results
+  u64 *hres = nullptr;                       // pinned host copy of them
+};
+DevPool g_dev;
+
+// size classes: powers of two up to 1 MiB, then eighths of a power of two
+size_t dev_class(size_t bytes) {
+  bytes = std::max<size_t>(bytes, 256);
+  if (bytes <= ((size_t)1 << 20)) {
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  const int e = 63 - __builtin_clzll((unsigned long long)bytes);
+  const size_t g = (size_t)1 << (e - 3);
+  return (bytes + g - 1) / g * g;
+}
+
+// return cached blocks to the device: those whose work has finished, or all
+// of them after waiting (WAIT)
+void dev_trim(bool wait) {
+  for (auto it = g_dev.cached.begin(); it != g_dev.cached.end();) {
+    DevBlock &b = it->second;
+    if (wait) (void)hipEventSynchronize(b.ev);
+    if (hipEventQuery(b.ev) == hipSuccess) {
+      (void)hipEventDestroy(b.ev);
+      dev_free(b.p, b.bytes);
+      it = g_dev.cached.erase(it);
+    } else {
+      (void)hipGetLastError();
+      ++it;
+    }
+  }
+}
+
+int dev_block_alloc(size_t bytes, void **out) {
+  const size_t c = dev_class(bytes);
+  auto range = g_dev.cached.equal_range(c);
+  for (auto it = range.first; it != range.second; ++it) {
+    if (hipEventQuery(it->second.ev) == hipSuccess) {
+      *out = it->second.p;
+      (void)hipEventDestroy(it->second.ev);
+      g_dev.cached.erase(it);
+      g_dev.live[*out] = c;
+      return HGX_OK;
+    }
+    (void)hipGetLastError();
+  }
+  int rc = dev_alloc(out, c);
+  if (rc == HGX_ENOMEM && !g_dev.cached.empty()) {
+    dev_trim(false);
+    rc = dev_alloc(out, c);
+    if (rc == HGX_ENOMEM) {
+      dev_trim(true);
+      rc = dev_alloc(out, c);
+    }
+  }
+  if (rc) return rc;
+  g_dev.live[*out] = c;
+  return HGX_OK;
+}
+
+int dev_block_release(void *p, hipStream_t s) {
+  auto it = g_dev.live.find(p);
+  if (it == g_dev.live.end()) return fail(HGX_EINVAL, "not a live block of hgx_device_alloc");
+  hipEvent_t ev = nullptr;
+  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  const hipError_t e = hipEventRecord(ev, s);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(ev);
+    return fail(HGX_EDEVICE, std::string("hipEventRecord failed: ") + hipGetErrorString(e));
+  }
+  g_dev.cached.emplace(it->second, DevBlock{p, it->second, ev});
+  g_dev.live.erase(it);
+  return HGX_OK;
+}
+
+// every block, cached or handed out (hgx_shutdown)
+void dev_pool_free() {
+  dev_trim(true);
+  for (auto &kv : g_dev.live) dev_free(kv.first, kv.second);
+  g_dev.live.clear();
+  if (g_dev.res) dev_free(g_dev.res, 256);
+  g_dev.res = nullptr;
+  if (g_dev.hres) (void)hipHostFree(g_dev.hres);
+  g_dev.hres = nullptr;
+}
+
+// the result words of the scans, on first use
+int dev_res_ready() {
+  if (g_dev.res) return HGX_OK;
+  int rc = dev_alloc((void **)&g_dev.res, 256);
+  if (rc) return rc;
+  const hipError_t e = hipHostMalloc((void **)&g_dev.hres, 256, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    dev_free(g_dev.res, 256);
+    g_dev.res = nullptr;
+    return fail(HGX_ENOMEM, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+  }
+  return HGX_OK;
+}
+
+// run the scan kernels on S and read their two result words back
+int scan_read(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, bool sign, u64 out[2]) {
+  int rc = dev_res_ready();
+  if (rc) return rc;
+  hgx_launch_top_diff(s, a, na, b, nb, g_dev.res);
+  if (sign) hgx_launch_cmp_sign(s, a, na, b, nb, g_dev.res);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(g_dev.hres, g_dev.res, 16, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  out[0] = g_dev.hres[0];
+  out[1] = g_dev.hres[1];
+  return HGX_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2160,6 +2289,7 @@ void hgx_shutdown(void) {
   g_kstats.clear();
   for (hipEvent_t e : g_timing_events) (void)hipEventDestroy(e);
   g_timing_events.clear();
+  dev_pool_free();
   for (auto &kv : g_ctx.plans) free_plan(*kv.second);
   g_ctx.plans.clear();
   for (auto &kv : g_ctx.graphs) (void)hipGraphExecDestroy(kv.second);
@@ -2698,6 +2828,191 @@ int hgx_timing_read(hgx_kernel_time *out, int max_entries) {
     n++;
   }
   return n;
+}
+
+
+// ---- device limb vectors (callers without an allocator of their own) ----
+int hgx_device_alloc(size_t nlimbs, uint64_t **d_out) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!d_out) return fail(HGX_EINVAL, "null output");
+  *d_out = nullptr;
+  void *p = nullptr;
+  rc = dev_block_alloc(std::max<size_t>(nlimbs, 1) * 8, &p);
+  if (rc) return rc;
+  *d_out = (uint64_t *)p;
+  return HGX_OK;
+}
+
+int hgx_device_free(uint64_t *d, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  if (!d) return HGX_OK;
+  int rc = check_ready();
+  if (rc) return rc;
+  return dev_block_release(d, stream ? (hipStream_t)stream : g_ctx.stream);
+}
+
+// Pinned host memory moves by one DMA copy, queued under the library lock
+// and awaited outside it (callers on different streams overlap); pageable
+// memory goes through the library's pinned staging, under the lock.
+int hgx_upload(uint64_t *d_dst, const uint64_t *src, size_t nlimbs, void *stream) {
+  hipStream_t s;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+    int rc = check_ready();
+    if (rc) return rc;
+    if (!nlimbs) return HGX_OK;
+    if (!d_dst || !src) return fail(HGX_EINVAL, "null pointer");
+    s = stream ? (hipStream_t)stream : g_ctx.stream;
+    if (!host_pinned(src)) {
+      PinScope pin;
+      rc = h2d(d_dst, src, nlimbs * 8, s);
+      if (rc) return rc;
+      HIPCHK(hipStreamSynchronize(s));  // SRC may be reused once this returns
+      return HGX_OK;
+    }
+    HIPCHK(hipMemcpyAsync(d_dst, src, nlimbs * 8, hipMemcpyHostToDevice, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return HGX_OK;
+}
+
+int hgx_download(uint64_t *dst, const uint64_t *d_src, size_t nlimbs, void *stream) {
+  hipStream_t s;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+    int rc = check_ready();
+    if (rc) return rc;
+    if (!nlimbs) return HGX_OK;
+    if (!dst || !d_src) return fail(HGX_EINVAL, "null pointer");
+    s = stream ? (hipStream_t)stream : g_ctx.stream;
+    if (!host_pinned(dst)) {
+      PinScope pin;
+      rc = d2h(dst, d_src, nlimbs * 8, s);
+      if (!rc) rc = pin_flush();
+      if (rc) return rc;
+      HIPCHK(hipStreamSynchronize(s));
+      return HGX_OK;
+    }
+    HIPCHK(hipMemcpyAsync(dst, d_src, nlimbs * 8, hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return HGX_OK;
+}
+
+int hgx_stream_create(void **stream) {
+  if (!stream) return fail(HGX_EINVAL, "null output");
+  *stream = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  hipStream_t st = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  *stream = st;
+  return HGX_OK;
+}
+
+int hgx_stream_destroy(void *stream) {
+  if (!stream) return HGX_OK;
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  HIPCHK(hipStreamDestroy((hipStream_t)stream));
+  return HGX_OK;
+}
+
+int hgx_device_copy(uint64_t *d_dst, const uint64_t *d_src, size_t nlimbs, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!nlimbs) return HGX_OK;
+  if (!d_dst || !d_src) return fail(HGX_EINVAL, "null pointer");
+  if (overlaps(d_dst, nlimbs, d_src, nlimbs) && d_dst != d_src) return fail(HGX_EINVAL, "overlapping copy");
+  HIPCHK(hipMemcpyAsync(d_dst, d_src, nlimbs * 8, hipMemcpyDeviceToDevice,
+                        stream ? (hipStream_t)stream : g_ctx.stream));
+  return HGX_OK;
+}
+
+int hgx_device_fill(uint64_t *d, size_t nlimbs, uint64_t value, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!nlimbs) return HGX_OK;
+  if (!d) return fail(HGX_EINVAL, "null pointer");
+  hgx_launch_fill(stream ? (hipStream_t)stream : g_ctx.stream, d, nlimbs, value);
+  HIPCHK(hipGetLastError());
+  return HGX_OK;
+}
+
+int hgx_device_significant(const uint64_t *d, size_t nlimbs, size_t *out, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!out || (nlimbs && !d)) return fail(HGX_EINVAL, "null pointer");
+  if (!nlimbs) {
+    *out = 0;
+    return HGX_OK;
+  }
+  u64 r[2];
+  rc = scan_read(stream ? (hipStream_t)stream : g_ctx.stream, d, nlimbs, nullptr, 0, false, r);
+  if (rc) return rc;
+  *out = (size_t)r[0];
+  return HGX_OK;
+}
+
+int hgx_device_compare(const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int *out, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!out || (na && !d_a) || (nb && !d_b)) return fail(HGX_EINVAL, "null pointer");
+  if (!na && !nb) {
+    *out = 0;
+    return HGX_OK;
+  }
+  u64 r[2];
+  rc = scan_read(stream ? (hipStream_t)stream : g_ctx.stream, d_a, na, d_b, nb, true, r);
+  if (rc) return rc;
+  *out = r[1] == 0 ? 0 : (r[1] == 1 ? 1 : -1);
+  return HGX_OK;
+}
+
+int hgx_shl_device(const uint64_t *d_a, size_t n, int bits, uint64_t *d_out, void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (bits < 0 || bits > 63) return fail(HGX_EINVAL, "shift must be 0..63 bits");
+  if (!d_out || (n && !d_a)) return fail(HGX_EINVAL, "null pointer");
+  if (n && overlaps(d_out, n + 1, d_a, n)) return fail(HGX_EINVAL, "the shift's output overlaps its input");
+  hgx_launch_shl(stream ? (hipStream_t)stream : g_ctx.stream, d_a, n, bits, d_out);
+  HIPCHK(hipGetLastError());
+  return HGX_OK;
+}
+
+int hgx_host_alloc(size_t bytes, void **p) {
+  if (!p) return fail(HGX_EINVAL, "null output");
+  *p = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  const hipError_t e = hipHostMalloc(p, std::max<size_t>(bytes, 1), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(HGX_ENOMEM, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+  }
+  return HGX_OK;
+}
+
+int hgx_host_free(void *p) {
+  if (!p) return HGX_OK;
+  HIPCHK(hipHostFree(p));
+  return HGX_OK;
+}
+
+int hgx_synchronize(void *stream) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(stream ? (hipStream_t)stream : g_ctx.stream));
+  return HGX_OK;
 }
 
 }  // extern "C"

@@ -1,100 +1,103 @@
-"""Newton iterations with the mantissas resident in device memory.
+"""Newton iterations with the mantissas resident in device memory, through
+the C ABI alone.
 
 The reference's mpd-reciprocal (src/mpd-reciprocal.lisp:27-73) and
 mpd-inv-sqrt (src/mpd-sqrt.lisp:40-87) chain dependent mpd-* products and
-mpd-+ additions.  Here every mantissa of the iteration stays in HBM: products
-through hgx_mul_device / hgx_sqr_device (the square for x*x, as the
-reference's (eq x y) dispatch), sums and differences through hgx_add_device /
-hgx_sub_device, limb shifts and truncation as device copies and views.  Only
-the scalars the iteration branches on (sizes, the leading limb for the seed,
-magnitude comparisons) are read back.  Every step computes the same integers
-as Context in mpd.py, so the results are bit-identical to it.
-
-Limb vectors are torch int64 tensors on the device holding uint64 bit
-patterns, little endian, trimmed (the top limb non-zero); torch is used for
-device memory and for the stream every call is ordered on.  Import torch
-before libhgx.so is first loaded (this module does): torch ships its own HIP
-runtime, and the process must bind both to it.
+mpd-+ additions.  Here every mantissa of the iteration stays in HBM, in
+blocks of hgx_device_alloc: products through hgx_mul_device / hgx_sqr_device
+(the square for x*x, as the reference's (eq x y) dispatch), sums and
+differences through hgx_add_device / hgx_sub_device, the alignment of
+mpd-+ by hgx_device_fill and hgx_device_copy, mpd-half! by hgx_shl_device,
+mpd-truncate! as a view.  Only the scalars the iteration branches on are
+read back (hgx_device_significant for sizes, hgx_device_compare for the sign
+of a difference, one limb for the seed).  Every call is ordered on the
+library's stream.  The steps are those of Context in mpd.py, on the same
+integers, so the results are bit-identical to it; an SBCL host would make
+the same calls through sb-alien (lisp/hgx.lisp).
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
-import torch  # noqa: F401  (before the library is loaded: see above)
 
 from . import _lib
 from . import mpd as _mpd
 from .mpz import check
 
-_SIGN_BIT = -(1 << 63)
 
+class DBuf:
+    """A block of device limbs (hgx_device_alloc), released (stream-ordered,
+    hgx_device_free) when its last view goes."""
 
-def _torch():
-    import torch
-    return torch
+    __slots__ = ("ptr", "n", "gen")
 
+    def __init__(self, n: int):
+        p = ctypes.c_void_p(0)
+        check(_lib.lib().hgx_device_alloc(max(n, 1), ctypes.byref(p)))
+        self.ptr, self.n, self.gen = int(p.value), n, _lib.generation
 
-def _stream() -> int:
-    """The current torch stream, on which the library calls are queued so
-    that they are ordered with torch's own work.  The null stream would send
-    them to the library's stream instead, unordered with torch's."""
-    h = _torch().cuda.current_stream().cuda_stream
-    if not h:
-        raise RuntimeError("mpd_device runs on a non-default torch stream (DeviceContext sets one)")
-    return h
+    def __del__(self):
+        if getattr(self, "gen", -1) == _lib.generation and _lib._initialised and _lib._lib is not None:
+            _lib._lib.hgx_device_free(self.ptr, None)
 
 
 @dataclass
 class DMPD:
-    """sign * magnitude * 2^(64 * exponent); magnitude = LIMBS (device), empty for zero."""
+    """sign * magnitude * 2^(64 * exponent); the magnitude is SIZE limbs at
+    limb OFF of BUF (trimmed: its top limb is non-zero), none for zero."""
     sign: int
-    limbs: "object"
+    buf: DBuf | None
+    off: int
+    size: int
     exponent: int
 
     @property
-    def size(self) -> int:
-        return int(self.limbs.numel())
+    def ptr(self) -> int:
+        return self.buf.ptr + 8 * self.off
 
 
-def _trim(t):
-    """Drop high zero limbs (reads the index of the top non-zero limb)."""
-    torch = _torch()
-    nz = torch.nonzero(t)
-    if nz.numel() == 0:
-        return t[:0]
-    return t[: int(nz[-1, 0]) + 1]
+def _L():
+    return _lib.lib()
+
+
+def _significant(ptr: int, n: int) -> int:
+    out = ctypes.c_size_t(0)
+    check(_L().hgx_device_significant(ptr, n, ctypes.byref(out), None))
+    return int(out.value)
+
+
+def _make(sign: int, buf: DBuf, n: int, exponent: int) -> DMPD:
+    """A result of up to N limbs in BUF, trimmed (sign 0 when it is zero)."""
+    k = _significant(buf.ptr, n)
+    return DMPD(sign if k else 0, buf if k else None, 0, k, exponent)
 
 
 def to_device(x: _mpd.MPD) -> DMPD:
-    torch = _torch()
     if x.mantissa == 0:
-        return DMPD(0, torch.empty(0, dtype=torch.int64, device="cuda"), x.exponent)
+        return DMPD(0, None, 0, 0, x.exponent)
     m = abs(x.mantissa)
     n = (m.bit_length() + 63) // 64
-    host = np.frombuffer(m.to_bytes(8 * n, "little"), dtype="<i8").copy()
-    return DMPD(1 if x.mantissa > 0 else -1, torch.from_numpy(host).to("cuda"), x.exponent)
+    host = np.frombuffer(m.to_bytes(8 * n, "little"), dtype="<u8").astype(np.uint64)
+    buf = DBuf(n)
+    check(_L().hgx_upload(buf.ptr, host.ctypes.data, n, None))
+    return DMPD(1 if x.mantissa > 0 else -1, buf, 0, n, x.exponent)
 
 
 def to_host(x: DMPD) -> _mpd.MPD:
     if x.sign == 0:
         return _mpd.MPD(0, x.exponent)
-    b = x.limbs.cpu().numpy().astype("<i8").tobytes()
-    return _mpd.MPD(x.sign * int.from_bytes(b, "little"), x.exponent)
+    host = np.empty(x.size, dtype=np.uint64)
+    check(_L().hgx_download(host.ctypes.data, x.ptr, x.size, None))
+    return _mpd.MPD(x.sign * int.from_bytes(host.astype("<u8").tobytes(), "little"), x.exponent)
 
 
-def _mag_cmp(a, b) -> int:
-    """Sign of |a| - |b| for trimmed magnitudes."""
-    torch = _torch()
-    if a.numel() != b.numel():
-        return 1 if a.numel() > b.numel() else -1
-    nz = torch.nonzero(a != b)
-    if nz.numel() == 0:
-        return 0
-    i = int(nz[-1, 0])
-    # unsigned order of the two limbs: flip the sign bits, compare as signed
-    x, y = int(a[i]) ^ _SIGN_BIT, int(b[i]) ^ _SIGN_BIT
-    return 1 if x > y else -1
+def _mag_cmp(a: DMPD, b: DMPD) -> int:
+    """Sign of |a| - |b|."""
+    out = ctypes.c_int(0)
+    check(_L().hgx_device_compare(a.ptr, a.size, b.ptr, b.size, ctypes.byref(out), None))
+    return out.value
 
 
 class DeviceContext:
@@ -104,62 +107,58 @@ class DeviceContext:
         self.truncate = truncate
         self.products = 0
         self.max_product_limbs = 0
-        self.stream = _torch().cuda.Stream()  # every device step of this context runs here
 
     # -- src/mpd.lisp
     def mpd_mul(self, a: DMPD, b: DMPD) -> DMPD:
         """mpd-* (src/mpd.lisp:83-88): mpz-* of the mantissas on the device."""
-        torch = _torch()
         self.products += 1
         self.max_product_limbs = max(self.max_product_limbs, a.size + b.size)
+        e = a.exponent + b.exponent
         if a.sign == 0 or b.sign == 0:
-            return DMPD(0, a.limbs[:0], a.exponent + b.exponent)
-        out = torch.empty(a.size + b.size, dtype=torch.int64, device="cuda")
-        L = _lib.lib()
+            return DMPD(0, None, 0, 0, e)
+        out = DBuf(a.size + b.size)
         if a is b:
-            check(L.hgx_sqr_device(a.limbs.data_ptr(), a.size, out.data_ptr(), _stream()))
+            check(_L().hgx_sqr_device(a.ptr, a.size, out.ptr, None))
         else:
-            check(L.hgx_mul_device(a.limbs.data_ptr(), a.size, b.limbs.data_ptr(), b.size, out.data_ptr(), _stream()))
-        return DMPD(a.sign * b.sign, _trim(out), a.exponent + b.exponent)
+            check(_L().hgx_mul_device(a.ptr, a.size, b.ptr, b.size, out.ptr, None))
+        return _make(a.sign * b.sign, out, a.size + b.size, e)
 
     @staticmethod
-    def _shifted(x: DMPD, k: int):
+    def _shifted(x: DMPD, k: int) -> DMPD:
         """The magnitude times 2^(64 k): K zero limbs below it."""
-        torch = _torch()
         if k == 0:
-            return x.limbs
-        out = torch.zeros(x.size + k, dtype=torch.int64, device="cuda")
-        out[k:] = x.limbs
-        return out
+            return x
+        out = DBuf(x.size + k)
+        check(_L().hgx_device_fill(out.ptr, k, 0, None))
+        check(_L().hgx_device_copy(out.ptr + 8 * k, x.ptr, x.size, None))
+        return DMPD(x.sign, out, 0, x.size + k, x.exponent - k)
 
     def mpd_add(self, a: DMPD, b: DMPD) -> DMPD:
         """mpd-+ (src/mpd.lisp:90-99): align to the smaller exponent, add."""
-        torch = _torch()
         m = min(a.exponent, b.exponent)
         if a.sign == 0:
-            return DMPD(b.sign, self._shifted(b, b.exponent - m), m) if b.sign else DMPD(0, a.limbs, m)
+            return self._shifted(b, b.exponent - m) if b.sign else DMPD(0, None, 0, 0, m)
         if b.sign == 0:
-            return DMPD(a.sign, self._shifted(a, a.exponent - m), m)
+            return self._shifted(a, a.exponent - m)
         x, y = self._shifted(a, a.exponent - m), self._shifted(b, b.exponent - m)
-        L = _lib.lib()
-        n = max(x.numel(), y.numel())
+        n = max(x.size, y.size)
         if a.sign == b.sign:
-            out = torch.empty(n + 1, dtype=torch.int64, device="cuda")
-            check(L.hgx_add_device(x.data_ptr(), x.numel(), y.data_ptr(), y.numel(), out.data_ptr(), _stream()))
-            return DMPD(a.sign, _trim(out), m)
+            out = DBuf(n + 1)
+            check(_L().hgx_add_device(x.ptr, x.size, y.ptr, y.size, out.ptr, None))
+            return _make(a.sign, out, n + 1, m)
         c = _mag_cmp(x, y)
         if c == 0:
-            return DMPD(0, x[:0], m)
+            return DMPD(0, None, 0, 0, m)
         if c < 0:
             x, y, s = y, x, b.sign
         else:
             s = a.sign
-        out = torch.empty(n, dtype=torch.int64, device="cuda")
-        check(L.hgx_sub_device(x.data_ptr(), x.numel(), y.data_ptr(), y.numel(), out.data_ptr(), _stream()))
-        return DMPD(s, _trim(out), m)
+        out = DBuf(n)
+        check(_L().hgx_sub_device(x.ptr, x.size, y.ptr, y.size, out.ptr, None))
+        return _make(s, out, n, m)
 
     def mpd_sub(self, a: DMPD, b: DMPD) -> DMPD:
-        return self.mpd_add(a, DMPD(-b.sign, b.limbs, b.exponent))
+        return self.mpd_add(a, DMPD(-b.sign, b.buf, b.off, b.size, b.exponent))
 
     @staticmethod
     def mpd_oom(x: DMPD) -> tuple[int, bool]:
@@ -171,44 +170,38 @@ class DeviceContext:
     @staticmethod
     def mpd_half(a: DMPD) -> DMPD:
         """mpd-half! (src/mpd.lisp:77-81): mantissa * 2^63, exponent - 1."""
-        torch = _torch()
         if a.sign == 0:
-            return DMPD(0, a.limbs, a.exponent - 1)
-        n = a.size
-        out = torch.empty(n + 1, dtype=torch.int64, device="cuda")
-        # limb i of the result: bit 0 of limb i at bit 63, bits 1..63 of limb i - 1 below it
-        low = torch.where(torch.bitwise_and(a.limbs, 1) == 1, torch.full_like(a.limbs, _SIGN_BIT),
-                          torch.zeros_like(a.limbs))
-        high = torch.bitwise_and(torch.bitwise_right_shift(a.limbs, 1), (1 << 63) - 1)
-        out[:n] = low
-        out[n] = 0
-        out[1:] |= high
-        return DMPD(a.sign, _trim(out), a.exponent - 1)
+            return DMPD(0, None, 0, 0, a.exponent - 1)
+        out = DBuf(a.size + 1)
+        check(_L().hgx_shl_device(a.ptr, a.size, 63, out.ptr, None))
+        return _make(a.sign, out, a.size + 1, a.exponent - 1)
 
     @staticmethod
     def mpd_truncate(a: DMPD, digits: int) -> DMPD:
-        """mpd-truncate! (src/mpd.lisp:56-70): keep the DIGITS high limbs."""
-        s = a.size
-        if digits >= s:
+        """mpd-truncate! (src/mpd.lisp:56-70): keep the DIGITS high limbs (a
+        view; the top limb stays non-zero)."""
+        if digits >= a.size:
             return a
-        d = s - digits
-        return DMPD(a.sign, _trim(a.limbs[d:]), a.exponent + d)
+        d = a.size - digits
+        return DMPD(a.sign, a.buf, a.off + d, digits, a.exponent + d)
 
     def _one(self) -> DMPD:
         return to_device(_mpd.MPD(1, 0))
 
-    def _top(self, a: DMPD) -> int:
-        return int(a.limbs[-1]) & ((1 << 64) - 1)
+    @staticmethod
+    def _top(a: DMPD) -> int:
+        v = np.zeros(1, dtype=np.uint64)
+        check(_L().hgx_download(v.ctypes.data, a.ptr + 8 * (a.size - 1), 1, None))
+        return int(v[0])
 
     # -- src/mpd-reciprocal.lisp
-    def _mpd_reciprocal(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
+    def mpd_reciprocal(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
         """mpd-reciprocal (src/mpd-reciprocal.lisp:27-73) with A and every
         iterate in device memory."""
         digits = max(1, -(-bits // 64))
         da = to_device(a)
-        size = da.size
         r = _mpd._round_div(_mpd.BASE, self._top(da))
-        x = to_device(_mpd.MPD(-r if a.mantissa < 0 else r, -a.exponent - size))
+        x = to_device(_mpd.MPD(-r if a.mantissa < 0 else r, -a.exponent - da.size))
         one = self._one()
         while True:
             h = self.mpd_sub(one, self.mpd_mul(da, x))
@@ -220,7 +213,7 @@ class DeviceContext:
                 x = self.mpd_truncate(x, digits + 2)
 
     # -- src/mpd-sqrt.lisp
-    def _mpd_inv_sqrt(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
+    def mpd_inv_sqrt(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
         """mpd-inv-sqrt (src/mpd-sqrt.lisp:40-87) with A and every iterate in
         device memory; x*x goes to the squaring path."""
         digits = max(1, -(-bits // 64))
@@ -236,47 +229,21 @@ class DeviceContext:
             if self.truncate:
                 x = self.mpd_truncate(x, digits + 2)
 
-    def _mpd_sqrt(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
+    def mpd_sqrt(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
         """mpd-sqrt (src/mpd-sqrt.lisp:89-90)."""
-        inv = to_device(self._mpd_inv_sqrt(a, bits))
+        inv = to_device(self.mpd_inv_sqrt(a, bits))
         return to_host(self.mpd_mul(to_device(a), inv))
 
     # -- src/pi.lisp
-    def _mpd_pi(self, prec_bits: int) -> _mpd.MPD:
+    def mpd_pi(self, prec_bits: int) -> _mpd.MPD:
         """mpd-pi (src/pi.lisp:53-74): binary splitting with batched GPU
         products, then the Newton steps and the final products on the device."""
         from . import series as _series
         guard = prec_bits + _mpd.DIGIT_BITS
         num_terms = 2 + int(guard // _mpd.CHUD_BITS_PER_TERM)
         comp = _series.binary_split(_series.chudnovsky_series(), 0, num_terms)
-        sqrt = to_device(self._mpd_sqrt(_mpd.MPD(10005, 0), guard))
-        recip = to_device(self._mpd_reciprocal(_mpd.MPD(comp.numerator(), 0), guard))
+        sqrt = to_device(self.mpd_sqrt(_mpd.MPD(10005, 0), guard))
+        recip = to_device(self.mpd_reciprocal(_mpd.MPD(comp.numerator(), 0), guard))
         final = self.mpd_mul(sqrt, recip)
         den = to_device(_mpd.MPD(comp.denominator(), 0))
         return to_host(self.mpd_mul(final, den))
-
-    def _on_stream(self, fn, *args):
-        torch = _torch()
-        with torch.cuda.stream(self.stream):
-            out = fn(*args)
-        self.stream.synchronize()
-        return out
-
-    def mpd_reciprocal(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
-        """mpd-reciprocal (src/mpd-reciprocal.lisp:27-73) with A and every
-        iterate in device memory."""
-        return self._on_stream(self._mpd_reciprocal, a, bits)
-
-    def mpd_inv_sqrt(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
-        """mpd-inv-sqrt (src/mpd-sqrt.lisp:40-87) with A and every iterate in
-        device memory; x*x goes to the squaring path."""
-        return self._on_stream(self._mpd_inv_sqrt, a, bits)
-
-    def mpd_sqrt(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
-        """mpd-sqrt (src/mpd-sqrt.lisp:89-90)."""
-        return self._on_stream(self._mpd_sqrt, a, bits)
-
-    def mpd_pi(self, prec_bits: int) -> _mpd.MPD:
-        """mpd-pi (src/pi.lisp:53-74): binary splitting with batched GPU
-        products, then the Newton steps and the final products on the device."""
-        return self._on_stream(self._mpd_pi, prec_bits)

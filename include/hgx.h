@@ -164,6 +164,54 @@ int hgx_add_device(const uint64_t *d_a, size_t na, const uint64_t *d_b,
 int hgx_sub_device(const uint64_t *d_a, size_t na, const uint64_t *d_b,
                    size_t nb, uint64_t *d_out, void *stream);
 
+// Device limb vectors for hosts without a device allocator of their own (an
+// SBCL host through sb-alien): enough to keep the mantissas of a Newton
+// iteration (mpd-reciprocal, src/mpd-reciprocal.lisp:27-73; mpd-inv-sqrt,
+// src/mpd-sqrt.lisp:40-87) in HBM between the products and sums above, as
+// the reference keeps them in its storage vectors (make-storage,
+// src/mpz-ram.lisp:12-17).  hgx_device_alloc hands out NLIMBS limbs of device
+// memory (counted against the HBM budget; the contents are undefined);
+// hgx_device_free releases a block after the work queued on STREAM before it
+// (the library keeps released blocks for reuse, and returns them to the
+// device when the budget needs them).  hgx_shutdown frees every block.
+// hgx_upload / hgx_download copy limbs between host memory (pageable or
+// pinned) and device memory, ordered on STREAM, and return when the copy is
+// complete.  hgx_device_copy (non-overlapping) and hgx_device_fill are
+// queued on STREAM.  hgx_device_significant writes the limb count up to the
+// top non-zero limb (mpz-size after optimize-storage, src/mpz-ram.lisp:77-92)
+// and hgx_device_compare the sign (-1, 0, 1) of |A| - |B| (both read as
+// zero-extended), each after the work queued before it.  hgx_shl_device
+// writes N + 1 limbs of A * 2^BITS (0 <= BITS < 64; the output must not
+// overlap A): mpd-half! (src/mpd.lisp:77-81) is the shift by 63 with the
+// exponent lowered by one.  hgx_host_alloc / hgx_host_free give pinned host
+// memory, which the DMA engines read and write directly.  STREAM null: the
+// library's stream.  No counterpart in the reference beyond the storage
+// layout.
+int hgx_device_alloc(size_t nlimbs, uint64_t **d_out);
+int hgx_device_free(uint64_t *d, void *stream);
+int hgx_upload(uint64_t *d_dst, const uint64_t *src, size_t nlimbs,
+               void *stream);
+int hgx_download(uint64_t *dst, const uint64_t *d_src, size_t nlimbs,
+                 void *stream);
+int hgx_device_copy(uint64_t *d_dst, const uint64_t *d_src, size_t nlimbs,
+                    void *stream);
+int hgx_device_fill(uint64_t *d, size_t nlimbs, uint64_t value, void *stream);
+int hgx_device_significant(const uint64_t *d, size_t nlimbs, size_t *out,
+                           void *stream);
+int hgx_device_compare(const uint64_t *d_a, size_t na, const uint64_t *d_b,
+                       size_t nb, int *out, void *stream);
+int hgx_shl_device(const uint64_t *d_a, size_t n, int bits, uint64_t *d_out,
+                   void *stream);
+int hgx_host_alloc(size_t bytes, void **p);
+int hgx_host_free(void *p);
+int hgx_synchronize(void *stream);
+
+// Streams for such hosts (default priority, non-blocking): work on different
+// streams may overlap on the device, and copies of pinned host memory on
+// them overlap each other.  hgx_stream_destroy waits for the stream first.
+int hgx_stream_create(void **stream);
+int hgx_stream_destroy(void *stream);
+
 // Device SplitMix64 fill for benchmarks and tests:
 // d_out[i] = mix(seed + (i + 1) * 0x9E3779B97F4A7C15); when FORCE_TOP is
 // non-zero the last limb is forced non-zero.

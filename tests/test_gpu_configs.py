@@ -134,3 +134,24 @@ def test_largest_products_checksum(hgx, oracle, n, square):
     # the top four limbs hold the product of the top two limbs of each
     # operand, plus at most the carry of everything below
     assert hi <= top <= hi + (1 << 130)
+
+
+def test_mpd_pi_2e24_bits_three_ways(hgx):
+    # mpd-pi to 2^24 bits with truncating Newton: GPU routings with host and
+    # with HBM iterates bit-identical to GMP-routed host products; pi's hex
+    # digits at the end, middle and start equal BBP's; the leading 2^24 bits
+    # equal MPFR's
+    from hypergeometrica_amd import mpd as M
+    from hypergeometrica_amd import mpd_device as D
+    from tests import _mpfr
+    from tests import _pi_check as PC
+    bits = 1 << 24
+    say("mpd-pi 2^24 bits")
+    r = M.Context(truncate=True).mpd_pi(bits)
+    assert D.DeviceContext(truncate=True).mpd_pi(bits) == r
+    if _gmp.available():
+        assert M.Context(mul=PC.gmp_mul, truncate=True).mpd_pi(bits) == r
+    dist = PC.bbp_check(r.mantissa, r.exponent, PC.bbp_positions(bits))
+    assert PC.bbp_ok(dist), dist
+    if _mpfr.available():
+        assert PC.mpfr_leading_bits_match(r.mantissa, r.exponent, bits)
