@@ -32,10 +32,16 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# hardware queues of this process's HIP runtime (read when it starts, so
+# before the library loads): the request server runs six streams beside the
+# library's, and with HIP's default of four queues streams share queues and
+# wait behind each other's copies (served rate 3.8 -> 4.0e9 limbs/s with 8)
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 from hypergeometrica_amd import workload as W  # noqa: E402
 
 HBM_PEAK = 8.0e12
-VALU_PEAK = 1024 * 2.4e9 / 4  # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz, a wave64 VALU instruction per 4 cycles
+PROBE_OPS = {"shoup": 0, "mont": 1, "addsub": 2, "fwd_butterfly": 3, "inv_butterfly": 4}
 METRIC = "NTT-multiply limbs/s batched vs per-call on GPU; %HBM roofline; p99 latency"
 
 
@@ -57,6 +63,43 @@ def kernel_bytes(name: str, L: int, na: int, nb: int, nops: int) -> float | None
     if name == "crt":
         return 8 * K * L + 8 * (na + nb) + 20 * runs
     return None
+
+
+def alu_probe(_lib, iters: int = 4096) -> dict:
+    """Rates of the library's own modular arithmetic on this GPU
+    (hgx_modmul_probe: register-only, every CU busy) and the shader clock
+    under that load: the integer-ALU ceiling of the transform kernels."""
+    import ctypes
+    out = (ctypes.c_double * 3)()
+    res = {}
+    for name, op in PROBE_OPS.items():
+        _lib.check(_lib.lib().hgx_modmul_probe(op, iters, out))
+        res[name] = {"ops_per_s": out[0], "clock_ghz": round(out[1], 3)}
+    return res
+
+
+def kernel_alu_ops(name: str, L: int, srow: int) -> dict | None:
+    """Modular operations of one launch of a transform kernel of the
+    product of two operands at transform length L (3 primes; column passes
+    of 8 stages, a row pass of SROW stages; the outermost column pass
+    applies tile-factored twiddles, two Shoup products per word, the inner
+    one direct twiddles, one): radix-2 butterflies, twiddle products and
+    the pointwise Montgomery products."""
+    K, half = 3, L // 2
+    if name == "fwd_col_first":
+        return {"fwd_butterfly": K * 2 * 8 * half, "shoup": K * 2 * L * 2}
+    if name == "fwd_col":
+        return {"fwd_butterfly": K * 2 * 8 * half, "shoup": K * 2 * L}
+    if name == "row_mul":
+        return {"fwd_butterfly": K * 2 * srow * half, "inv_butterfly": K * srow * half, "mont": K * L}
+    if name == "inv_col":  # the inner (direct) and outermost (factored) passes, averaged
+        return {"inv_butterfly": K * 8 * half, "shoup": K * L * 3 // 2}
+    return None
+
+
+def alu_floor_s(ops: dict, probe: dict) -> float:
+    """Seconds the operations take at the probe's rates."""
+    return sum(n / probe[op]["ops_per_s"] for op, n in ops.items())
 
 
 def cpu_baseline(sample_bits: int = 22, headline_bits: int = 26) -> dict:
@@ -728,19 +771,26 @@ def main() -> int:
             traffic = json.load(open(tpath)).get("bytes_per_launch")
         except Exception:
             traffic = None
-    # integer-VALU roofline of the same kernel: VALU wave-instructions per launch
-    # (SQ_INSTS_VALU, committed counter profile) over the live launch time,
-    # against one wave-instruction per 4 cycles on each of the 1024 SIMDs
+    # integer-ALU roofline of the same kernel: the time its modular
+    # arithmetic takes at the rates of the register-only probe, over its
+    # measured time
+    probe = alu_probe(_lib)
+    srow = (tuned.get(L_bits, (None,))[0] if tuned else None)
+    srow = (L_bits - 8 * srow) if srow is not None else (L_bits - 16)
+    for name, k in kernels.items():
+        ops = kernel_alu_ops(name, L, srow)
+        if ops:
+            k["alu_floor_ms"] = round(1e3 * alu_floor_s(ops, probe), 4)
+            k["alu_frac"] = round(k["alu_floor_ms"] / k["avg_ms"], 4)
     valu = None
-    vpath = os.path.join(ROOT, "profiles", f"valu_{dom}.json")
-    if os.path.exists(vpath):
-        try:
-            vi = json.load(open(vpath))["valu_wave_insts_per_launch"]
-            peak = VALU_PEAK
-            valu = {"kernel": dom, "wave_insts_per_launch": vi, "achieved": vi / (d["avg_ms"] * 1e-3), "peak": peak,
-                    "unit": "wave-instructions/s", "frac": round(vi / (d["avg_ms"] * 1e-3) / peak, 4)}
-        except Exception:
-            valu = None
+    ops = kernel_alu_ops(dom, L, srow)
+    if ops:
+        fl = alu_floor_s(ops, probe)
+        valu = {"kernel": dom, "bound": "alu", "ops_per_launch": ops, "floor_ms": round(1e3 * fl, 4),
+                "measured_ms": d["avg_ms"], "frac": round(1e3 * fl / d["avg_ms"], 4),
+                "probe": probe, "probe_clock_ghz": probe["fwd_butterfly"]["clock_ghz"],
+                "note": "frac = time of the launch's butterflies, twiddle and pointwise products at the register-only "
+                        "probe's rates (hgx_modmul_probe) over the launch's measured time"}
     roofline = {"bound": "hbm", "kernel": dom,
                 "achieved": round(achieved / 1e9, 1) if achieved else None,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -26,7 +26,7 @@ EXPORTS = (
     "hgx_server_stats", "hgx_mul_device_noalloc",
     "hgx_device_alloc", "hgx_device_free", "hgx_upload", "hgx_download", "hgx_device_copy", "hgx_device_fill",
     "hgx_device_significant", "hgx_device_compare", "hgx_shl_device", "hgx_host_alloc", "hgx_host_free",
-    "hgx_synchronize", "hgx_stream_create", "hgx_stream_destroy",
+    "hgx_synchronize", "hgx_stream_create", "hgx_stream_destroy", "hgx_modmul_probe",
 )
 
 
@@ -114,6 +114,7 @@ def load() -> ctypes.CDLL:
         L.hgx_synchronize.argtypes = [_vp]
         L.hgx_stream_create.argtypes = [ctypes.POINTER(_vp)]
         L.hgx_stream_destroy.argtypes = [_vp]
+        L.hgx_modmul_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 

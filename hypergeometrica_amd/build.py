@@ -27,7 +27,7 @@ HIPCC_FLAGS = [
 
 # kernel families compile in parallel; the runtime holds no device code
 SOURCES = ["hgx_runtime.hip", "hgx_k_col.hip", "hgx_k_row_mul.hip", "hgx_k_row_sqr.hip", "hgx_k_row_tr.hip",
-           "hgx_k_arith.hip"]
+           "hgx_k_arith.hip", "hgx_k_probe.hip"]
 OBJDIR = os.path.join(HERE, "_build")
 
 

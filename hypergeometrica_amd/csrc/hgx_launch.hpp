@@ -42,5 +42,12 @@ void hgx_launch_cmp_sign(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 
 // out[0..n] = a * 2^bits, 0 <= bits < 64 (out must not overlap a)
 void hgx_launch_shl(hipStream_t s, const u64 *a, u64 n, int bits, u64 *out);
 u64 hgx_addsub_blocks(u64 n);
+// register-only probe of one modular operation (0 Shoup product, 1
+// Montgomery product, 2 sum and difference, 3 forward butterfly, 4 inverse
+// butterfly), hgx_probe_chains() chains per thread, ITERS steps each;
+// CLK[0..1] = shader cycles and 100 MHz ticks of workgroup 0's loop
+bool hgx_launch_probe(int op, dim3 grid, hipStream_t s, const Mod &M, u64 w, u64 wq, u64 wm, int iters, u64 *out,
+                      u64 *clk);
+int hgx_probe_chains();
 void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M, u64 rm,
                             u64 rmq);

@@ -3015,4 +3015,47 @@ int hgx_synchronize(void *stream) {
   return HGX_OK;
 }
 
+
+// ---- throughput probe of the modular arithmetic (integer-ALU ceiling) ----
+int hgx_modmul_probe(int op, int iters, double *out) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!out || op < 0 || op > 4 || iters < 1) return fail(HGX_EINVAL, "bad probe arguments");
+  hipDeviceProp_t prop{};
+  HIPCHK(hipGetDeviceProperties(&prop, g_ctx.device));
+  // eight workgroups of 256 threads per CU: every SIMD holds 8 waves
+  const unsigned blocks = (unsigned)prop.multiProcessorCount * 8;
+  const u64 threads = (u64)blocks * 256;
+  void *mem = nullptr;
+  rc = dev_block_alloc(threads * 8 + 256, &mem);
+  if (rc) return rc;
+  u64 *outd = (u64 *)mem, *clk = outd + threads;
+  const Mod M = mod_of(g_ctx.ps, 0);
+  const u64 w = powmod(g_ctx.gen[0], (M.p - 1) >> 20, M.p);  // a 2^20-th root of unity
+  hipStream_t s = g_ctx.stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  (void)hgx_launch_probe(op, dim3(blocks), s, M, w, shoup_companion(w, M.p), to_mont(w, M.p), 16, outd, clk);  // warm-up
+  (void)hipEventRecord(e0, s);
+  (void)hgx_launch_probe(op, dim3(blocks), s, M, w, shoup_companion(w, M.p), to_mont(w, M.p), iters, outd, clk);
+  (void)hipEventRecord(e1, s);
+  u64 cyc[2] = {0, 0};
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(cyc, clk, 16, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)dev_block_release(mem, s);
+  if (e != hipSuccess) return fail(HGX_EDEVICE, std::string("probe failed: ") + hipGetErrorString(e));
+  const double ops = (double)threads * hgx_probe_chains() * iters;
+  out[0] = ops / (ms * 1e-3);
+  out[1] = cyc[1] ? (double)cyc[0] / ((double)cyc[1] / 100e6) / 1e9 : 0.0;
+  out[2] = ms * 1e-3;
+  return HGX_OK;
+}
+
 }  // extern "C"

@@ -52,6 +52,25 @@ HGX_HD u32 mul_u24(u32 a, u32 b) {
 // high 32 bits of a 32 x 32-bit product (v_mul_hi_u32)
 HGX_HD u32 mulhi32(u32 a, u32 b) { return (u32)(((u64)a * b) >> 32); }
 
+// 32 x 32 + 64 -> 65-bit multiply-add: the low 64 bits, and the carry out in
+// CARRY.  v_mad_u64_u32 writes that carry to an SGPR lane mask, a form the
+// compiler does not select from C (it adds the carry with a compare and two
+// selects, or builds a zero-extended addend with a move); hence the asm,
+// whose only inputs and outputs are registers.
+HGX_HD u64 mad64_carry(u32 a, u32 b, u64 c, u32 &carry) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  u64 d;
+  unsigned long long m;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(m) : "v"(a), "v"(b), "v"(c));
+  asm("v_cndmask_b32 %0, 0, 1, %1" : "=v"(carry) : "s"(m));
+  return d;
+#else
+  const unsigned __int128 s = (unsigned __int128)((u64)a * b) + c;
+  carry = (u32)(s >> 64);
+  return (u64)s;
+#endif
+}
+
 // One modulus as the kernels use it: p, its negation 2^64 - p (kept as its
 // own value so that r - p compiles to a single 64-bit add, with no carry
 // through an SGPR), and the Proth factor c of p = c*2^55 + 1.
@@ -111,11 +130,13 @@ HGX_HD u64 mont_mul(u64 a, u64 b, const Mod &M) {
 HGX_HD u64 shoup_mul(u64 a, u64 w, u64 wq, const Mod &M) {
   const u32 a0 = (u32)a, a1 = (u32)(a >> 32), w0 = (u32)w, w1 = (u32)(w >> 32);
   const u32 q0 = (u32)wq, q1 = (u32)(wq >> 32);
-  // q = hi64(a*wq): the middle column is split over two multiply-adds so
-  // that no sum leaves 64 bits
-  const u64 x = (u64)a1 * q0 + mulhi32(a0, q0);
-  const u64 y = (u64)a0 * q1 + (u32)x;
-  const u64 q = (u64)a1 * q1 + (x >> 32) + (y >> 32);
+  // q = hi64(a*wq) = a1 q1 + floor((a1 q0 + a0 q1 + hi32(a0 q0)) / 2^32):
+  // y = a0 q1 + hi32(a0 q0) stays below 2^64, a1 q0 + y may carry into bit
+  // 64, and that carry is bit 32 of the addend of a1 q1
+  const u64 y = (u64)a0 * q1 + mulhi32(a0, q0);
+  u32 cx;
+  const u64 x = mad64_carry(a1, q0, y, cx);
+  const u64 q = (u64)a1 * q1 + ((x >> 32) | ((u64)cx << 32));
   // r = a*w - q*p mod 2^64 with q*p = q + ((q*c mod 2^9) << 55).  The terms
   // that reach only the high word are summed in 32 bits; (q mod 2^24) *
   // (2^24 - c) << 23 is -(q*c mod 2^9) << 23 mod 2^32, so q needs no mask.

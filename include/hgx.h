@@ -282,6 +282,18 @@ typedef struct {
 int hgx_timing_enable(int on);
 int hgx_timing_read(hgx_kernel_time *out, int max_entries);
 
+// Throughput of the library's own modular arithmetic on this GPU, the
+// integer-ALU ceiling the transforms are measured against: OP 0 Shoup
+// product by a fixed factor (the reference's m*/fast2,
+// src/modular-arithmetic.lisp:206-231), 1 Montgomery product, 2 modular sum
+// and difference (m+ / m-, :25-48), 3 forward radix-2 butterfly
+// (src/number-theoretic-transform.lisp:66-81), 4 inverse butterfly
+// (:126-141).  Register-only, 8 independent chains per thread, 8
+// workgroups of 256 threads per CU, ITERS steps per chain.  OUT[0] =
+// operations per second, OUT[1] = shader clock in GHz under that load (cycle
+// counter over the 100 MHz real-time counter), OUT[2] = seconds.
+int hgx_modmul_probe(int op, int iters, double *out);
+
 #ifdef __cplusplus
 }
 #endif
