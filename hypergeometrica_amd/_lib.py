@@ -27,6 +27,7 @@ EXPORTS = (
     "hgx_device_alloc", "hgx_device_free", "hgx_upload", "hgx_download", "hgx_device_copy", "hgx_device_fill",
     "hgx_device_significant", "hgx_device_compare", "hgx_shl_device", "hgx_host_alloc", "hgx_host_free",
     "hgx_synchronize", "hgx_stream_create", "hgx_stream_destroy", "hgx_modmul_probe",
+    "hgx_kernel_set", "hgx_kernel_get",
 )
 
 
@@ -88,6 +89,8 @@ def load() -> ctypes.CDLL:
         _ip = ctypes.POINTER(ctypes.c_int)
         L.hgx_plan_get.argtypes = [ctypes.c_int, _ip, _ip, _ip]
         L.hgx_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.hgx_kernel_set.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.hgx_kernel_get.argtypes = [ctypes.c_int, _ip, _ip]
         L.hgx_staging_limit.argtypes = [_sz]
         L.hgx_sub_device.argtypes = [_vp, _sz, _vp, _sz, _vp, _vp]
         L.hgx_mul_batch_device.argtypes = [ctypes.POINTER(MulDesc), _sz, _vp]

@@ -448,13 +448,19 @@ __global__ __launch_bounds__(kColNT8) void k_inv_col8(PrimeSet ps, ColTw tw, Job
 // 512-thread kernels above
 constexpr unsigned kColSmallBelow = 512;
 
-static bool col_small(dim3 grid, int bbits) { return bbits <= 28 && grid.x * grid.y * grid.z < kColSmallBelow; }
+// the 512-thread tile (8 rows per thread): forced by WIDTH 512, refused by
+// 256, else for launches too small to fill the GPU; blocks above 2^28 words
+// need the 64-bit offsets of the 256-thread kernel
+static bool col_small(dim3 grid, int bbits, int width) {
+  if (bbits > 28 || width == 256) return false;
+  return width == 512 || grid.x * grid.y * grid.z < kColSmallBelow;
+}
 
 static int tw_form(const ColTw &tw) { return tw.D[0] ? TW_DIRECT : (tw.DM[0] ? TW_DIRECT_MONT : TW_FACTORED); }
 
 void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr,
-                        int jt_bits, u64 L, int bbits, int kbase) {
-  if (col_small(grid, bbits)) {
+                        int jt_bits, u64 L, int bbits, int kbase, int width) {
+  if (col_small(grid, bbits, width)) {
     const dim3 b(kColNT8);
     switch (tw_form(tw) + (first ? 3 : 0)) {
       case TW_FACTORED: hipLaunchKernelGGL((k_fwd_col8<false, TW_FACTORED>), grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;
@@ -502,8 +508,8 @@ void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps
 }
 
 void hgx_launch_inv_col(dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr, int jt_bits,
-                        u64 L, int bbits, int kbase) {
-  if (col_small(grid, bbits)) {
+                        u64 L, int bbits, int kbase, int width) {
+  if (col_small(grid, bbits, width)) {
     const dim3 b(kColNT8);
     switch (tw_form(tw)) {
       case TW_FACTORED: hipLaunchKernelGGL(k_inv_col8<TW_FACTORED>, grid, b, 0, s, ps, tw, jr, jt_bits, L, bbits, kbase); break;

@@ -6,10 +6,12 @@
 
 #include "hgx_types.hpp"
 
+// column passes; WIDTH 256 or 512 picks the tile's thread count (512: 8
+// rows per thread, blocks of at most 2^28 words), 0 the default rule
 void hgx_launch_fwd_col(bool first, dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr,
-                        int jt_bits, u64 L, int bbits, int kbase);
+                        int jt_bits, u64 L, int bbits, int kbase, int width = 0);
 void hgx_launch_inv_col(dim3 grid, hipStream_t s, const PrimeSet &ps, const ColTw &tw, const JobRef &jr, int jt_bits,
-                        u64 L, int bbits, int kbase);
+                        u64 L, int bbits, int kbase, int width = 0);
 // blocks of 2^cbits limbs (cbits 9 or 10)
 void hgx_launch_crt(dim3 grid, hipStream_t s, const PrimeSet &ps, const JobRef &jr, int bpj_bits, int cbits, u64 L);
 // row passes; each returns false when SBITS is out of range

@@ -157,10 +157,18 @@ struct Plan {
 // A length's pass structure: NCOL column passes (row pass of n - 8 NCOL
 // bits), the largest block with the direct table (2^DM words), and whether
 // larger blocks (up to 2^kMontDirectMaxBits words) use direct Montgomery tables.
+// COLW and ROWT choose kernels, not tables: the column tile's thread count
+// (256 or 512; 0 the default rule: 512 for launches of fewer than 512
+// workgroups) and the multiply row pass's tile (2^ROWT words, 12, 11 or 10;
+// 0 the default rule: smaller tiles for launches too small to fill the GPU).
 struct Choice {
   int ncol, dm, md;
-  bool operator==(const Choice &o) const { return ncol == o.ncol && dm == o.dm && md == o.md; }
+  int colw = 0, rowt = 0;
+  bool operator==(const Choice &o) const {
+    return ncol == o.ncol && dm == o.dm && md == o.md && colw == o.colw && rowt == o.rowt;
+  }
   bool operator!=(const Choice &o) const { return !(*this == o); }
+  bool same_tables(const Choice &o) const { return ncol == o.ncol && dm == o.dm && md == o.md; }
 };
 constexpr int kMontDirectMaxBits = 28;
 
@@ -834,27 +842,33 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
   } else {
     const int jt = n - 12;  // tiles per job
     const unsigned grid = njobs << jt;
+    const Choice ch = choice_for(n);
     {
       HGX_TIMED("fwd_col_first", s);
       hgx_launch_fwd_col(true, dim3(grid * 3, 1, nops), s, g_ctx.ps, coltw(plan->col[0], 0), jr, jt, L,
-                         plan->col[0].bbits, 0);
+                         plan->col[0].bbits, 0, ch.colw);
     }
     for (int i = 1; i < plan->ncol; i++) {
       HGX_TIMED("fwd_col", s);
       hgx_launch_fwd_col(false, dim3(grid, 3, nops), s, g_ctx.ps, coltw(plan->col[i], 0), jr, jt, L,
-                         plan->col[i].bbits, 0);
+                         plan->col[i].bbits, 0, ch.colw);
     }
     RowParams rp{};
     rp.L = L;
     rp.tw_bits = 12;
     rp.kbase = 0;
-    if (plan->srow <= HGX_HALF_TILE_BITS && 3ull * grid < kHalfTileBelow) {
-      // too few 4096-word tiles to fill the GPU: 2048-word tiles, twice the
-      // workgroups (1024-word tiles, four times, when that is still few),
-      // for a lower latency
+    // the row tile: as chosen for the length, else 2048-word tiles (twice
+    // the workgroups; 1024-word tiles, four times, when that is still few)
+    // when 4096-word tiles are too few to fill the GPU, for a lower latency
+    int tb = 0;
+    if (ch.rowt == HGX_HALF_TILE_BITS || ch.rowt == HGX_HALF_TILE_BITS - 1) {
+      if (plan->srow <= ch.rowt) tb = ch.rowt;
+    } else if (ch.rowt == 0 && plan->srow <= HGX_HALF_TILE_BITS && 3ull * grid < kHalfTileBelow) {
+      tb = (plan->srow < HGX_HALF_TILE_BITS && 6ull * grid < kHalfTileBelow) ? HGX_HALF_TILE_BITS - 1
+                                                                             : HGX_HALF_TILE_BITS;
+    }
+    if (tb) {
       HGX_TIMED(sqr ? "row_sqr_half" : "row_mul_half", s);
-      const int tb = (plan->srow < HGX_HALF_TILE_BITS && 6ull * grid < kHalfTileBelow) ? HGX_HALF_TILE_BITS - 1
-                                                                                        : HGX_HALF_TILE_BITS;
       const int jh = n - tb;
       rp.tw_bits = tb;
       if (!hgx_launch_row_half(!sqr, tb, plan->srow, dim3(njobs << jh, 3), s, g_ctx.ps, rp, jr, jh))
@@ -867,7 +881,7 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
     for (int i = plan->ncol - 1; i >= 0; i--) {
       HGX_TIMED("inv_col", s);
       hgx_launch_inv_col(dim3(grid, 3), s, g_ctx.ps, coltw(plan->col[i], i == 0 ? 2 : 1), jr, jt, L, plan->col[i].bbits,
-                         0);
+                         0, ch.colw);
     }
   }
   {
@@ -2031,13 +2045,15 @@ int set_choice_locked(int bits, const Choice &c) {
                                 std::to_string(max_ncol(bits)) + " column passes");
   if (dm < 12 || dm > 24) return fail(HGX_EINVAL, "direct-table limit must be 12..24 bits");
   if (c.md != 0 && c.md != 1) return fail(HGX_EINVAL, "direct Montgomery flag must be 0 or 1");
+  if (c.colw != 0 && c.colw != 256 && c.colw != 512) return fail(HGX_EINVAL, "column tile threads must be 0, 256 or 512");
+  if (c.rowt != 0 && (c.rowt < 10 || c.rowt > 12)) return fail(HGX_EINVAL, "row tile must be 0 or 10..12 bits");
   // products of this length may still run on any stream with the old tables
   HIPCHK(hipDeviceSynchronize());
   const Choice old = choice_for(bits);
   g_ctx.choice[bits] = c;
   if (old != c) {
-    clear_graphs();
-    drop_plan(plan_key(bits, old));
+    clear_graphs();  // captured launches hold the old kernels
+    if (!old.same_tables(c)) drop_plan(plan_key(bits, old));
   }
   return HGX_OK;
 }
@@ -2055,13 +2071,21 @@ void tune_cache_load() {
   if (!path) return;
   FILE *f = fopen(path, "r");
   if (!f) return;
-  char arch[64];
-  int bits, ncol, dm, md;
+  char line[256];
   const std::string mine = device_arch();
-  while (fscanf(f, "%63s %d %d %d %d", arch, &bits, &ncol, &dm, &md) == 5)
-    if (mine == arch && bits >= 1 && bits <= kMaxBits && ncol >= min_ncol(bits) && ncol <= max_ncol(bits) &&
-        dm >= 12 && dm <= 24 && (md == 0 || md == 1))
-      g_ctx.choice[bits] = Choice{ncol, dm, md};
+  while (fgets(line, sizeof line, f)) {
+    char arch[64];
+    int bits, ncol, dm, md, colw = 0, rowt = 0;
+    const int got = sscanf(line, "%63s %d %d %d %d %d %d", arch, &bits, &ncol, &dm, &md, &colw, &rowt);
+    if ((got == 5 || got == 7) && mine == arch && bits >= 1 && bits <= kMaxBits && ncol >= min_ncol(bits) &&
+        ncol <= max_ncol(bits) && dm >= 12 && dm <= 24 && (md == 0 || md == 1) &&
+        (colw == 0 || colw == 256 || colw == 512) && (rowt == 0 || (rowt >= 10 && rowt <= 12))) {
+      Choice c{ncol, dm, md};
+      c.colw = colw;
+      c.rowt = rowt;
+      g_ctx.choice[bits] = c;
+    }
+  }
   fclose(f);
 }
 
@@ -2072,7 +2096,8 @@ void tune_cache_save() {
   if (!f) return;
   const std::string arch = device_arch();
   for (auto &kv : g_ctx.choice)
-    fprintf(f, "%s %d %d %d %d\n", arch.c_str(), kv.first, kv.second.ncol, kv.second.dm, kv.second.md);
+    fprintf(f, "%s %d %d %d %d %d %d\n", arch.c_str(), kv.first, kv.second.ncol, kv.second.dm, kv.second.md,
+            kv.second.colw, kv.second.rowt);
   fclose(f);
 }
 
@@ -2696,7 +2721,31 @@ int hgx_plan_set(int bits, int ncol, int direct_max_bits, int mont_direct) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   int rc = check_ready();
   if (rc) return rc;
-  return set_choice_locked(bits, Choice{ncol, direct_max_bits, mont_direct});
+  Choice c = choice_for(bits);
+  c.ncol = ncol;
+  c.dm = direct_max_bits;
+  c.md = mont_direct;
+  return set_choice_locked(bits, c);
+}
+
+int hgx_kernel_set(int bits, int col_threads, int row_tile_bits) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (bits < 1 || bits > kMaxBits) return fail(HGX_EINVAL, "transform bits out of range");
+  Choice c = choice_for(bits);
+  c.colw = col_threads;
+  c.rowt = row_tile_bits;
+  return set_choice_locked(bits, c);
+}
+
+int hgx_kernel_get(int bits, int *col_threads, int *row_tile_bits) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  if (bits < 1 || bits > kMaxBits || !col_threads || !row_tile_bits) return fail(HGX_EINVAL, "bad arguments");
+  const Choice c = choice_for(bits);
+  *col_threads = c.colw;
+  *row_tile_bits = c.rowt;
+  return HGX_OK;
 }
 
 int hgx_plan_get(int bits, int *ncol, int *direct_max_bits, int *mont_direct) {
@@ -2714,6 +2763,19 @@ int hgx_plan_get(int bits, int *ncol, int *direct_max_bits, int *mont_direct) {
 // 2^(bits-1)-limb operands (SplitMix64 limbs, device memory) under every
 // pass structure (column-pass counts; direct-table limits 12, 16, 20, 22 where
 // they change a pass) and keep the fastest: best of REPS runs after one warm-up.
+// Lengths from which one product fills the GPU (its launches have at least
+// kHalfTileBelow workgroups): only there are the kernels tuned on a single
+// product; below it the default rule adapts them to the launch size, which
+// for batches is larger than one product's.
+constexpr int kTuneKernelsFrom = 20;
+
+// For every length 2^bits, min_bits <= bits <= max_bits, time a product of
+// two 2^(bits-1)-limb operands (SplitMix64 limbs, device memory) and keep the
+// fastest choice: first the pass structure (column-pass counts; direct-table
+// limits 12, 16, 20, 22 where they change a pass; direct Montgomery tables),
+// then, from 2^kTuneKernelsFrom on, the kernels on it (column tiles of 256 or
+// 512 threads, row tiles of 2^12, 2^11 or 2^10 words), each the best of REPS
+// runs after a warm-up.
 int hgx_autotune(int min_bits, int max_bits, int reps) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   int rc = check_ready();
@@ -2724,7 +2786,7 @@ int hgx_autotune(int min_bits, int max_bits, int reps) {
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  for (int n = std::max(min_bits, kRowMaxBits + 1); n <= max_bits; n++) {
+  for (int n = std::max(min_bits, kRowMaxBits + 1); n <= max_bits && !rc; n++) {
     const u64 L = 1ull << n, half = L / 2;
     // candidates: every column-pass count; the direct-table limits that
     // change which passes use the direct table; direct Montgomery tables
@@ -2744,37 +2806,77 @@ int hgx_autotune(int min_bits, int max_bits, int reps) {
         if (nmont) cand.push_back(Choice{ncol, dm, 1});
       }
     }
-    if (cand.size() < 2) continue;
+    const bool tune_kernels = n >= kTuneKernelsFrom;
+    if (cand.size() < 2 && !tune_kernels) continue;
     rc = ensure(g_ctx.stage, 16 * L);
     if (rc) break;
     u64 *a = (u64 *)g_ctx.stage.ptr, *b = a + half, *out = b + half;
     hgx_launch_splitmix64(dim3((unsigned)((half + 255) / 256)), s, a, half, 0x48475800 + n, 1);
     hgx_launch_splitmix64(dim3((unsigned)((half + 255) / 256)), s, b, half, 0x48475900 + n, 1);
-    float best = 1e30f;
-    Choice pick = choice_for(n);
-    for (auto &c : cand) {
-      rc = set_choice_locked(n, c);
-      if (rc) break;
-      float t = 1e30f;
-      for (int r = 0; r <= reps && !rc; r++) {
+    // best of REPS timed runs of one candidate (run 0 builds the tables)
+    auto time_choice = [&](const Choice &c, float &t) -> int {
+      int r2 = set_choice_locked(n, c);
+      if (r2) return r2;
+      t = 1e30f;
+      for (int r = 0; r <= reps; r++) {
         HIPCHK(hipEventRecord(e0, s));
-        rc = mul_device_locked(a, half, b, half, out, false, s);
+        r2 = mul_device_locked(a, half, b, half, out, false, s);
+        if (r2) return r2;
         HIPCHK(hipEventRecord(e1, s));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (r > 0) t = std::min(t, ms);  // run 0 builds the tables
+        if (r > 0) t = std::min(t, ms);
+      }
+      return HGX_OK;
+    };
+    float best = 1e30f;
+    Choice pick = choice_for(n);
+    pick.colw = pick.rowt = 0;
+    if (cand.size() >= 2) {
+      for (auto &c : cand) {  // setting the next candidate drops this one's tables
+        float t;
+        rc = time_choice(c, t);
+        if (rc) break;
+        if (t < best) {
+          best = t;
+          pick = c;
+        }
       }
       if (rc) break;
-      if (t < best) {
-        best = t;
-        pick = c;
-      }
-      // setting the next candidate drops this one's tables
+    } else {
+      rc = time_choice(pick, best);
+      if (rc) break;
     }
-    if (rc) break;
+    if (tune_kernels) {
+      // kernels on the chosen tables; the default rule stays unless an
+      // explicit choice is clearly faster
+      const int srow = n - 8 * pick.ncol;
+      const bool colw_any = pick.ncol > 0 && n - 8 * (pick.ncol - 1) <= 28;
+      Choice kpick = pick;
+      float kbest = best;
+      for (int colw : {0, 256, 512}) {
+        if (colw && !colw_any) continue;
+        for (int rowt : {0, 12, 11, 10}) {
+          if (rowt && srow > rowt) continue;
+          if (!colw && !rowt) continue;  // the default, timed above
+          Choice c = pick;
+          c.colw = colw;
+          c.rowt = rowt;
+          float t;
+          rc = time_choice(c, t);
+          if (rc) break;
+          if (t < 0.98f * kbest) {
+            kbest = t;
+            kpick = c;
+          }
+        }
+        if (rc) break;
+      }
+      if (rc) break;
+      pick = kpick;
+    }
     rc = set_choice_locked(n, pick);
-    if (rc) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

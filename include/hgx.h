@@ -260,9 +260,23 @@ int hgx_graphs_enable(int on);
 int hgx_plan_set(int bits, int ncol, int direct_max_bits, int mont_direct);
 int hgx_plan_get(int bits, int *ncol, int *direct_max_bits, int *mont_direct);
 
+// Kernels per transform length 2^BITS, on whatever pass structure it has:
+// COL_THREADS 256 or 512 threads per column tile (512: 8 rows per thread and
+// two LDS exchanges, blocks of at most 2^28 words; 256 otherwise) and
+// ROW_TILE_BITS 12, 11 or 10: the multiply row pass on tiles of 2^12 words
+// (radix-16 rounds) or of 2^11 / 2^10 words (radix-8 / radix-4 rounds; only
+// for rows that fit the tile).  0 for either: the default rule, which takes
+// the 512-thread column tile and the smaller row tiles for launches too small
+// to fill the GPU.  No counterpart in the reference.  Same results for every
+// choice.
+int hgx_kernel_set(int bits, int col_threads, int row_tile_bits);
+int hgx_kernel_get(int bits, int *col_threads, int *row_tile_bits);
+
 // Autotuner: for every length 2^bits with MIN_BITS <= bits <= MAX_BITS that
 // offers a choice, time a product of two 2^(bits-1)-limb operands under each
-// pass structure (best of REPS runs after a warm-up) and keep the fastest.
+// pass structure, then (from 2^20 on, where one product fills the GPU) under
+// each kernel choice on the fastest structure (best of REPS runs after a
+// warm-up), and keep the fastest.
 // With HGX_TUNE_CACHE set in the environment the choices are saved to that
 // file and read back by later hgx_init calls on the same GPU architecture.
 int hgx_autotune(int min_bits, int max_bits, int reps);

@@ -511,20 +511,72 @@ def test_autotune_keeps_valid_choices(hgx, oracle, tmp_path, monkeypatch):
     cache = tmp_path / "tune.txt"
     monkeypatch.setenv("HGX_TUNE_CACHE", str(cache))
     saved = {b: _plan(L, b) for b in range(18, 22)}
+    saved_k = {b: _kernels(L, b) for b in range(18, 22)}
     try:
         _lib.check(L.hgx_autotune(18, 21, 1))
         lines = cache.read_text().split("\n")
         rows = [ln.split() for ln in lines if ln.strip()]
-        assert rows and all(len(r) == 5 for r in rows)
+        assert rows and all(len(r) == 7 for r in rows)
         for b in range(18, 22):
             ncol, dm, md = _plan(L, b)
             assert (b - 12 + 7) // 8 <= ncol <= (b - 4) // 8 and 12 <= dm <= 24 and md in (0, 1)
+            colw, rowt = _kernels(L, b)
+            assert colw in (0, 256, 512) and rowt in (0, 10, 11, 12)
+            assert b >= 20 or (colw, rowt) == (0, 0)
             na = (1 << (b - 1)) - 5
             a, bb = oracle.splitmix64(b, na), oracle.splitmix64(b + 50, na + 1)
             assert np.array_equal(hgx.mul_limbs(a, bb), _gmp.mul(a, bb))
     finally:
         for b, c in saved.items():
             _lib.check(L.hgx_plan_set(b, *c))
+            _lib.check(L.hgx_kernel_set(b, *saved_k[b]))
+
+
+def _kernels(L, bits):
+    import ctypes
+    from hypergeometrica_amd import _lib
+    c, r = ctypes.c_int(), ctypes.c_int()
+    _lib.check(L.hgx_kernel_get(bits, ctypes.byref(c), ctypes.byref(r)))
+    return c.value, r.value
+
+
+@pytest.mark.parametrize("bits", [17, 20, 22, 27])
+def test_every_kernel_choice_gives_the_same_products(hgx, oracle, bits):
+    # column tiles of 256 and 512 threads and row tiles of 2^12, 2^11 and
+    # 2^10 words, on each column-pass count: products against GMP (the mod
+    # 2^61-1 residue and exact low limbs at 2^27), squares, and the default
+    # kernels restored after
+    from hypergeometrica_amd import _lib
+    L = _lib.lib()
+    default, default_k = _plan(L, bits), _kernels(L, bits)
+    lo, hi = (bits - 12 + 7) // 8, (bits - 4) // 8
+    try:
+        for ncol in range(lo, hi + 1):
+            srow = bits - 8 * ncol
+            _lib.check(L.hgx_plan_set(bits, ncol, default[1], default[2]))
+            for colw in (256, 512):
+                for rowt in (12, 11, 10):
+                    if srow > rowt:
+                        assert L.hgx_kernel_set(bits, colw, rowt) == 0  # accepted, falls back to 4096-word tiles
+                    _lib.check(L.hgx_kernel_set(bits, colw, rowt))
+                    assert _kernels(L, bits) == (colw, rowt)
+                    say("kernels", bits, ncol, colw, rowt)
+                    a = oracle.splitmix64(bits * 1000 + colw + rowt, (1 << (bits - 1)) - 11)
+                    b = oracle.splitmix64(ncol * 7 + rowt, (1 << (bits - 1)) - 2)
+                    got = hgx.mul_limbs(a, b)
+                    if bits <= 22:
+                        assert np.array_equal(got, _gmp.mul(a, b)), (ncol, colw, rowt)
+                        assert np.array_equal(hgx.sqr_limbs(b), _gmp.mul(b, b.copy())), (ncol, colw, rowt)
+                    else:
+                        from tests import _residue as RES
+                        assert RES.residue(got) == RES.mulmod(RES.residue(a), RES.residue(b)), (ncol, colw, rowt)
+                        lo4 = (oracle.limbs_to_int(a[:4]) * oracle.limbs_to_int(b[:4])) % (1 << 256)
+                        assert oracle.limbs_to_int(got[:4]) == lo4
+        assert L.hgx_kernel_set(bits, 384, 0) == _lib.HGX_EINVAL
+        assert L.hgx_kernel_set(bits, 0, 9) == _lib.HGX_EINVAL
+    finally:
+        _lib.check(L.hgx_plan_set(bits, *default))
+        _lib.check(L.hgx_kernel_set(bits, *default_k))
 
 
 # ---------------------------------------------------------------- unbalanced products
