@@ -41,7 +41,8 @@ os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 from hypergeometrica_amd import workload as W  # noqa: E402
 
 HBM_PEAK = 8.0e12
-PROBE_OPS = {"shoup": 0, "mont": 1, "addsub": 2, "fwd_butterfly": 3, "inv_butterfly": 4}
+PROBE_OPS = {"shoup": 0, "mont": 1, "addsub": 2, "fwd_butterfly": 3, "inv_butterfly": 4, "lds_exchange": 5,
+             "wave_exchange": 6}
 METRIC = "NTT-multiply limbs/s batched vs per-call on GPU; %HBM roofline; p99 latency"
 
 

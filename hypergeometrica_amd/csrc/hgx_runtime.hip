@@ -3123,7 +3123,7 @@ int hgx_modmul_probe(int op, int iters, double *out) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   int rc = check_ready();
   if (rc) return rc;
-  if (!out || op < 0 || op > 4 || iters < 1) return fail(HGX_EINVAL, "bad probe arguments");
+  if (!out || op < 0 || op > 6 || iters < 1) return fail(HGX_EINVAL, "bad probe arguments");
   hipDeviceProp_t prop{};
   HIPCHK(hipGetDeviceProperties(&prop, g_ctx.device));
   // eight workgroups of 256 threads per CU: every SIMD holds 8 waves
@@ -3153,7 +3153,7 @@ int hgx_modmul_probe(int op, int iters, double *out) {
   (void)hipEventDestroy(e1);
   (void)dev_block_release(mem, s);
   if (e != hipSuccess) return fail(HGX_EDEVICE, std::string("probe failed: ") + hipGetErrorString(e));
-  const double ops = (double)threads * hgx_probe_chains() * iters;
+  const double ops = (double)threads * (op >= 5 ? 16 : hgx_probe_chains()) * iters;
   out[0] = ops / (ms * 1e-3);
   out[1] = cyc[1] ? (double)cyc[0] / ((double)cyc[1] / 100e6) / 1e9 : 0.0;
   out[2] = ms * 1e-3;

@@ -302,7 +302,10 @@ int hgx_timing_read(hgx_kernel_time *out, int max_entries);
 // src/modular-arithmetic.lisp:206-231), 1 Montgomery product, 2 modular sum
 // and difference (m+ / m-, :25-48), 3 forward radix-2 butterfly
 // (src/number-theoretic-transform.lisp:66-81), 4 inverse butterfly
-// (:126-141).  Register-only, 8 independent chains per thread, 8
+// (:126-141); 5 and 6 move 16 words per thread between the lanes of groups
+// of 8, as a row pass's rounds hand values on: 5 through LDS with a barrier,
+// 6 inside the wave (ds_swizzle xor stages and VALU selects), counted in
+// words moved.  Register-only, 8 independent chains per thread, 8
 // workgroups of 256 threads per CU, ITERS steps per chain.  OUT[0] =
 // operations per second, OUT[1] = shader clock in GHz under that load (cycle
 // counter over the 100 MHz real-time counter), OUT[2] = seconds.
