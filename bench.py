@@ -442,6 +442,69 @@ def pcie_probe(_lib, mib: int = 1024, reps: int = 3) -> dict:
             "bound_limbs_per_s": r_bi * 1e9 / 8}
 
 
+def copy_replay(_lib, count: int = 3000, seed: int = W.SERVE_SEED, lanes: int = 6) -> dict:
+    """The copies of the serving stream without its kernels: the first COUNT
+    requests' operand uploads and product downloads (sizes of serve_shapes
+    SEED, pinned host memory), LANES threads each on a stream of its own
+    taking the next request in turn (upload a, upload b, download the
+    product), all requests available at once.  What the copy engines give
+    for this size mix at this concurrency: the served rate's ceiling."""
+    import ctypes
+    import threading
+    L_ = _lib.lib()
+    na_l, nb_l = W.serve_shapes(count, seed)
+    nmax = 1 << 24
+    HA, HB = Pinned(_lib, nmax), Pinned(_lib, nmax)
+    streams, bufs = [], []
+    try:
+        for _ in range(lanes):
+            s = ctypes.c_void_p(0)
+            _lib.check(L_.hgx_stream_create(ctypes.byref(s)))
+            streams.append(s)
+            bufs.append((Dev(_lib, nmax), Dev(_lib, nmax), Dev(_lib, 2 * nmax), Pinned(_lib, 2 * nmax)))
+        nxt = [0]
+        lock = threading.Lock()
+        errs = []
+
+        def worker(k):
+            A, B, O, H = bufs[k]
+            s = streams[k]
+            try:
+                while True:
+                    with lock:
+                        i = nxt[0]
+                        nxt[0] += 1
+                    if i >= count:
+                        return
+                    na, nb = na_l[i], nb_l[i]
+                    _lib.check(L_.hgx_upload(A.ptr, HA.ptr, na, s))
+                    _lib.check(L_.hgx_upload(B.ptr, HB.ptr, nb, s))
+                    _lib.check(L_.hgx_download(H.ptr, O.ptr, na + nb, s))
+            except Exception as e:  # noqa: BLE001 -- re-raised below
+                errs.append(e)
+
+        th = [threading.Thread(target=worker, args=(k,)) for k in range(lanes)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        dt = time.perf_counter() - t0
+        if errs:
+            raise errs[0]
+    finally:
+        for s in streams:
+            _lib.check(L_.hgx_stream_destroy(s))
+        for group in bufs:
+            for b in group:
+                b.free()
+        HA.free()
+        HB.free()
+    limbs = sum(na_l) + sum(nb_l)
+    return {"requests": count, "lanes": lanes, "seconds": round(dt, 4), "limbs_per_s": limbs / dt,
+            "GBps_each_way": round(8 * limbs / dt / 1e9, 1)}
+
+
 class _Stream:
     """Pinned host buffers of the serving legs: two random 2^24-limb operand
     buffers (requests take slices of them) with their prefix residues mod
@@ -828,6 +891,8 @@ def main() -> int:
                               search=() if args.no_serve_search else (1500.0, 1200.0, 1000.0, 800.0, 600.0, 400.0))
         serving["pcie"] = pcie_probe(_lib)
         serving["frac_of_pcie_bound"] = round(serving["limbs_per_s"] / serving["pcie"]["bound_limbs_per_s"], 4)
+        serving["copy_replay"] = copy_replay(_lib, lanes=serving["server"]["lanes"])
+        serving["frac_of_copy_replay"] = round(serving["limbs_per_s"] / serving["copy_replay"]["limbs_per_s"], 4)
 
     staged = None
     if not args.no_staged:

@@ -1490,9 +1490,11 @@ struct SrvReq {
   std::string err;
 };
 
-// Six lanes measured fastest with one stream per lane (3.5-3.8e9 limbs/s on a
-// saturated mixed-size stream, against 2.8-3.0e9 with four and 3.3-3.6e9 with
-// eight).  HGX_SERVE_LANES (1..16) and HGX_SERVE_SPLIT_D2H (0/1) choose others.
+// Six lanes measured fastest with one stream per lane (4.0e9 limbs/s on a
+// saturated mixed-size stream with 8 hardware queues, against 3.5e9 with four
+// lanes and 3.6-3.95e9 with eight or twelve); a second stream per lane for the
+// downloads measured 2.1-3.1e9.  HGX_SERVE_LANES (1..16) and
+// HGX_SERVE_SPLIT_D2H (0/1) choose others.
 constexpr int kMaxLanes = 16;
 constexpr int kDefaultLanes = 6;
 constexpr int kDefaultSplitD2H = 0;
