@@ -640,7 +640,9 @@ def serve_stream(_lib, st: _Stream, rate: float, seconds: float, seed: int, chec
             "outputs_checked": checked[0], "mismatches": len(mism), "gmp_checked": len(kept),
             "gmp_mismatches": gmp_bad,
             "server_rounds": stats1["rounds"] - stats0["rounds"],
-            "server_thread_launch_frac": round(launched / wall, 4)}
+            "server_thread_launch_frac": round(launched / wall, 4),
+            "server_thread_retire_frac": round((stats1["retire_s"] - stats0["retire_s"]) / wall, 4),
+            "server_thread_wait_frac": round((stats1["wait_s"] - stats0["wait_s"]) / wall, 4)}
 
 
 def serving_leg(_lib, rate: float = 2000.0, seconds: float = 10.0, slots: int = 32,
@@ -682,6 +684,7 @@ def serving_leg(_lib, rate: float = 2000.0, seconds: float = 10.0, slots: int = 
     main["sustained"] = sustained
     main["rate_search"] = tried
     main["server"] = {"lanes": stats["lanes"], "split_downloads": stats["split_downloads"],
+                      "uploads": stats["uploads"], "downloads": stats["downloads"],
                       "staged": stats["staged"], "failed": stats["failed"]}
     return main
 
@@ -737,6 +740,7 @@ def main() -> int:
     ap.add_argument("--no-square", action="store_true", help="skip the 2^bits-limb square leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-server leg")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the 2000/s serving stream")
+    ap.add_argument("--serve-slots", type=int, default=32, help="pinned product buffers of the serving client")
     ap.add_argument("--no-serve-search", action="store_true", help="skip the sustainable-rate search")
     ap.add_argument("--no-latency", action="store_true", help="skip the 2^16 x 2^16 per-call latency leg")
     ap.add_argument("--no-pi", action="store_true", help="skip the mpd-pi leg")
@@ -887,7 +891,7 @@ def main() -> int:
     serving = None
     if not args.no_serve:
         print("# serving ...", file=sys.stderr, flush=True)
-        serving = serving_leg(_lib, seconds=args.serve_seconds,
+        serving = serving_leg(_lib, seconds=args.serve_seconds, slots=args.serve_slots,
                               search=() if args.no_serve_search else (1500.0, 1200.0, 1000.0, 800.0, 600.0, 400.0))
         serving["pcie"] = pcie_probe(_lib)
         serving["frac_of_pcie_bound"] = round(serving["limbs_per_s"] / serving["pcie"]["bound_limbs_per_s"], 4)

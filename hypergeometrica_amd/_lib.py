@@ -45,8 +45,9 @@ class MulDesc(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
                 ("staged", ctypes.c_uint64), ("queued", ctypes.c_uint64), ("in_flight", ctypes.c_uint64),
-                ("uptime_s", ctypes.c_double), ("launch_s", ctypes.c_double), ("lanes", ctypes.c_int),
-                ("split_downloads", ctypes.c_int)]
+                ("uptime_s", ctypes.c_double), ("launch_s", ctypes.c_double), ("retire_s", ctypes.c_double),
+                ("wait_s", ctypes.c_double), ("lanes", ctypes.c_int),
+                ("split_downloads", ctypes.c_int), ("uploads", ctypes.c_int), ("downloads", ctypes.c_int)]
 
 
 class KernelTime(ctypes.Structure):

@@ -1487,17 +1487,37 @@ struct SrvReq {
   bool ev_ok = true;  // done_ev was recorded after the request's last work
   int lane = -1;      // lane of its round
   int state = 0;      // 0 queued, 1 in flight, 2 done, <0 failed
+  // copy streams: the end of its round's uploads (held by the round's first
+  // request), the end of its kernels, and the download stream once the
+  // download is issued (-1 before)
+  hipEvent_t up_ev = nullptr, kern_ev = nullptr;
+  bool kern_ok = true;     // kern_ev was recorded after its kernels
+  bool lane_done = false;  // its kernels finished: no longer counted on the lane
+  int up = -1, dl = -1;
   std::string err;
 };
 
-// Six lanes measured fastest with one stream per lane (4.0e9 limbs/s on a
-// saturated mixed-size stream with 8 hardware queues, against 3.5e9 with four
-// lanes and 3.6-3.95e9 with eight or twelve); a second stream per lane for the
-// downloads measured 2.1-3.1e9.  HGX_SERVE_LANES (1..16) and
-// HGX_SERVE_SPLIT_D2H (0/1) choose others.
+// Stream layouts.  By default every copy of a round runs on its lane's
+// stream, in order with the round's kernels: six lanes serve a saturated
+// mixed-size stream at 4.0-4.1e9 limbs/s with 32 requests outstanding.
+// HGX_SERVE_COPY_STREAMS=1 moves the copies to streams shared by the lanes:
+// all uploads of a round go to an upload stream, the round's kernels wait for
+// them on a compute lane, and each product's download is issued on the less
+// loaded of two download streams once its kernels are seen to be done.  The
+// copy engines move this mix at 48 GB/s each way with one or two streams per
+// direction, against 38 GB/s when six streams each run uploads and downloads
+// in turn; served, the layout reached 4.3-4.8e9 limbs/s with 128 requests
+// outstanding in some processes and 3.5-3.7e9 in others, so it is not the
+// default.  HGX_SERVE_SPLIT_D2H=1 gives each lane a second stream for its
+// downloads; HGX_SERVE_UPLOADS / HGX_SERVE_DOWNLOADS (1..4) set the copy
+// stream counts and HGX_SERVE_LANES (1..16) the compute lanes.
 constexpr int kMaxLanes = 16;
+constexpr int kMaxCopyStreams = 4;
 constexpr int kDefaultLanes = 6;
 constexpr int kDefaultSplitD2H = 0;
+constexpr int kDefaultCopyStreams = 0;
+constexpr int kDefaultUploads = 1;
+constexpr int kDefaultDownloads = 2;
 constexpr size_t kServeTableBytes = 4 << 20;  // pinned pool of the server's host job tables
 
 struct Lane {
@@ -1524,6 +1544,10 @@ struct Server {
   int nlanes = kDefaultLanes;
   bool split_d2h = kDefaultSplitD2H;
   Lane lanes[kMaxLanes];
+  bool copy_streams = kDefaultCopyStreams;
+  int nup = kDefaultUploads, ndown = kDefaultDownloads;
+  hipStream_t up[kMaxCopyStreams] = {}, down[kMaxCopyStreams] = {};
+  size_t up_pending[kMaxCopyStreams] = {}, down_pending[kMaxCopyStreams] = {};  // bytes not yet seen done
   hipStream_t stage_stream = nullptr;  // the stager's
   PinStage stage_pin;                  // the stager's pinned staging
   std::vector<hipEvent_t> event_pool;
@@ -1531,6 +1555,8 @@ struct Server {
   std::chrono::steady_clock::time_point t0;
   uint64_t completed = 0, failed = 0, rounds = 0, staged_done = 0;
   double launch_s = 0;  // server thread time spent issuing rounds
+  double retire_s = 0;  // ... observing completions
+  double wait_s = 0;    // ... waiting for arrivals or completions
   // failure injection for tests: the FAIL_COPY-th copy the server issues is
   // reported failed without being issued (HGX_SERVE_FAIL_COPY; 0: none)
   uint64_t fail_copy = 0, copies = 0;
@@ -1625,12 +1651,16 @@ bool srv_copy(SrvReq &r, void *dst, const void *src, size_t bytes, hipMemcpyKind
   return false;
 }
 
-// Launch one admission round on lane LI (called without g_srv.mu held).
-// Every failure lands in the error of the requests it concerns; a request
-// with an error is not downloaded, and completes as failed.
-void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, int li) {
+// Launch one admission round on lane LI, its uploads on upload stream UI
+// with copy streams (called without g_srv.mu held).  Every failure lands in
+// the error of the requests it concerns; a request with an error is not
+// downloaded, and completes as failed.  With copy streams the products'
+// downloads are issued later (issue_download), once their kernels are done.
+void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, int li, int ui) {
   Lane &ln = g_srv.lanes[li];
   const hipStream_t s = ln.s;
+  const bool cstreams = g_srv.copy_streams;
+  const hipStream_t cs = cstreams ? g_srv.up[ui] : s;  // copies in
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   // unbalanced products (split into chunk products) after the grouped ones
   std::vector<std::shared_ptr<SrvReq>> splits;
@@ -1651,17 +1681,32 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, int li) {
     u64 *db = r.sqr ? da : (u64 *)(base + align_up(r.d.na * 8));
     u64 *dout = (u64 *)(base + align_up(r.d.na * 8) + (r.sqr ? 0 : align_up(r.d.nb * 8)));
     char *work = (char *)dout + align_up((r.d.na + r.d.nb) * 8);
-    if (srv_copy(r, da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s) && !r.sqr)
-      srv_copy(r, db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
+    if (srv_copy(r, da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, cs) && !r.sqr)
+      srv_copy(r, db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, cs);
     jobs[i] = make_job(da, r.d.na, db, r.d.nb, dout, r.n, r.sqr, work);
   }
-  // one launch sequence per (length, square) group; every job's memory lies
+  // the split products' operands in
+  for (auto &rp : splits) {
+    SrvReq &r = *rp;
+    u64 *da = (u64 *)(g_srv.arena + r.off);
+    u64 *db = (u64 *)(g_srv.arena + r.off + align_up(r.d.na * 8));
+    if (srv_copy(r, da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, cs))
+      srv_copy(r, db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, cs);
+  }
+  // one launch sequence per (length, square) group, its job table (several
+  // jobs) in the arena, uploaded with the operands; every job's memory lies
   // in the arena, so a request whose upload failed only computes garbage
-  size_t g = 0;
-  while (g < batch.size()) {
+  struct Group {
+    size_t g, h;
+    MulJob *dtab;  // null: one launch sequence per job
+    bool table_failed;
+    std::string err;
+  };
+  std::vector<Group> groups;
+  for (size_t g = 0; g < batch.size();) {
     size_t h = g;
     while (h < batch.size() && batch[h]->n == batch[g]->n && batch[h]->sqr == batch[g]->sqr) h++;
-    int rc = HGX_OK;
+    Group gr{g, h, nullptr, false, std::string()};
     size_t toff = (size_t)-1;
     SrvReq &owner = *batch[h - 1];
     if (h - g > 1) {
@@ -1673,30 +1718,60 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, int li) {
       owner.table_off = toff;
       owner.table_bytes = (h - g) * sizeof(MulJob);
       std::copy(jobs.begin() + g, jobs.begin() + h, owner.table_host);
-      MulJob *dtab = (MulJob *)(g_srv.arena + toff);
+      gr.dtab = (MulJob *)(g_srv.arena + toff);
       SrvReq probe;
-      if (!srv_copy(probe, dtab, owner.table_host, owner.table_bytes, hipMemcpyHostToDevice, s)) {
-        rc = fail(HGX_EDEVICE, probe.err);
-      } else {
-        JobRef jr{};
-        jr.table = dtab;
-        rc = enqueue_group(jr, (unsigned)(h - g), batch[g]->n, batch[g]->sqr, s);
+      if (!srv_copy(probe, gr.dtab, owner.table_host, owner.table_bytes, hipMemcpyHostToDevice, cs)) {
+        gr.table_failed = true;
+        gr.err = probe.err;
       }
+    }
+    groups.push_back(gr);
+    g = h;
+  }
+  // with copy streams the lane's kernels wait for the uploads; if the wait
+  // cannot be queued, the server waits for the uploads itself
+  if (cstreams) {
+    hipEvent_t ue = nullptr;  // held by one request of the round
+    for (auto *v : {&batch, &splits})
+      for (auto &r : *v)
+        if (!ue && r->up_ev) ue = r->up_ev;
+    hipError_t e = ue ? hipEventRecord(ue, cs) : hipErrorInvalidHandle;
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, ue, 0);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      if (hipStreamSynchronize(cs) != hipSuccess) {
+        (void)hipGetLastError();
+        for (auto *v : {&batch, &splits})
+          for (auto &r : *v)
+            if (r->err.empty()) r->err = "the uploads of a served round could not be awaited";
+      }
+    }
+  }
+  for (const Group &gr : groups) {
+    if (gr.table_failed) {
+      for (size_t i = gr.g; i < gr.h; i++)
+        if (batch[i]->err.empty()) batch[i]->err = gr.err;
+      continue;
+    }
+    int rc = HGX_OK;
+    if (gr.dtab) {
+      JobRef jr{};
+      jr.table = gr.dtab;
+      rc = enqueue_group(jr, (unsigned)(gr.h - gr.g), batch[gr.g]->n, batch[gr.g]->sqr, s);
     } else {
-      for (size_t i = g; i < h && rc == HGX_OK; i++) {
+      for (size_t i = gr.g; i < gr.h && rc == HGX_OK; i++) {
         JobRef jr{};
         jr.one = jobs[i];
         rc = enqueue_group(jr, 1, batch[i]->n, batch[i]->sqr, s);
       }
     }
     if (rc) {
-      for (size_t i = g; i < h; i++)
+      for (size_t i = gr.g; i < gr.h; i++)
         if (batch[i]->err.empty()) batch[i]->err = g_err;
     }
-    g = h;
   }
-  // the split products: operands in, chunk products and the image sum in the
-  // request's own arena block, so split products on different lanes overlap
+  // the split products: chunk products and the image sum in the request's
+  // own arena block, so split products on different lanes overlap
   for (size_t i = 0; i < splits.size(); i++) {
     SrvReq &r = *splits[i];
     char *base = g_srv.arena + r.off;
@@ -1705,8 +1780,6 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, int li) {
     u64 *dout = (u64 *)(base + align_up(r.d.na * 8) + align_up(r.d.nb * 8));
     char *work = (char *)dout + align_up((r.d.na + r.d.nb) * 8);
     const SplitBytes sb = split_bytes(r.d.na, r.d.nb, r.sp);
-    if (srv_copy(r, da, r.d.a, r.d.na * 8, hipMemcpyHostToDevice, s))
-      srv_copy(r, db, r.d.b, r.d.nb * 8, hipMemcpyHostToDevice, s);
     MulJob *tab;
     {
       std::lock_guard<std::mutex> lk2(g_srv.mu);
@@ -1715,6 +1788,19 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, int li) {
     int rc = split_product_at(da, r.d.na, db, r.d.nb, dout, r.sp, s, work, (uint8_t *)(work + sb.main), tab);
     if (rc && r.err.empty()) r.err = g_err;
     batch.push_back(splits[i]);
+  }
+  if (cstreams) {
+    // the end of every request's kernels; its download comes later
+    for (auto &rp : batch) {
+      SrvReq &r = *rp;
+      const hipError_t e = hipEventRecord(r.kern_ev, s);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        r.kern_ok = false;
+        if (r.err.empty()) r.err = std::string("recording the kernels' end failed: ") + hipGetErrorString(e);
+      }
+    }
+    return;
   }
   // products out and completion events (the product follows the operands in
   // every request's arena block), on the download stream after the kernels
@@ -1741,36 +1827,76 @@ void launch_round(std::vector<std::shared_ptr<SrvReq>> &batch, int li) {
 }
 
 // Return a finished request's memory and complete its ticket (g_srv.mu held).
+size_t product_bytes(const SrvReq &r) { return (r.d.na + r.d.nb) * 8; }
+
+// the request's kernels are done (or failed): it no longer counts on its
+// lane or its upload stream (g_srv.mu held)
+void release_lane_locked(SrvReq &r) {
+  if (r.lane_done) return;
+  r.lane_done = true;
+  g_srv.lanes[r.lane].pending -= r.bytes;
+  if (r.up >= 0) g_srv.up_pending[r.up] -= r.bytes;
+}
+
 void finish_locked(SrvReq &r) {
   arena_free(r.off, r.bytes);
   if (r.table_off != (size_t)-1) arena_free(r.table_off, r.table_bytes);
   if (r.host_off != (size_t)-1) range_free(g_srv.table_free, r.host_off, r.host_bytes);
-  g_srv.lanes[r.lane].pending -= r.bytes;
-  if (r.done_ev) g_srv.event_pool.push_back(r.done_ev);
-  r.done_ev = nullptr;
+  release_lane_locked(r);
+  if (r.dl >= 0) g_srv.down_pending[r.dl] -= product_bytes(r);
+  for (hipEvent_t *e : {&r.done_ev, &r.kern_ev, &r.up_ev}) {
+    if (*e) g_srv.event_pool.push_back(*e);
+    *e = nullptr;
+  }
   r.state = r.err.empty() ? 2 : HGX_EDEVICE;
   (r.state == 2 ? g_srv.completed : g_srv.failed)++;
 }
 
+// Wait for every stream a request may still use (after a failed event).
+void sync_request_streams(const SrvReq &r) {
+  const Lane &ln = g_srv.lanes[r.lane];
+  (void)hipStreamSynchronize(ln.s);
+  if (ln.d != ln.s) (void)hipStreamSynchronize(ln.d);
+  if (r.up >= 0) (void)hipStreamSynchronize(g_srv.up[r.up]);
+  if (r.dl >= 0) (void)hipStreamSynchronize(g_srv.down[r.dl]);
+  (void)hipGetLastError();
+}
+
 // Retire the requests whose work has finished (g_srv.mu held).  A request
-// whose completion event was not recorded or reports an error waits for its
-// lane's streams before its memory is returned, and fails.
-void retire_locked() {
+// whose completion event was not recorded or reports an error waits for the
+// streams it used before its memory is returned, and fails.  With copy
+// streams, a request whose kernels are seen done gets a download stream (the
+// one with fewer bytes in flight) and is returned, for the caller to issue
+// its download (issue_download) without the server's lock.
+std::vector<std::shared_ptr<SrvReq>> retire_locked() {
+  std::vector<std::shared_ptr<SrvReq>> downloads;
   bool any = false;
   for (size_t i = 0; i < g_srv.inflight.size();) {
     SrvReq &r = *g_srv.inflight[i];
-    const hipError_t q = r.ev_ok ? hipEventQuery(r.done_ev) : hipErrorInvalidHandle;
+    const bool kernels = g_srv.copy_streams && r.dl < 0;  // waiting for its kernels
+    const hipError_t q = kernels ? (r.kern_ok ? hipEventQuery(r.kern_ev) : hipErrorInvalidHandle)
+                                 : (r.ev_ok ? hipEventQuery(r.done_ev) : hipErrorInvalidHandle);
     if (q == hipErrorNotReady) {
+      i++;
+      continue;
+    }
+    if (q == hipSuccess && kernels) {
+      release_lane_locked(r);
+      int d = 0;
+      for (int k = 1; k < g_srv.ndown; k++)
+        if (g_srv.down_pending[k] < g_srv.down_pending[d]) d = k;
+      r.dl = d;
+      g_srv.down_pending[d] += product_bytes(r);
+      downloads.push_back(g_srv.inflight[i]);
       i++;
       continue;
     }
     if (q != hipSuccess) {
       (void)hipGetLastError();
-      if (r.err.empty()) r.err = std::string("completion event failed: ") + hipGetErrorString(q);
-      const Lane &ln = g_srv.lanes[r.lane];
-      (void)hipStreamSynchronize(ln.s);
-      if (ln.d != ln.s) (void)hipStreamSynchronize(ln.d);
-      (void)hipGetLastError();
+      if (r.err.empty())
+        r.err = std::string(kernels ? "the kernels' end event failed: " : "completion event failed: ") +
+                hipGetErrorString(q);
+      sync_request_streams(r);
     }
     finish_locked(r);
     g_srv.inflight[i] = g_srv.inflight.back();
@@ -1778,12 +1904,41 @@ void retire_locked() {
     any = true;
   }
   if (any) g_srv.cv.notify_all();
+  return downloads;
+}
+
+// The download of a request whose kernels are done, on its download stream,
+// and its completion event (server thread, without g_srv.mu: a download into
+// pageable memory may take the copy's time).  A request with an error is not
+// downloaded.
+void issue_download(SrvReq &r) {
+  const hipStream_t d = g_srv.down[r.dl];
+  const u64 *dout = (const u64 *)(g_srv.arena + r.off + align_up(r.d.na * 8) + (r.sqr ? 0 : align_up(r.d.nb * 8)));
+  if (r.err.empty()) srv_copy(r, r.d.out, dout, product_bytes(r), hipMemcpyDeviceToHost, d);
+  const hipError_t e = hipEventRecord(r.done_ev, d);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    r.ev_ok = false;
+    if (r.err.empty()) r.err = std::string("recording the completion event failed: ") + hipGetErrorString(e);
+  }
 }
 
 void server_loop() {
   std::unique_lock<std::mutex> lk(g_srv.mu);
+  using clock = std::chrono::steady_clock;
+  auto since = [](clock::time_point t) { return std::chrono::duration<double>(clock::now() - t).count(); };
   while (!g_srv.stop) {
-    retire_locked();
+    const auto tr = clock::now();
+    auto downloads = retire_locked();
+    g_srv.retire_s += since(tr);
+    if (!downloads.empty()) {
+      lk.unlock();
+      const auto t = std::chrono::steady_clock::now();
+      for (auto &r : downloads) issue_download(*r);
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+      lk.lock();
+      g_srv.launch_s += dt;
+    }
     // admit in arrival order while memory lasts, at most a 1/lanes share of
     // the arena per round: under load several rounds are in flight on the
     // lanes, so one round's uploads and downloads overlap another's kernels
@@ -1805,9 +1960,16 @@ void server_loop() {
       if (!round.empty() && round_bytes + r->bytes > share) break;
       const size_t off = arena_alloc(r->bytes);
       if (off == (size_t)-1) break;
+      // completion event; with copy streams also the kernels' end and, for
+      // the round's first request, the end of the round's uploads
       hipEvent_t ev = take_event();
+      hipEvent_t kev = (ev && g_srv.copy_streams) ? take_event() : nullptr;
+      hipEvent_t uev = (kev && round.empty()) ? take_event() : nullptr;
       g_srv.queue.pop_front();
-      if (!ev) {
+      const bool ok = ev && (!g_srv.copy_streams || (kev && (!round.empty() || uev)));
+      if (!ok) {
+        for (hipEvent_t e : {ev, kev, uev})
+          if (e) g_srv.event_pool.push_back(e);
         arena_free(off, r->bytes);
         r->err = "no completion event could be created";
         r->state = HGX_EDEVICE;
@@ -1818,34 +1980,46 @@ void server_loop() {
       r->off = off;
       r->state = 1;
       r->done_ev = ev;
+      r->kern_ev = kev;
+      r->up_ev = uev;
       round.push_back(r);
       round_bytes += r->bytes;
     }
     if (!round.empty()) {
       // the lane with the least work in flight (request sizes span four
       // orders of magnitude: rotating through the lanes would queue small
-      // requests behind a large one while other lanes idle)
-      int li = 0;
+      // requests behind a large one while other lanes idle), and so for the
+      // upload stream
+      int li = 0, ui = -1;
       for (int i = 1; i < g_srv.nlanes; i++)
         if (g_srv.lanes[i].pending < g_srv.lanes[li].pending) li = i;
+      if (g_srv.copy_streams) {
+        ui = 0;
+        for (int i = 1; i < g_srv.nup; i++)
+          if (g_srv.up_pending[i] < g_srv.up_pending[ui]) ui = i;
+      }
       for (auto &r : round) {
         r->lane = li;
+        r->up = ui;
         g_srv.lanes[li].pending += r->bytes;
+        if (ui >= 0) g_srv.up_pending[ui] += r->bytes;
         g_srv.inflight.push_back(r);
       }
       lk.unlock();
       const auto t = std::chrono::steady_clock::now();
-      launch_round(round, li);
+      launch_round(round, li, ui);
       const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
       lk.lock();
       g_srv.launch_s += dt;
       g_srv.rounds++;
       continue;
     }
+    const auto tw = clock::now();
     if (g_srv.inflight.empty())
       g_srv.cv.wait(lk, [] { return g_srv.stop || !g_srv.queue.empty(); });
     else
       g_srv.cv.wait_for(lk, std::chrono::microseconds(20));
+    g_srv.wait_s += since(tw);
   }
 }
 
@@ -1911,6 +2085,12 @@ void server_release() {
     if (ln.k) (void)hipEventDestroy(ln.k);
     ln = Lane{};
   }
+  for (int i = 0; i < kMaxCopyStreams; i++) {
+    if (g_srv.up[i]) (void)hipStreamDestroy(g_srv.up[i]);
+    if (g_srv.down[i]) (void)hipStreamDestroy(g_srv.down[i]);
+    g_srv.up[i] = g_srv.down[i] = nullptr;
+    g_srv.up_pending[i] = g_srv.down_pending[i] = 0;
+  }
   if (g_srv.stage_stream) (void)hipStreamDestroy(g_srv.stage_stream);
   g_srv.stage_stream = nullptr;
   pin_free(g_srv.stage_pin);
@@ -1941,6 +2121,14 @@ int server_start_locked() {
   if (lanes < 1 || lanes > kMaxLanes) return fail(HGX_EINVAL, "HGX_SERVE_LANES must be 1..16");
   g_srv.nlanes = (int)lanes;
   g_srv.split_d2h = env_long("HGX_SERVE_SPLIT_D2H", kDefaultSplitD2H) != 0;
+  g_srv.copy_streams = env_long("HGX_SERVE_COPY_STREAMS", kDefaultCopyStreams) != 0;
+  const long nup = env_long("HGX_SERVE_UPLOADS", kDefaultUploads);
+  const long ndown = env_long("HGX_SERVE_DOWNLOADS", kDefaultDownloads);
+  if (nup < 1 || nup > kMaxCopyStreams || ndown < 1 || ndown > kMaxCopyStreams)
+    return fail(HGX_EINVAL, "HGX_SERVE_UPLOADS and HGX_SERVE_DOWNLOADS must be 1..4");
+  g_srv.nup = (int)nup;
+  g_srv.ndown = (int)ndown;
+  if (g_srv.copy_streams) g_srv.split_d2h = false;  // downloads are on the download streams
   g_srv.fail_copy = (uint64_t)std::max(0L, env_long("HGX_SERVE_FAIL_COPY", 0));
   size_t arena = 0;
   if (const char *e = getenv("HGX_SERVE_ARENA_BYTES")) arena = strtoull(e, nullptr, 10);
@@ -1969,6 +2157,10 @@ int server_start_locked() {
       if (e != hipSuccess) ln.d = ln.s;
     }
   }
+  for (int i = 0; g_srv.copy_streams && i < g_srv.nup && e == hipSuccess; i++)
+    e = hipStreamCreateWithFlags(&g_srv.up[i], hipStreamNonBlocking);
+  for (int i = 0; g_srv.copy_streams && i < g_srv.ndown && e == hipSuccess; i++)
+    e = hipStreamCreateWithFlags(&g_srv.down[i], hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&g_srv.stage_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -1979,7 +2171,7 @@ int server_start_locked() {
     std::lock_guard<std::mutex> lk(g_srv.mu);
     g_srv.stop = false;
     g_srv.completed = g_srv.failed = g_srv.rounds = g_srv.staged_done = g_srv.copies = 0;
-    g_srv.launch_s = 0;
+    g_srv.launch_s = g_srv.retire_s = g_srv.wait_s = 0;
     g_srv.t0 = std::chrono::steady_clock::now();
     g_srv.started = true;
   }
@@ -2006,14 +2198,28 @@ void server_stop(bool at_exit) {
     g_srv.started = false;
     return;
   }
-  for (int i = 0; i < g_srv.nlanes; i++) {
-    (void)hipStreamSynchronize(g_srv.lanes[i].s);
-    if (g_srv.lanes[i].d != g_srv.lanes[i].s) (void)hipStreamSynchronize(g_srv.lanes[i].d);
+  // launched rounds finish: kernels end, the downloads of products whose
+  // kernels are done are issued, downloads end (at most three passes)
+  for (int pass = 0; pass < 3; pass++) {
+    for (int i = 0; i < g_srv.nlanes; i++) {
+      (void)hipStreamSynchronize(g_srv.lanes[i].s);
+      if (g_srv.lanes[i].d != g_srv.lanes[i].s) (void)hipStreamSynchronize(g_srv.lanes[i].d);
+    }
+    for (int i = 0; i < kMaxCopyStreams; i++) {
+      if (g_srv.up[i]) (void)hipStreamSynchronize(g_srv.up[i]);
+      if (g_srv.down[i]) (void)hipStreamSynchronize(g_srv.down[i]);
+    }
+    (void)hipGetLastError();
+    std::vector<std::shared_ptr<SrvReq>> downloads;
+    {
+      std::lock_guard<std::mutex> lk(g_srv.mu);
+      downloads = retire_locked();
+    }
+    if (downloads.empty()) break;
+    for (auto &r : downloads) issue_download(*r);
   }
-  (void)hipGetLastError();
   {
     std::lock_guard<std::mutex> lk(g_srv.mu);
-    retire_locked();
     for (auto &r : g_srv.inflight) {  // an event that never completes
       if (r->err.empty()) r->err = "the request's completion was never observed";
       finish_locked(*r);
@@ -2713,8 +2919,12 @@ int hgx_server_stats(hgx_server_counters *out) {
   out->in_flight = g_srv.inflight.size();
   out->uptime_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - g_srv.t0).count();
   out->launch_s = g_srv.launch_s;
+  out->retire_s = g_srv.retire_s;
+  out->wait_s = g_srv.wait_s;
   out->lanes = g_srv.nlanes;
   out->split_downloads = g_srv.split_d2h ? 1 : 0;
+  out->uploads = g_srv.copy_streams ? g_srv.nup : 0;
+  out->downloads = g_srv.copy_streams ? g_srv.ndown : 0;
   return HGX_OK;
 }
 

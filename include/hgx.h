@@ -139,14 +139,18 @@ int hgx_wait(hgx_ticket ticket);
 // Counters of the request server since it started (all zero before the
 // first hgx_submit): requests completed and failed, admission rounds
 // launched, requests computed host-staged (larger than the server's arena),
-// requests queued and on the device now, seconds since start, and seconds
-// the server thread spent issuing rounds (copies and kernel launches) -- the
+// requests queued and on the device now, seconds since start, seconds the
+// server thread spent issuing rounds (copies and kernel launches) -- the
 // host-side cost per request that a request loop resident on the device
-// would remove.  No counterpart in the reference.
+// would remove --, observing completions and waiting for work, and its
+// streams: compute lanes, whether each lane has a
+// second stream for downloads, and the upload and download streams shared by
+// the lanes (0 when every copy runs on its round's lane).  No counterpart in
+// the reference.
 typedef struct {
   uint64_t completed, failed, rounds, staged, queued, in_flight;
-  double uptime_s, launch_s;
-  int lanes, split_downloads;
+  double uptime_s, launch_s, retire_s, wait_s;
+  int lanes, split_downloads, uploads, downloads;
 } hgx_server_counters;
 
 int hgx_server_stats(hgx_server_counters *out);

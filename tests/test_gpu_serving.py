@@ -64,11 +64,20 @@ def _stream_operands(oracle, count, seed):
     return reqs
 
 
-def test_poisson_stream_equals_solo_and_gmp(hgx, oracle):
+# the server's two stream layouts: every copy on its round's lane (the
+# default), and uploads and downloads on streams shared by the compute lanes
+MODES = {"lane_streams": {}, "copy_streams": {"HGX_SERVE_COPY_STREAMS": 1}}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_poisson_stream_equals_solo_and_gmp(hgx, oracle, server_env, mode):
     # 24 requests of the serving mix, submitted at Poisson arrival times
     # (2000/s) so that they overlap on the device; every product equals its
     # solo run and GMP's
+    from hypergeometrica_amd import _lib
     from hypergeometrica_amd import workload as W
+    if MODES[mode]:
+        server_env(**MODES[mode])
     count, seed = 24, 0x48475804
     reqs = _stream_operands(oracle, count, seed)
     say("serving mix:", [(a.size, b.size) for a, b in reqs])
@@ -88,14 +97,17 @@ def test_poisson_stream_equals_solo_and_gmp(hgx, oracle):
         assert np.array_equal(x, y), i
         want = _gmp.mul(a, b if b is not a else a.copy())
         assert np.array_equal(y, want), (i, a.size, b.size)
+    st = _lib.server_stats()
+    assert (st["uploads"], st["downloads"]) == ((1, 2) if MODES[mode] else (0, 0))
 
 
-def test_failed_copy_fails_only_its_request(hgx, oracle, server_env):
+@pytest.mark.parametrize("mode", list(MODES))
+def test_failed_copy_fails_only_its_request(hgx, oracle, server_env, mode):
     # the second copy the server issues (the first request's second operand)
     # is reported failed: that request fails with the upload error, is not
     # downloaded, and the next one is served correctly
     from hypergeometrica_amd._lib import HGX_EDEVICE
-    server_env(HGX_SERVE_FAIL_COPY=2)
+    server_env(HGX_SERVE_FAIL_COPY=2, **MODES[mode])
     from hypergeometrica_amd import _lib
     L = _lib.lib()
     a = oracle.splitmix64(0x6001, 5000)
