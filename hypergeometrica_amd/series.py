@@ -122,10 +122,12 @@ def fold_partial(series: Series, lower: int, upper: int) -> Partial:
     a, b, p, q = series.a, series.b, series.p, series.q
     P = p(lower)
     Q, B, R = q(lower), b(lower), P * a(lower)
+    BP = B * P  # B * P of the terms so far, kept so that no step multiplies two long numbers
     for k in range(lower + 1, upper):
         pk, qk, bk = p(k), q(k), b(k)
         # COMBINE with the base case (p_k, q_k, b_k, p_k a_k) on the right
-        R = bk * qk * R + B * P * pk * a(k)
+        R = (bk * qk) * R + BP * (pk * a(k))
+        BP *= bk * pk
         P *= pk
         Q *= qk
         B *= bk
