@@ -27,7 +27,7 @@ EXPORTS = (
     "hgx_device_alloc", "hgx_device_free", "hgx_upload", "hgx_download", "hgx_device_copy", "hgx_device_fill",
     "hgx_device_significant", "hgx_device_compare", "hgx_shl_device", "hgx_host_alloc", "hgx_host_free",
     "hgx_synchronize", "hgx_stream_create", "hgx_stream_destroy", "hgx_modmul_probe",
-    "hgx_kernel_set", "hgx_kernel_get",
+    "hgx_kernel_set", "hgx_kernel_get", "hgx_series_split", "hgx_partial_get", "hgx_partial_free",
 )
 
 
@@ -48,6 +48,12 @@ class ServerStats(ctypes.Structure):
                 ("uptime_s", ctypes.c_double), ("launch_s", ctypes.c_double), ("retire_s", ctypes.c_double),
                 ("wait_s", ctypes.c_double), ("lanes", ctypes.c_int),
                 ("split_downloads", ctypes.c_int), ("uploads", ctypes.c_int), ("downloads", ctypes.c_int)]
+
+
+class SeriesDesc(ctypes.Structure):
+    """hgx_series: the four term functions a, b, p, q as polynomials in k."""
+    _fields_ = [("degree", ctypes.c_int * 4), ("coef", (ctypes.c_int64 * 8) * 4), ("at_zero", ctypes.c_int * 4),
+                ("value0", ctypes.c_int64 * 4)]
 
 
 class KernelTime(ctypes.Structure):
@@ -119,6 +125,10 @@ def load() -> ctypes.CDLL:
         L.hgx_stream_create.argtypes = [ctypes.POINTER(_vp)]
         L.hgx_stream_destroy.argtypes = [_vp]
         L.hgx_modmul_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.hgx_series_split.argtypes = [ctypes.POINTER(SeriesDesc), _u64, _u64, ctypes.POINTER(_vp)]
+        L.hgx_partial_get.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_sz), _vp]
+        L.hgx_partial_free.argtypes = [_vp]
+        L.hgx_partial_free.restype = None
         _lib = L
     return _lib
 

@@ -25,9 +25,10 @@ HIPCC_FLAGS = [
     "-Wno-unused-result",
 ]
 
-# kernel families compile in parallel; the runtime holds no device code
+# kernel families compile in parallel; the runtime and the series engine
+# (hgx_series.cpp, host code only) hold no device code
 SOURCES = ["hgx_runtime.hip", "hgx_k_col.hip", "hgx_k_row_mul.hip", "hgx_k_row_sqr.hip", "hgx_k_row_tr.hip",
-           "hgx_k_arith.hip", "hgx_k_probe.hip"]
+           "hgx_k_arith.hip", "hgx_k_probe.hip", "hgx_series.cpp"]
 OBJDIR = os.path.join(HERE, "_build")
 
 
@@ -38,7 +39,7 @@ def build(verbose: bool = False) -> str:
     procs = []
     objs = []
     for src in SOURCES:
-        obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmd = [hipcc, *compile_flags, "-c", "-o", obj, os.path.join(CSRC, src)]
         if verbose:

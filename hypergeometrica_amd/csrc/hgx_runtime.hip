@@ -2440,6 +2440,12 @@ int scan_read(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, bool si
 
 }  // namespace
 
+// for the library's other host translation units (hgx_series.cpp): record
+// an error for hgx_last_error on the calling thread, and whether hgx_init
+// has succeeded
+int hgx_internal_fail(int code, const char *msg) { return fail(code, msg ? msg : ""); }
+bool hgx_internal_ready() { return g_ctx.ready; }
+
 // ===========================================================================
 // C ABI
 // ===========================================================================

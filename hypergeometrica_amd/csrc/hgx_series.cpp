@@ -1,0 +1,534 @@
+// hgx_series.cpp -- binary splitting of a hypergeometric series, one tree
+// height at a time, with the products of each height batched on the GPU.
+//
+// The reference evaluates the partial sum of a series over [lower, upper)
+// depth first (binary-split, src/binary-splitting.lisp:123-133): split at
+// floor((lower + upper) / 2), recurse, COMBINE (:112-121) the two partials
+// with seven products and one sum, and at one term the base case (:100-110).
+// This engine builds the same tree, folds every subtree of at most
+// kFoldTerms terms on the host (the partial of an interval is the same four
+// integers for any split, so a left fold of the base cases gives the
+// recursion's numbers), and then evaluates the tree bottom up, one height at
+// a time.  A height's products go out in two rounds (five products that need
+// only the children, then the two that need round-1 results); a product
+// whose smaller operand has more than *ntt-multiply-threshold* limbs
+// (src/multiply.lisp:9) joins one batched GPU call (hgx_mul_batch), the
+// others are multiplied on the host by schoolbook while the GPU works, as
+// the reference's mpz-* multiplies them (src/multiply.lisp:21-42).  All
+// arithmetic is exact, so every partial equals the reference's.
+//
+// Series terms are polynomials in k with 64-bit signed coefficients and an
+// optional separate value at k = 0, which covers the reference's series
+// (make-e-series, make-exp-series, make-ramanujan-series,
+// make-chudnovsky-series, make-catalan-series, make-apery-series).
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/hgx.h"
+
+typedef uint64_t u64;
+typedef unsigned __int128 u128;
+
+// defined in hgx_runtime.hip
+int hgx_internal_fail(int code, const char *msg);
+bool hgx_internal_ready();
+
+struct hgx_partial {
+  int sign[4];
+  std::vector<u64> mag[4];  // P, Q, B, R: trimmed magnitudes
+};
+
+namespace {
+
+constexpr u64 kFoldTerms = 32;
+constexpr size_t kHostMaxLimbs = 100;  // *ntt-multiply-threshold*, src/multiply.lisp:9
+constexpr int kMaxDeg = 7;
+constexpr int kSmallLimbs = 16;  // term values and products of two of them
+
+// A signed integer of at most kSmallLimbs limbs (a term value).
+struct Small {
+  int s = 0;  // -1, 0, 1
+  int n = 0;  // limbs in use (trimmed)
+  u64 m[kSmallLimbs] = {};
+};
+
+// A signed integer of any size.
+struct Big {
+  int s = 0;
+  std::vector<u64> m;
+};
+
+void trim(std::vector<u64> &m) {
+  while (!m.empty() && m.back() == 0) m.pop_back();
+}
+
+void small_trim(Small &x) {
+  while (x.n > 0 && x.m[x.n - 1] == 0) x.n--;
+  if (x.n == 0) x.s = 0;
+}
+
+int mag_cmp(const u64 *a, size_t na, const u64 *b, size_t nb) {
+  if (na != nb) return na < nb ? -1 : 1;
+  for (size_t i = na; i-- > 0;)
+    if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+  return 0;
+}
+
+// r[0 .. na+nb) = a * b by schoolbook, the outer loop over the shorter
+// operand B (%multiply-storage/schoolboy, src/multiply.lisp:37-42)
+void mag_mul(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *r) {
+  std::fill(r, r + na + nb, 0);
+  for (size_t j = 0; j < nb; j++) {
+    const u64 bj = b[j];
+    if (!bj) continue;
+    u64 carry = 0;
+    u64 *rj = r + j;
+    for (size_t i = 0; i < na; i++) {
+      const u128 t = (u128)a[i] * bj + rj[i] + carry;
+      rj[i] = (u64)t;
+      carry = (u64)(t >> 64);
+    }
+    rj[na] = carry;
+  }
+}
+
+// x += y (magnitudes); x grows as needed
+void mag_add_to(std::vector<u64> &x, const u64 *y, size_t ny) {
+  if (x.size() < ny) x.resize(ny, 0);
+  u64 c = 0;
+  size_t i = 0;
+  for (; i < ny; i++) {
+    const u128 t = (u128)x[i] + y[i] + c;
+    x[i] = (u64)t;
+    c = (u64)(t >> 64);
+  }
+  for (; c && i < x.size(); i++) {
+    x[i] += 1;
+    c = (x[i] == 0);
+  }
+  if (c) x.push_back(1);
+}
+
+// x = |x - y| for |x| >= |y| (magnitudes)
+void mag_sub_from(std::vector<u64> &x, const u64 *y, size_t ny) {
+  u64 b = 0;
+  size_t i = 0;
+  for (; i < ny; i++) {
+    const u64 xi = x[i], t = xi - y[i];
+    const u64 b2 = (xi < y[i]) || (t < b);
+    x[i] = t - b;
+    b = b2;
+  }
+  for (; b && i < x.size(); i++) {
+    b = (x[i] == 0);
+    x[i] -= 1;
+  }
+  trim(x);
+}
+
+// x = x + y for signed x, y (mpz-+)
+void big_add(Big &x, const Big &y) {
+  if (y.s == 0) return;
+  if (x.s == 0) {
+    x = y;
+    return;
+  }
+  if (x.s == y.s) {
+    mag_add_to(x.m, y.m.data(), y.m.size());
+    return;
+  }
+  const int c = mag_cmp(x.m.data(), x.m.size(), y.m.data(), y.m.size());
+  if (c == 0) {
+    x.s = 0;
+    x.m.clear();
+  } else if (c > 0) {
+    mag_sub_from(x.m, y.m.data(), y.m.size());
+  } else {
+    std::vector<u64> t = y.m;
+    mag_sub_from(t, x.m.data(), x.m.size());
+    x.m.swap(t);
+    x.s = y.s;
+  }
+}
+
+// x = x * v for a small v; TMP is scratch
+void big_mul_small(Big &x, const Small &v, std::vector<u64> &tmp) {
+  if (x.s == 0) return;
+  if (v.s == 0) {
+    x.s = 0;
+    x.m.clear();
+    return;
+  }
+  tmp.resize(x.m.size() + v.n);
+  mag_mul(x.m.data(), x.m.size(), v.m, v.n, tmp.data());
+  trim(tmp);
+  x.m.swap(tmp);
+  x.s *= v.s;
+}
+
+Big big_of(const Small &v) {
+  Big b;
+  b.s = v.s;
+  b.m.assign(v.m, v.m + v.n);
+  return b;
+}
+
+// z = x * y for small x, y (the product fits: each factor is a term value)
+int small_mul(const Small &x, const Small &y, Small &z) {
+  z = Small();
+  if (x.s == 0 || y.s == 0) return HGX_OK;
+  if (x.n + y.n > kSmallLimbs) return hgx_internal_fail(HGX_EINVAL, "series term values exceed 1024 bits");
+  u64 r[2 * kSmallLimbs];
+  mag_mul(x.m, x.n, y.m, y.n, r);
+  z.n = x.n + y.n;
+  std::copy(r, r + z.n, z.m);
+  z.s = x.s * y.s;
+  small_trim(z);
+  return HGX_OK;
+}
+
+// v = v * k + c for k >= 0 and a signed 64-bit c (one Horner step)
+int small_horner(Small &v, u64 k, int64_t c) {
+  // v * k
+  u64 carry = 0;
+  for (int i = 0; i < v.n; i++) {
+    const u128 t = (u128)v.m[i] * k + carry;
+    v.m[i] = (u64)t;
+    carry = (u64)(t >> 64);
+  }
+  if (carry) {
+    if (v.n == kSmallLimbs) return hgx_internal_fail(HGX_EINVAL, "series term values exceed 1024 bits");
+    v.m[v.n++] = carry;
+  }
+  small_trim(v);
+  if (c == 0) return HGX_OK;
+  const int cs = c < 0 ? -1 : 1;
+  const u64 cm = c < 0 ? (u64)0 - (u64)c : (u64)c;
+  if (v.s == 0) {
+    v.s = cs;
+    v.n = 1;
+    v.m[0] = cm;
+    return HGX_OK;
+  }
+  if (v.s == cs) {  // magnitudes add
+    u64 cy = cm;
+    for (int i = 0; cy && i < v.n; i++) {
+      v.m[i] += cy;
+      cy = (v.m[i] < cy);
+    }
+    if (cy) {
+      if (v.n == kSmallLimbs) return hgx_internal_fail(HGX_EINVAL, "series term values exceed 1024 bits");
+      v.m[v.n++] = cy;
+    }
+    return HGX_OK;
+  }
+  // opposite signs: |v| - cm or cm - |v|
+  if (v.n == 1 && v.m[0] < cm) {
+    v.m[0] = cm - v.m[0];
+    v.s = cs;
+    return HGX_OK;
+  }
+  u64 b = cm;
+  for (int i = 0; b && i < v.n; i++) {
+    const u64 old = v.m[i];
+    v.m[i] = old - b;
+    b = (old < b);
+  }
+  small_trim(v);
+  return HGX_OK;
+}
+
+// One of the series' four term functions: a polynomial in k, or v0 at k = 0.
+struct Poly {
+  int deg = 0;
+  int64_t c[kMaxDeg + 1] = {};
+  bool at_zero = false;
+  int64_t v0 = 0;
+};
+
+int poly_eval(const Poly &f, u64 k, Small &out) {
+  out = Small();
+  if (k == 0 && f.at_zero) return small_horner(out, 0, f.v0);
+  for (int i = f.deg; i >= 0; i--) {
+    const int rc = small_horner(out, k, f.c[i]);
+    if (rc) return rc;
+  }
+  return HGX_OK;
+}
+
+struct Series {
+  Poly a, b, p, q;
+};
+
+struct Part {
+  Big P, Q, B, R;
+};
+
+// The partial over [lo, up): the base case at LO (binary-split-base-case=1,
+// src/binary-splitting.lisp:100-110) combined with the base cases lo+1, ...
+// one at a time on the right: COMBINE with (p_k, q_k, b_k, p_k a_k) gives
+// R <- b_k q_k R + B P p_k a_k, P <- P p_k, Q <- Q q_k, B <- B b_k.
+int fold(const Series &s, u64 lo, u64 up, Part &out) {
+  Small pk, qk, bk, ak, t1, t2;
+  int rc = poly_eval(s.p, lo, pk);
+  if (!rc) rc = poly_eval(s.q, lo, qk);
+  if (!rc) rc = poly_eval(s.b, lo, bk);
+  if (!rc) rc = poly_eval(s.a, lo, ak);
+  if (!rc) rc = small_mul(pk, ak, t1);
+  if (rc) return rc;
+  out.P = big_of(pk);
+  out.Q = big_of(qk);
+  out.B = big_of(bk);
+  out.R = big_of(t1);
+  Big BP = out.B;  // B * P of the terms so far
+  std::vector<u64> tmp;
+  big_mul_small(BP, pk, tmp);
+  for (u64 k = lo + 1; k < up; k++) {
+    rc = poly_eval(s.p, k, pk);
+    if (!rc) rc = poly_eval(s.q, k, qk);
+    if (!rc) rc = poly_eval(s.b, k, bk);
+    if (!rc) rc = poly_eval(s.a, k, ak);
+    if (!rc) rc = small_mul(bk, qk, t1);
+    if (!rc) rc = small_mul(pk, ak, t2);
+    if (rc) return rc;
+    big_mul_small(out.R, t1, tmp);
+    Big x = BP;
+    big_mul_small(x, t2, tmp);
+    big_add(out.R, x);
+    rc = small_mul(bk, pk, t1);
+    if (rc) return rc;
+    big_mul_small(BP, t1, tmp);
+    big_mul_small(out.P, pk, tmp);
+    big_mul_small(out.Q, qk, tmp);
+    big_mul_small(out.B, bk, tmp);
+  }
+  return HGX_OK;
+}
+
+unsigned worker_count() {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return std::min(hw, 16u);
+}
+
+// fn(i) for i < n on up to worker_count() threads; the first failure wins
+int parallel_for(size_t n, const std::function<int(size_t)> &fn) {
+  if (n == 0) return HGX_OK;
+  const unsigned nt = (unsigned)std::min<size_t>(worker_count(), n);
+  std::atomic<size_t> next{0};
+  std::atomic<int> err{HGX_OK};
+  std::string msg;
+  std::atomic_flag msg_taken = ATOMIC_FLAG_INIT;
+  auto work = [&]() {
+    for (;;) {
+      if (err.load() != HGX_OK) return;
+      const size_t i = next.fetch_add(1);
+      if (i >= n) return;
+      const int rc = fn(i);
+      if (rc != HGX_OK) {
+        int expect = HGX_OK;
+        if (err.compare_exchange_strong(expect, rc) && !msg_taken.test_and_set()) msg = hgx_last_error();
+        return;
+      }
+    }
+  };
+  if (nt == 1) {
+    work();
+  } else {
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+  }
+  if (err.load() != HGX_OK) return hgx_internal_fail(err.load(), msg.c_str());
+  return HGX_OK;
+}
+
+// The products OUT[i] = X[i] * Y[i]: on the GPU in one batched call when the
+// smaller magnitude has more than kHostMaxLimbs limbs, by schoolbook on the
+// host threads otherwise (while the GPU call runs).
+int mul_many(const std::vector<const Big *> &xs, const std::vector<const Big *> &ys, std::vector<Big> &out) {
+  const size_t n = xs.size();
+  out.assign(n, Big());
+  std::vector<size_t> gpu, host;
+  for (size_t i = 0; i < n; i++) {
+    const Big &x = *xs[i], &y = *ys[i];
+    if (x.s == 0 || y.s == 0) continue;
+    out[i].s = x.s * y.s;
+    out[i].m.resize(x.m.size() + y.m.size());
+    (std::min(x.m.size(), y.m.size()) > kHostMaxLimbs ? gpu : host).push_back(i);
+  }
+  int rc_gpu = HGX_OK;
+  std::string gpu_err;
+  std::thread gpu_thread;
+  if (!gpu.empty()) {
+    if (!hgx_internal_ready())
+      return hgx_internal_fail(HGX_ENOTINIT, "hgx_init has not been called (a series product needs the GPU)");
+    gpu_thread = std::thread([&]() {
+      std::vector<hgx_mul_desc> d(gpu.size());
+      for (size_t k = 0; k < gpu.size(); k++) {
+        const size_t i = gpu[k];
+        const Big &x = *xs[i], &y = *ys[i];
+        d[k] = hgx_mul_desc{x.m.data(), x.m.size(), y.m.data(), y.m.size(), out[i].m.data()};
+      }
+      rc_gpu = hgx_mul_batch(d.data(), d.size());
+      if (rc_gpu) gpu_err = hgx_last_error();
+    });
+  }
+  const int rc_host = parallel_for(host.size(), [&](size_t k) {
+    const size_t i = host[k];
+    const Big &x = *xs[i], &y = *ys[i];
+    if (x.m.size() >= y.m.size())
+      mag_mul(x.m.data(), x.m.size(), y.m.data(), y.m.size(), out[i].m.data());
+    else
+      mag_mul(y.m.data(), y.m.size(), x.m.data(), x.m.size(), out[i].m.data());
+    return (int)HGX_OK;
+  });
+  if (gpu_thread.joinable()) gpu_thread.join();
+  if (rc_gpu) return hgx_internal_fail(rc_gpu, gpu_err.c_str());
+  if (rc_host) return rc_host;
+  for (size_t i = 0; i < n; i++) trim(out[i].m);
+  return HGX_OK;
+}
+
+// The split tree down to subtrees of at most kFoldTerms terms (BLOCKS);
+// every inner node as (height, lo, mid, up); returns the height of [lo, up).
+int build_tree(u64 lo, u64 up, std::vector<std::array<u64, 4>> &nodes, std::vector<std::pair<u64, u64>> &blocks) {
+  if (up - lo <= kFoldTerms) {
+    blocks.push_back({lo, up});
+    return 0;
+  }
+  const u64 m = lo + (up - lo) / 2;  // floor((lo + up) / 2) without overflow
+  const int h = 1 + std::max(build_tree(lo, m, nodes, blocks), build_tree(m, up, nodes, blocks));
+  nodes.push_back({(u64)h, lo, m, up});
+  return h;
+}
+
+int parse_poly(const hgx_series &s, int which, Poly &f) {
+  f.deg = s.degree[which];
+  if (f.deg < 0 || f.deg > kMaxDeg) return hgx_internal_fail(HGX_EINVAL, "series polynomial degree must be 0..7");
+  for (int i = 0; i <= f.deg; i++) f.c[i] = s.coef[which][i];
+  f.at_zero = s.at_zero[which] != 0;
+  f.v0 = s.value0[which];
+  return HGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hgx_series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, hgx_partial **out) {
+  if (!desc || !out) return hgx_internal_fail(HGX_EINVAL, "null series or result pointer");
+  *out = nullptr;
+  if (upper <= lower) return hgx_internal_fail(HGX_EINVAL, "the interval [lower, upper) is empty");
+  if (upper > (1ull << 40)) return hgx_internal_fail(HGX_EINVAL, "series terms are limited to k < 2^40");
+  Series s;
+  int rc = parse_poly(*desc, 0, s.a);
+  if (!rc) rc = parse_poly(*desc, 1, s.b);
+  if (!rc) rc = parse_poly(*desc, 2, s.p);
+  if (!rc) rc = parse_poly(*desc, 3, s.q);
+  if (rc) return rc;
+  std::vector<std::array<u64, 4>> nodes;
+  std::vector<std::pair<u64, u64>> blocks;
+  build_tree(lower, upper, nodes, blocks);
+  std::vector<Part> leaf(blocks.size());
+  rc = parallel_for(blocks.size(), [&](size_t i) { return fold(s, blocks[i].first, blocks[i].second, leaf[i]); });
+  if (rc) return rc;
+  // partials by their interval's lower end (children of a node start at lo
+  // and mid, and no two live partials share a lower end)
+  std::vector<std::pair<u64, Part>> live;
+  live.reserve(blocks.size());
+  for (size_t i = 0; i < blocks.size(); i++) live.emplace_back(blocks[i].first, std::move(leaf[i]));
+  leaf.clear();
+  std::sort(live.begin(), live.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  auto find = [&](u64 lo) -> Part & {
+    auto it = std::lower_bound(live.begin(), live.end(), lo, [](const auto &x, u64 v) { return x.first < v; });
+    return it->second;
+  };
+  std::stable_sort(nodes.begin(), nodes.end(), [](const auto &x, const auto &y) { return x[0] < y[0]; });
+  for (size_t h0 = 0; h0 < nodes.size();) {
+    size_t h1 = h0;
+    while (h1 < nodes.size() && nodes[h1][0] == nodes[h0][0]) h1++;
+    const size_t cnt = h1 - h0;
+    // round 1: P_l P_r, Q_l Q_r, B_l B_r, Q_r R_l, P_l R_r (combine, :112-121)
+    std::vector<const Big *> xs, ys;
+    xs.reserve(5 * cnt);
+    ys.reserve(5 * cnt);
+    std::vector<Part *> L(cnt), R(cnt);
+    for (size_t i = 0; i < cnt; i++) {
+      L[i] = &find(nodes[h0 + i][1]);
+      R[i] = &find(nodes[h0 + i][2]);
+      const Part &l = *L[i], &r = *R[i];
+      for (auto pr : {std::make_pair(&l.P, &r.P), std::make_pair(&l.Q, &r.Q), std::make_pair(&l.B, &r.B),
+                      std::make_pair(&r.Q, &l.R), std::make_pair(&l.P, &r.R)}) {
+        xs.push_back(pr.first);
+        ys.push_back(pr.second);
+      }
+    }
+    std::vector<Big> o1;
+    rc = mul_many(xs, ys, o1);
+    if (rc) return rc;
+    // round 2: B_r (Q_r R_l), B_l (P_l R_r)
+    xs.clear();
+    ys.clear();
+    for (size_t i = 0; i < cnt; i++) {
+      xs.push_back(&R[i]->B);
+      ys.push_back(&o1[5 * i + 3]);
+      xs.push_back(&L[i]->B);
+      ys.push_back(&o1[5 * i + 4]);
+    }
+    std::vector<Big> o2;
+    rc = mul_many(xs, ys, o2);
+    if (rc) return rc;
+    // R = B_r Q_r R_l + B_l P_l R_r; the node's partial replaces its left
+    // child's slot (same lower end), the right child's slot is emptied
+    rc = parallel_for(cnt, [&](size_t i) {
+      Part &l = *L[i], &r = *R[i];
+      big_add(o2[2 * i], o2[2 * i + 1]);
+      l.P = std::move(o1[5 * i]);
+      l.Q = std::move(o1[5 * i + 1]);
+      l.B = std::move(o1[5 * i + 2]);
+      l.R = std::move(o2[2 * i]);
+      r = Part();
+      return (int)HGX_OK;
+    });
+    if (rc) return rc;
+    // drop the right children's slots
+    std::vector<u64> gone;
+    gone.reserve(cnt);
+    for (size_t i = 0; i < cnt; i++) gone.push_back(nodes[h0 + i][2]);
+    std::sort(gone.begin(), gone.end());
+    live.erase(std::remove_if(live.begin(), live.end(),
+                              [&](const auto &x) { return std::binary_search(gone.begin(), gone.end(), x.first); }),
+               live.end());
+    h0 = h1;
+  }
+  Part &top = find(lower);
+  hgx_partial *res = new (std::nothrow) hgx_partial;
+  if (!res) return hgx_internal_fail(HGX_ENOMEM, "out of host memory for the series result");
+  Big *src[4] = {&top.P, &top.Q, &top.B, &top.R};
+  for (int i = 0; i < 4; i++) {
+    res->sign[i] = src[i]->s;
+    res->mag[i] = std::move(src[i]->m);
+  }
+  *out = res;
+  return HGX_OK;
+}
+
+int hgx_partial_get(const hgx_partial *x, int which, int *sign, size_t *nlimbs, uint64_t *limbs) {
+  if (!x || which < 0 || which > 3) return hgx_internal_fail(HGX_EINVAL, "bad partial or component index");
+  if (sign) *sign = x->sign[which];
+  if (nlimbs) *nlimbs = x->mag[which].size();
+  if (limbs && !x->mag[which].empty()) std::memcpy(limbs, x->mag[which].data(), 8 * x->mag[which].size());
+  return HGX_OK;
+}
+
+void hgx_partial_free(hgx_partial *x) { delete x; }
+
+}  // extern "C"

@@ -236,12 +236,14 @@ class DeviceContext:
 
     # -- src/pi.lisp
     def mpd_pi(self, prec_bits: int) -> _mpd.MPD:
-        """mpd-pi (src/pi.lisp:53-74): binary splitting with batched GPU
-        products, then the Newton steps and the final products on the device."""
+        """mpd-pi (src/pi.lisp:53-74): binary splitting by the library's
+        native engine (hgx_series_split: host folds and small products in
+        C++, each tree height's large products in one batched GPU call), then
+        the Newton steps and the final products on the device."""
         from . import series as _series
         guard = prec_bits + _mpd.DIGIT_BITS
         num_terms = 2 + int(guard // _mpd.CHUD_BITS_PER_TERM)
-        comp = _series.binary_split(_series.chudnovsky_series(), 0, num_terms)
+        comp = _series.binary_split_native(_series.chudnovsky_series(), 0, num_terms)
         sqrt = to_device(self.mpd_sqrt(_mpd.MPD(10005, 0), guard))
         recip = to_device(self.mpd_reciprocal(_mpd.MPD(comp.numerator(), 0), guard))
         final = self.mpd_mul(sqrt, recip)

@@ -26,12 +26,33 @@ from . import mpz as _mpz
 
 
 @dataclass(frozen=True)
+class Poly:
+    """A term function that is a polynomial in k with integer coefficients
+    (constant term first), or AT_ZERO at k = 0 when given: every series of
+    the reference has this form (src/binary-splitting.lisp:140-231,
+    src/pi.lisp:24-43), and the native engine (hgx_series_split) takes it."""
+    coef: tuple
+    at_zero: int | None = None
+
+    def __call__(self, k: int) -> int:
+        if k == 0 and self.at_zero is not None:
+            return self.at_zero
+        v = 0
+        for c in reversed(self.coef):
+            v = v * k + c
+        return v
+
+
+ONE = Poly((1,))
+
+
+@dataclass(frozen=True)
 class Series:
     """sum_k a(k)/b(k) * p(0)...p(k) / (q(0)...q(k)) (src/binary-splitting.lisp:19-34)."""
-    a: Callable[[int], int] = lambda n: 1
-    b: Callable[[int], int] = lambda n: 1
-    p: Callable[[int], int] = lambda n: 1
-    q: Callable[[int], int] = lambda n: 1
+    a: Callable[[int], int] = ONE
+    b: Callable[[int], int] = ONE
+    p: Callable[[int], int] = ONE
+    q: Callable[[int], int] = ONE
 
 
 @dataclass
@@ -178,16 +199,54 @@ def binary_split(series: Series, lower: int, upper: int, mul_batch=None, fold: i
     return parts[(lower, upper)]
 
 
+def binary_split_native(series: Series, lower: int, upper: int) -> Partial:
+    """binary-split (src/binary-splitting.lisp:123-133) by the library's
+    native engine (hgx_series_split): the same tree evaluated one height at
+    a time in C++, small products on host threads and every product above
+    *ntt-multiply-threshold* limbs of a height in one batched GPU call.  The
+    series' four term functions must be Poly."""
+    import ctypes
+
+    from . import _lib
+    fns = (series.a, series.b, series.p, series.q)
+    if not all(isinstance(f, Poly) for f in fns):
+        raise TypeError("the native engine takes series whose term functions are Poly")
+    desc = _lib.SeriesDesc()
+    for i, f in enumerate(fns):
+        if not 1 <= len(f.coef) <= 8:
+            raise ValueError("term polynomials have degree 0..7")
+        desc.degree[i] = len(f.coef) - 1
+        for j, c in enumerate(f.coef):
+            desc.coef[i][j] = c
+        desc.at_zero[i] = 0 if f.at_zero is None else 1
+        desc.value0[i] = 0 if f.at_zero is None else f.at_zero
+    L = _lib.load() if not _lib._initialised else _lib.lib()
+    res = ctypes.c_void_p(0)
+    _lib.check(L.hgx_series_split(ctypes.byref(desc), lower, upper, ctypes.byref(res)))
+    try:
+        vals = []
+        for w in range(4):
+            sign, n = ctypes.c_int(0), ctypes.c_size_t(0)
+            _lib.check(L.hgx_partial_get(res, w, ctypes.byref(sign), ctypes.byref(n), None))
+            buf = np.empty(n.value, dtype=np.uint64)
+            if n.value:
+                _lib.check(L.hgx_partial_get(res, w, None, None, buf.ctypes.data))
+            vals.append(sign.value * _int(buf))
+    finally:
+        L.hgx_partial_free(res)
+    return Partial(lower, upper, *vals)
+
+
 # ------------------------------------------------------------------ series
 
 def e_series() -> Series:
     """make-e-series (src/binary-splitting.lisp:150-154)."""
-    return Series(q=lambda n: 1 if n == 0 else n)
+    return Series(q=Poly((0, 1), at_zero=1))
 
 
 def exp_series(num: int, den: int) -> Series:
     """make-exp-series for x = num/den (src/binary-splitting.lisp:140-148)."""
-    return Series(p=lambda n: 1 if n == 0 else num, q=lambda n: 1 if n == 0 else n * den)
+    return Series(p=Poly((num,), at_zero=1), q=Poly((0, den), at_zero=1))
 
 
 RAMA_A, RAMA_B, RAMA_C = 1103, 26390, 396
@@ -195,9 +254,9 @@ RAMA_A, RAMA_B, RAMA_C = 1103, 26390, 396
 
 def ramanujan_series() -> Series:
     """make-ramanujan-series (src/binary-splitting.lisp:167-184)."""
-    return Series(a=lambda n: RAMA_A + n * RAMA_B,
-                  p=lambda n: 1 if n == 0 else -3 + n * (22 + n * (-48 + n * 32)),
-                  q=lambda n: 1 if n == 0 else n ** 3 * (RAMA_C ** 4 // 8))
+    return Series(a=Poly((RAMA_A, RAMA_B)),
+                  p=Poly((-3, 22, -48, 32), at_zero=1),
+                  q=Poly((0, 0, 0, RAMA_C ** 4 // 8), at_zero=1))
 
 
 CHUD_A, CHUD_B, CHUD_C = 13591409, 545140134, 640320
@@ -206,21 +265,22 @@ CHUD_PREFACTOR = 10  # computes pi/10 (src/pi.lisp:14-20)
 
 def chudnovsky_series() -> Series:
     """make-chudnovsky-series (src/pi.lisp:24-43)."""
-    return Series(a=lambda n: CHUD_A * CHUD_PREFACTOR + n * CHUD_B * CHUD_PREFACTOR,
-                  b=lambda n: (8 * CHUD_C) // 12,
-                  p=lambda n: 1 if n == 0 else 5 + n * (-46 + n * (108 + n * -72)),
-                  q=lambda n: 1 if n == 0 else n ** 3 * (CHUD_C ** 3 // 24))
+    return Series(a=Poly((CHUD_A * CHUD_PREFACTOR, CHUD_B * CHUD_PREFACTOR)),
+                  b=Poly(((8 * CHUD_C) // 12,)),
+                  p=Poly((5, -46, 108, -72), at_zero=1),
+                  q=Poly((0, 0, 0, CHUD_C ** 3 // 24), at_zero=1))
 
 
 def catalan_series() -> Series:
     """make-catalan-series (src/binary-splitting.lisp:203-213)."""
-    return Series(b=lambda n: 1 + 2 * n, p=lambda n: 1 if n == 0 else n, q=lambda n: 1 if n == 0 else 2 + 4 * n)
+    return Series(b=Poly((1, 2)), p=Poly((0, 1), at_zero=1), q=Poly((2, 4), at_zero=1))
 
 
 def apery_series() -> Series:
-    """make-apery-series (src/binary-splitting.lisp:218-231)."""
-    return Series(a=lambda n: 77 + n * (250 + n * 205), b=lambda n: 2,
-                  p=lambda n: 1 if n == 0 else -(n ** 5), q=lambda n: 32 * (1 + 2 * n) ** 5)
+    """make-apery-series (src/binary-splitting.lisp:218-231): q(n) = 32 (1 + 2n)^5 expanded."""
+    return Series(a=Poly((77, 250, 205)), b=Poly((2,)),
+                  p=Poly((0, 0, 0, 0, 0, -1), at_zero=1),
+                  q=Poly((32, 320, 1280, 2560, 2560, 1024)))
 
 
 CHUD_DECIMALS_PER_TERM = 14.181647462725477  # log10(151931373056000), src/pi.lisp:10

@@ -315,6 +315,40 @@ int hgx_timing_read(hgx_kernel_time *out, int max_entries);
 // counter over the 100 MHz real-time counter), OUT[2] = seconds.
 int hgx_modmul_probe(int op, int iters, double *out);
 
+// Binary splitting of a hypergeometric series
+//     sum over k >= 0 of a(k)/b(k) * p(0)...p(k) / (q(0)...q(k))
+// (the series struct, src/binary-splitting.lisp:19-34) over [LOWER, UPPER):
+// the partial P, Q, B, R of binary-split (:123-133), built from COMBINE
+// (:112-121) and the base case (:100-110), evaluated one tree height at a
+// time.  Each height's products whose smaller operand has more than 100
+// limbs (*ntt-multiply-threshold*, src/multiply.lisp:9) go to the GPU in one
+// hgx_mul_batch call; smaller ones are multiplied on the host, as mpz-*
+// does (src/multiply.lisp:37-42).  An interval whose products are all small
+// needs no hgx_init.  Replaces a Lisp-side binary-split whose term functions
+// are polynomials: a(k), b(k), p(k), q(k) are polynomials in k (index 0 a,
+// 1 b, 2 p, 3 q; coefficients constant term first, degree 0..7) or, where
+// AT_ZERO[i] is set, VALUE0[i] at k = 0 (the reference's series give 1 there:
+// make-exp-series, make-e-series, make-ramanujan-series, :140-184;
+// make-chudnovsky-series, src/pi.lisp:24-43; make-catalan-series,
+// make-apery-series, :203-231).  UPPER <= 2^40; term values up to 1024 bits.
+// *OUT receives a result that the caller reads with hgx_partial_get (WHICH 0
+// P, 1 Q, 2 B, 3 R: SIGN in {-1, 0, 1}, NLIMBS of the trimmed magnitude,
+// copied to LIMBS when it is not null) and releases with hgx_partial_free.
+typedef struct {
+  int degree[4];
+  int64_t coef[4][8];
+  int at_zero[4];
+  int64_t value0[4];
+} hgx_series;
+
+typedef struct hgx_partial hgx_partial;
+
+int hgx_series_split(const hgx_series *series, uint64_t lower, uint64_t upper,
+                     hgx_partial **out);
+int hgx_partial_get(const hgx_partial *x, int which, int *sign, size_t *nlimbs,
+                    uint64_t *limbs);
+void hgx_partial_free(hgx_partial *x);
+
 #ifdef __cplusplus
 }
 #endif

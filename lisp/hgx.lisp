@@ -25,6 +25,9 @@
 ;;;;   DMPZ             magnitudes held in device memory (hgx_device_alloc
 ;;;;                    and friends), for Newton iterations whose iterates
 ;;;;                    stay in HBM between products and sums
+;;;;   HGX-BINARY-SPLIT BINARY-SPLIT of a series with polynomial terms by the
+;;;;                    library's native engine (hgx_series_split): the same
+;;;;                    PARTIAL, each tree height's large products batched
 ;;;;
 ;;;; Limbs are the reference's storage: little-endian 64-bit digits in C
 ;;;; memory (src/vec.lisp:17-65, src/ram-vec.lisp:31-97); VEC-DIGIT-POINTER
@@ -66,6 +69,16 @@
       (b sb-sys:system-area-pointer)
       (nb sb-alien:unsigned-long)
       (out sb-sys:system-area-pointer)))
+
+;; A series for hgx_series_split: a(k), b(k), p(k), q(k) (index 0..3) as
+;; polynomials in k, coefficients constant term first (COEF is the C array
+;; coef[4][8], row by row), or VALUE0 at k = 0 where AT-ZERO is non-zero.
+(sb-alien:define-alien-type hgx-series
+    (sb-alien:struct hgx-series
+      (degree (sb-alien:array sb-alien:int 4))
+      (coef (sb-alien:array (sb-alien:signed 64) 32))
+      (at-zero (sb-alien:array sb-alien:int 4))
+      (value0 (sb-alien:array (sb-alien:signed 64) 4))))
 
 ;;; Foreign routines.  Pointers into host memory are system-area pointers;
 ;;; device pointers and streams are unsigned 64-bit addresses (0 = the
@@ -148,6 +161,16 @@
 
 (sb-alien:define-alien-routine ("hgx_autotune" %hgx-autotune) sb-alien:int
   (min-bits sb-alien:int) (max-bits sb-alien:int) (reps sb-alien:int))
+
+;; the result of hgx_series_split is an opaque address (0 = none)
+(sb-alien:define-alien-routine ("hgx_series_split" %hgx-series-split) sb-alien:int
+  (series (* hgx-series)) (lower sb-alien:unsigned-long) (upper sb-alien:unsigned-long)
+  (out sb-alien:unsigned-long :out))
+(sb-alien:define-alien-routine ("hgx_partial_get" %hgx-partial-get) sb-alien:int
+  (x sb-alien:unsigned-long) (which sb-alien:int) (sign sb-alien:int :out)
+  (nlimbs sb-alien:unsigned-long :out) (limbs sb-sys:system-area-pointer))
+(sb-alien:define-alien-routine ("hgx_partial_free" %hgx-partial-free) sb-alien:void
+  (x sb-alien:unsigned-long))
 
 ;;; Errors
 
@@ -499,3 +522,69 @@ dropped (MPD-TRUNCATE!'s mantissa step, src/mpd.lisp:56-70)."
                                          (+ (dmpz-address x) (* 8 drop))
                                          keep 0)))
           (values (%dmpz-trim d) drop)))))
+
+;;; Binary splitting by the native engine
+;;;
+;;; BINARY-SPLIT (src/binary-splitting.lisp:123-133) takes a SERIES of four
+;;; closures.  HGX-BINARY-SPLIT takes the same four term functions as
+;;; polynomials, the form every series of the reference has, and returns
+;;; the PARTIAL that BINARY-SPLIT returns for them: the library builds the
+;;; same tree, folds its small subtrees and multiplies small operands on
+;;; host threads, and sends each tree height's products above
+;;; *NTT-MULTIPLY-THRESHOLD* to the GPU in one batched call.
+
+(defun %fill-term (desc index term)
+  "TERM is a list of integer coefficients, constant term first, optionally
+preceded by :AT-ZERO V for a function whose value at k = 0 is V."
+  (let ((at-zero nil) (v0 0) (coefs term))
+    (when (eq (first term) :at-zero)
+      (setf at-zero t v0 (second term) coefs (cddr term)))
+    (assert (<= 1 (length coefs) 8) () "term polynomials have degree 0..7")
+    (setf (sb-alien:deref (sb-alien:slot desc 'degree) index) (1- (length coefs))
+          (sb-alien:deref (sb-alien:slot desc 'at-zero) index) (if at-zero 1 0)
+          (sb-alien:deref (sb-alien:slot desc 'value0) index) v0)
+    (loop :for j :below 8
+          :for c := (or (nth j coefs) 0)
+          :do (setf (sb-alien:deref (sb-alien:slot desc 'coef) (+ (* 8 index) j)) c))))
+
+(defun %partial-mpz (handle which)
+  (multiple-value-bind (rc sign n)
+      (with-hgx-traps-masked (%hgx-partial-get handle which (sb-sys:int-sap 0)))
+    (hgx-check rc)
+    (let ((storage (make-storage (max n 1))))
+      (when (plusp n)
+        (multiple-value-bind (rc2 s2 n2)
+            (with-hgx-traps-masked (%hgx-partial-get handle which (vec-digit-pointer storage)))
+          (declare (ignore s2 n2))
+          (hgx-check rc2)))
+      (optimize-mpz (make-instance 'mpz/ram :sign (if (zerop sign) 1 sign) :storage storage)))))
+
+(defun hgx-binary-split (a b p q lower upper)
+  "The PARTIAL of BINARY-SPLIT over [LOWER, UPPER) for the series with term
+functions A, B, P, Q given as polynomial lists (see %FILL-TERM)."
+  (assert (> upper lower))
+  (hgx-ensure)
+  (let ((desc (sb-alien:make-alien hgx-series))
+        (handle 0))
+    (unwind-protect
+         (progn
+           (loop :for term :in (list a b p q)
+                 :for i :from 0
+                 :do (%fill-term (sb-alien:deref desc) i term))
+           (multiple-value-bind (rc h)
+               (with-hgx-traps-masked (%hgx-series-split desc lower upper))
+             (hgx-check rc)
+             (setf handle h))
+           (make-partial :lower lower :upper upper
+                         :p (%partial-mpz handle 0) :q (%partial-mpz handle 1)
+                         :b (%partial-mpz handle 2) :r (%partial-mpz handle 3)))
+      (unless (zerop handle) (with-hgx-traps-masked (%hgx-partial-free handle)))
+      (sb-alien:free-alien desc))))
+
+(defun hgx-chudnovsky-terms ()
+  "MAKE-CHUDNOVSKY-SERIES's term functions (src/pi.lisp:24-43) as polynomials:
+a, b, p, q for HGX-BINARY-SPLIT."
+  (list (list (* 10 13591409) (* 10 545140134))
+        (list (floor (* 8 640320) 12))
+        (list :at-zero 1 5 -46 108 -72)
+        (list :at-zero 1 0 0 0 (floor (expt 640320 3) 24))))

@@ -66,13 +66,13 @@ def kind_ok(ctype: str, lisp: list) -> bool:
         # the C parameter points to the returned value
         assert pointer, ctype
         base = ctype.replace("const", "").strip().rsplit("*", 1)[0].strip()
-        if base in ("size_t", "uint64_t", "hgx_ticket", "uint64_t *"):
+        if base in ("size_t", "uint64_t", "hgx_ticket", "uint64_t *", "hgx_partial *"):
             return ltype == "sb-alien:unsigned-long"
         if base == "int":
             return ltype == "sb-alien:int"
         return False
-    if isinstance(ltype, list):  # (* hgx-mul-desc)
-        return pointer and ltype[0] == "*" and "hgx_mul_desc" in ctype
+    if isinstance(ltype, list):  # (* hgx-mul-desc), (* hgx-series)
+        return pointer and ltype[0] == "*" and ltype[1].replace("-", "_") in ctype
     if pointer or ctype.split()[0] in ("size_t", "uint64_t", "hgx_ticket"):
         return ltype in WORD
     if ctype.split()[0] == "int":
