@@ -17,6 +17,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <new>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -488,6 +489,70 @@ void pin_free(PinStage &P) {
     (void)hipHostFree(P.buf[i]);
   }
   P = PinStage{};
+}
+
+// ---- host packs of batched products from host memory ----
+// hgx_mul_batch gathers the operands of many products into one contiguous
+// pack, moves it with one DMA, and scatters the products back.  The packs
+// live in pinned memory (grown on demand up to kPackHostMax bytes per
+// direction, used under g_ctx.mu), so the DMA engines read and write them
+// directly, and the gather and scatter run on several threads.
+constexpr size_t kPackHostMax = (size_t)1 << 30;
+struct HostBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+HostBuf g_pack_in, g_pack_out;
+
+// pinned memory of at least BYTES in B, or null (beyond kPackHostMax, or
+// the allocation failed: the caller then uses pageable memory)
+u64 *host_buf(HostBuf &b, size_t bytes) {
+  if (bytes > kPackHostMax) return nullptr;
+  if (b.bytes >= bytes) return (u64 *)b.p;
+  if (b.p) (void)hipHostFree(b.p);
+  b = HostBuf{};
+  const size_t want = std::min(kPackHostMax, std::max(bytes, (size_t)64 << 20));
+  if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return nullptr;
+  }
+  b.bytes = want;
+  return (u64 *)b.p;
+}
+
+void host_buf_free(HostBuf &b) {
+  if (b.p) (void)hipHostFree(b.p);
+  b = HostBuf{};
+}
+
+struct Piece {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+
+// the copies of PIECES, their concatenated bytes split evenly over up to 8
+// threads (at least 4 MiB each)
+void copy_pieces(const std::vector<Piece> &pieces) {
+  size_t total = 0;
+  for (const Piece &p : pieces) total += p.bytes;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min(8u, std::max(1u, hw / 2)), std::max<size_t>(1, total >> 22));
+  auto work = [&](size_t t) {
+    const size_t lo = total * t / nt, hi = total * (t + 1) / nt;
+    size_t at = 0;
+    for (const Piece &p : pieces) {
+      const size_t a = std::max(lo, at), b = std::min(hi, at + p.bytes);
+      if (a < b) std::memcpy((char *)p.dst + (a - at), (const char *)p.src + (a - at), b - a);
+      at += p.bytes;
+      if (at >= hi) break;
+    }
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
 }
 
 // ---- plans ----
@@ -2547,6 +2612,8 @@ void hgx_shutdown(void) {
   g_ctx.jobs_host = nullptr;
   g_ctx.jobs_host_cap = 0;
   pin_free(g_pin);
+  host_buf_free(g_pack_in);
+  host_buf_free(g_pack_out);
   (void)hipEventDestroy(g_ctx.jobs_copied);
   dev_free(g_ctx.tables, 3 * 2 * 2048 * 2 * sizeof(u64));
   g_ctx.tables = nullptr;
@@ -2707,6 +2774,10 @@ int hgx_mul_batch(const hgx_mul_desc *descs, size_t count) {
       const size_t need = align_up((in_words + iw + out_words + ow) * 8) + w +
                           align_up((k1 - k0 + 1) * sizeof(MulJob)) + kStageSlack;
       if (k1 > k0 && need > avail) break;
+      // a pack that fits the pinned buffers goes no further
+      if (k1 > k0 && 8 * std::max(in_words + iw, out_words + ow) > kPackHostMax &&
+          8 * std::max(in_words, out_words) <= kPackHostMax)
+        break;
       in_words += iw;
       out_words += ow;
       max_work = w;
@@ -2714,22 +2785,34 @@ int hgx_mul_batch(const hgx_mul_desc *descs, size_t count) {
     }
     rc = ensure(g_ctx.pack, align_up((in_words + out_words) * 8));
     if (rc) return rc;
-    std::vector<u64> hin(in_words);
+    // the packs: pinned when they fit, pageable (staged) otherwise
+    std::unique_ptr<u64[]> own_in, own_out;
+    u64 *hin = host_buf(g_pack_in, in_words * 8);
+    u64 *hout = hin ? host_buf(g_pack_out, out_words * 8) : nullptr;
+    if (!hin || !hout) {
+      own_in.reset(new (std::nothrow) u64[std::max<size_t>(in_words, 1)]);
+      own_out.reset(new (std::nothrow) u64[std::max<size_t>(out_words, 1)]);
+      if (!own_in || !own_out) return fail(HGX_ENOMEM, "out of host memory for a batch's operand pack");
+      hin = own_in.get();
+      hout = own_out.get();
+    }
     std::vector<hgx_mul_desc> dd(k1 - k0);
+    std::vector<Piece> pieces;
+    pieces.reserve(2 * (k1 - k0));
     u64 *din = (u64 *)g_ctx.pack.ptr, *dout = din + in_words;
     size_t at = 0, ot = 0;
     for (size_t k = k0; k < k1; k++) {
       const hgx_mul_desc &d = descs[rest[k]];
       const bool sqr = (d.a == d.b && d.na == d.nb);
       hgx_mul_desc &e = dd[k - k0];
-      std::memcpy(&hin[at], d.a, d.na * 8);
+      pieces.push_back(Piece{hin + at, d.a, d.na * 8});
       e.a = din + at;
       e.na = d.na;
       at += d.na;
       if (sqr) {
         e.b = e.a;
       } else {
-        std::memcpy(&hin[at], d.b, d.nb * 8);
+        pieces.push_back(Piece{hin + at, d.b, d.nb * 8});
         e.b = din + at;
         at += d.nb;
       }
@@ -2737,22 +2820,24 @@ int hgx_mul_batch(const hgx_mul_desc *descs, size_t count) {
       e.out = dout + ot;
       ot += d.na + d.nb;
     }
+    copy_pieces(pieces);
     PinScope pin;
-    rc = h2d(din, hin.data(), in_words * 8, s);
+    rc = h2d(din, hin, in_words * 8, s);
     if (rc) return rc;
     rc = batch_device_locked(dd.data(), dd.size(), s);
     if (rc) return rc;
-    std::vector<u64> hout(out_words);
-    rc = d2h(hout.data(), dout, out_words * 8, s);
+    rc = d2h(hout, dout, out_words * 8, s);
     if (!rc) rc = pin_flush();
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s));
+    pieces.clear();
     ot = 0;
     for (size_t k = k0; k < k1; k++) {
       const hgx_mul_desc &d = descs[rest[k]];
-      std::memcpy(d.out, &hout[ot], (d.na + d.nb) * 8);
+      pieces.push_back(Piece{d.out, hout + ot, (d.na + d.nb) * 8});
       ot += d.na + d.nb;
     }
+    copy_pieces(pieces);
     k0 = k1;
   }
   return HGX_OK;

@@ -26,6 +26,9 @@
 #include <atomic>
 #include <cstring>
 #include <functional>
+#include <memory>
+#include <new>
+#include <utility>
 #include <string>
 #include <thread>
 #include <vector>
@@ -35,13 +38,36 @@
 typedef uint64_t u64;
 typedef unsigned __int128 u128;
 
+// Limb vectors whose resize leaves new limbs uninitialised: products are
+// written whole by the GPU or by mag_mul, so zeroing them first would only
+// add a pass over memory.
+template <class T>
+struct NoInit : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInit<U>;
+  };
+  NoInit() = default;
+  template <class U>
+  NoInit(const NoInit<U> &) noexcept {}
+  template <class U>
+  void construct(U *p) noexcept {
+    ::new ((void *)p) U;
+  }
+  template <class U, class... A>
+  void construct(U *p, A &&...a) {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+typedef std::vector<u64, NoInit<u64>> Limbs;
+
 // defined in hgx_runtime.hip
 int hgx_internal_fail(int code, const char *msg);
 bool hgx_internal_ready();
 
 struct hgx_partial {
   int sign[4];
-  std::vector<u64> mag[4];  // P, Q, B, R: trimmed magnitudes
+  Limbs mag[4];  // P, Q, B, R: trimmed magnitudes
 };
 
 namespace {
@@ -61,10 +87,10 @@ struct Small {
 // A signed integer of any size.
 struct Big {
   int s = 0;
-  std::vector<u64> m;
+  Limbs m;
 };
 
-void trim(std::vector<u64> &m) {
+void trim(Limbs &m) {
   while (!m.empty() && m.back() == 0) m.pop_back();
 }
 
@@ -99,7 +125,7 @@ void mag_mul(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *r) {
 }
 
 // x += y (magnitudes); x grows as needed
-void mag_add_to(std::vector<u64> &x, const u64 *y, size_t ny) {
+void mag_add_to(Limbs &x, const u64 *y, size_t ny) {
   if (x.size() < ny) x.resize(ny, 0);
   u64 c = 0;
   size_t i = 0;
@@ -116,7 +142,7 @@ void mag_add_to(std::vector<u64> &x, const u64 *y, size_t ny) {
 }
 
 // x = |x - y| for |x| >= |y| (magnitudes)
-void mag_sub_from(std::vector<u64> &x, const u64 *y, size_t ny) {
+void mag_sub_from(Limbs &x, const u64 *y, size_t ny) {
   u64 b = 0;
   size_t i = 0;
   for (; i < ny; i++) {
@@ -150,7 +176,7 @@ void big_add(Big &x, const Big &y) {
   } else if (c > 0) {
     mag_sub_from(x.m, y.m.data(), y.m.size());
   } else {
-    std::vector<u64> t = y.m;
+    Limbs t = y.m;
     mag_sub_from(t, x.m.data(), x.m.size());
     x.m.swap(t);
     x.s = y.s;
@@ -158,7 +184,7 @@ void big_add(Big &x, const Big &y) {
 }
 
 // x = x * v for a small v; TMP is scratch
-void big_mul_small(Big &x, const Small &v, std::vector<u64> &tmp) {
+void big_mul_small(Big &x, const Small &v, Limbs &tmp) {
   if (x.s == 0) return;
   if (v.s == 0) {
     x.s = 0;
@@ -287,7 +313,7 @@ int fold(const Series &s, u64 lo, u64 up, Part &out) {
   out.B = big_of(bk);
   out.R = big_of(t1);
   Big BP = out.B;  // B * P of the terms so far
-  std::vector<u64> tmp;
+  Limbs tmp;
   big_mul_small(BP, pk, tmp);
   for (u64 k = lo + 1; k < up; k++) {
     rc = poly_eval(s.p, k, pk);
