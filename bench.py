@@ -334,6 +334,39 @@ def latency_leg(h, _lib, n: int = 1 << 16, reps: int = 200) -> dict:
     th = timed(host_call)
     td = timed(dev_call)
     same = bool(np.array_equal(hO.arr, download(_lib, O.ptr, 2 * n)))
+    want = hO.arr.copy()
+
+    def threaded(nthreads: int, calls: int = 100) -> dict:
+        """NTHREADS threads each making CALLS hgx_mul calls of this product
+        from and to pageable memory of their own (numpy arrays), as the
+        reference's parallel build calls mpz-* from lparallel workers."""
+        import threading
+        bufs = [(hA.arr.copy(), hB.arr.copy(), np.empty(2 * n, dtype=np.uint64)) for _ in range(nthreads)]
+        go = threading.Barrier(nthreads + 1)
+        errs = []
+
+        def work(k):
+            a, b, o = bufs[k]
+            go.wait()
+            try:
+                for _ in range(calls):
+                    _lib.check(L_.hgx_mul(a.ctypes.data, n, b.ctypes.data, n, o.ctypes.data))
+            except Exception as e:  # reported with the leg
+                errs.append(repr(e))
+
+        ths = [threading.Thread(target=work, args=(k,)) for k in range(nthreads)]
+        for t in ths:
+            t.start()
+        go.wait()
+        t0 = time.perf_counter()
+        for t in ths:
+            t.join()
+        dt = time.perf_counter() - t0
+        ok = not errs and all(np.array_equal(o, want) for _, _, o in bufs)
+        return {"threads": nthreads, "calls": nthreads * calls, "seconds": round(dt, 4),
+                "limbs_per_s": 2 * n * nthreads * calls / dt, "outputs_identical": ok}
+
+    conc = [threaded(k) for k in (1, 2, 4, 8)]
     for d in (A, B, O, hA, hB, hO):
         d.free()
 
@@ -343,7 +376,7 @@ def latency_leg(h, _lib, n: int = 1 << 16, reps: int = 200) -> dict:
     return {"workload": f"one {n} x {n}-limb product (L=2^{(2 * n).bit_length() - 1}), {reps} calls",
             "host_p50_ms": q(th, 0.5), "host_p99_ms": q(th, 0.99), "host_limbs_per_s": 2 * n / th[len(th) // 2],
             "device_p50_ms": q(td, 0.5), "device_p99_ms": q(td, 0.99), "device_limbs_per_s": 2 * n / td[len(td) // 2],
-            "outputs_identical": same}
+            "outputs_identical": same, "concurrent_host_calls": conc}
 
 
 def pi_leg(bits: int = 1 << 28, identical_bits: int = 1 << 22, mpfr_bits: int = 1 << 24, host_iterates: bool = False,

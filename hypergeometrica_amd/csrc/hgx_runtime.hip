@@ -256,7 +256,12 @@ void timing_drain() {
 
 #define HGX_TIMED(name, s) KTimer kt_##__LINE__(name, s)
 
+bool slots_trim();
+
 int dev_alloc(void **p, size_t bytes) {
+  if (g_ctx.allocated + bytes > g_ctx.budget && slots_trim()) {
+    // idle call slots gave their memory back; try again
+  }
   if (g_ctx.allocated + bytes > g_ctx.budget)
     return fail(HGX_ENOMEM, "HBM budget exceeded: need " + std::to_string(bytes) + " more bytes, " +
                                 std::to_string(g_ctx.budget - g_ctx.allocated) + " left");
@@ -2503,6 +2508,178 @@ int scan_read(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, bool si
   return HGX_OK;
 }
 
+// ---- concurrent host-memory products (call slots) ----
+// hgx_mul / hgx_sqr calls from several threads run on call slots: a slot
+// has a stream, device memory for its product's operands, product and work
+// arrays, pinned staging and a pinned chunk table of its own.  g_ctx.mu is
+// held only while a call sizes its slot and builds root tables, and while it
+// queues its kernels; its transfers and the wait for its product run
+// without it, so calls on different slots overlap, one's copies with
+// another's kernels.  Products whose device working set exceeds 1/8 of the
+// budget (or that must be host-staged) take the library for their whole
+// duration, as a single product of that size fills the GPU by itself.
+constexpr int kCallSlots = 4;
+struct CallSlot {
+  hipStream_t s = nullptr;
+  Workspace mem, scan;  // operands, product and work memory; a split product's carry scratch
+  PinStage pin;
+  MulJob *jobs = nullptr;  // pinned chunk table of a split product
+  size_t jobs_cap = 0;
+  bool busy = false;
+};
+struct CallSlots {
+  std::mutex mu;
+  std::condition_variable cv;
+  CallSlot slot[kCallSlots];
+  int active = 0;        // calls holding a slot
+  bool closing = false;  // hgx_shutdown waits for the active calls
+};
+CallSlots g_slots;
+
+// a free slot (waiting for one), or -1 while the library shuts down
+int slot_acquire() {
+  std::unique_lock<std::mutex> lk(g_slots.mu);
+  for (;;) {
+    if (g_slots.closing) return -1;
+    for (int i = 0; i < kCallSlots; i++)
+      if (!g_slots.slot[i].busy) {
+        g_slots.slot[i].busy = true;
+        g_slots.active++;
+        return i;
+      }
+    g_slots.cv.wait(lk);
+  }
+}
+
+void slot_release(int i) {
+  std::lock_guard<std::mutex> lk(g_slots.mu);
+  g_slots.slot[i].busy = false;
+  g_slots.active--;
+  g_slots.cv.notify_all();
+}
+
+// device memory of a slot (g_ctx.mu held; the slot's stream is idle)
+int slot_ensure(Workspace &w, size_t bytes) {
+  if (w.bytes >= bytes) return HGX_OK;
+  dev_free(w.ptr, w.bytes);
+  w = Workspace{};
+  const size_t want = std::max(bytes, (size_t)1 << 20);
+  int rc = dev_alloc(&w.ptr, want);
+  if (rc) return rc;
+  w.bytes = want;
+  return HGX_OK;
+}
+
+// release the device memory of every slot not in use (g_ctx.mu held), so
+// that a larger product or the server may have it; whether any was held
+bool slots_trim() {
+  std::lock_guard<std::mutex> lk(g_slots.mu);
+  bool any = false;
+  for (CallSlot &S : g_slots.slot) {
+    if (S.busy) continue;
+    any = any || S.mem.ptr || S.scan.ptr;
+    dev_free(S.mem.ptr, S.mem.bytes);
+    dev_free(S.scan.ptr, S.scan.bytes);
+    S.mem = Workspace{};
+    S.scan = Workspace{};
+  }
+  return any;
+}
+
+// every slot's resources (g_ctx.mu held, no slot active)
+void slots_free() {
+  for (CallSlot &S : g_slots.slot) {
+    if (S.s) (void)hipStreamSynchronize(S.s);
+    dev_free(S.mem.ptr, S.mem.bytes);
+    dev_free(S.scan.ptr, S.scan.bytes);
+    pin_free(S.pin);
+    if (S.jobs) (void)hipHostFree(S.jobs);
+    if (S.s) (void)hipStreamDestroy(S.s);
+    S = CallSlot{};
+  }
+}
+
+// whether a host-memory product runs on a call slot (g_ctx.mu held);
+// HGX_CALL_SLOTS=0 in the environment turns the slots off
+bool slot_product(size_t na, size_t nb, bool sqr) {
+  static const bool on = [] {
+    const char *e = getenv("HGX_CALL_SLOTS");
+    return !(e && *e == '0');
+  }();
+  return on && !needs_staging(na, nb, sqr) && inhbm_bytes(na, nb, sqr) <= g_ctx.budget / 8;
+}
+
+// The product of host-memory operands on a call slot (g_ctx.mu not held).
+int mul_host_slot(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, bool sqr) {
+  const int si = slot_acquire();
+  if (si < 0) return fail(HGX_ENOTINIT, "the library is shutting down");
+  CallSlot &S = g_slots.slot[si];
+  struct Release {
+    CallSlot &S;
+    int i;
+    ~Release() {
+      // nothing may still run on the slot's memory once it is free (a
+      // failed call can return with work queued)
+      if (S.s && hipStreamSynchronize(S.s) != hipSuccess) (void)hipGetLastError();
+      slot_release(i);
+    }
+  } release{S, si};
+  const size_t nout = na + nb;
+  const size_t ba = align_up(na * 8), bb = sqr ? 0 : align_up(nb * 8), bo = align_up(nout * 8);
+  const int n = log2_ceil((u64)hgx_product_length(na, nb));
+  Split sp;
+  bool split = false;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+    int rc = check_ready();
+    if (rc) return rc;
+    if (!S.s) HIPCHK(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
+    split = !sqr && g_ctx.split_on && choose_split(na, nb, sp);
+    size_t work = 0;
+    if (split) {
+      const SplitBytes sb = split_bytes(na, nb, sp);
+      work = sb.main;
+      rc = slot_ensure(S.scan, sb.scan);
+      if (!rc && S.jobs_cap < sp.k) {
+        if (S.jobs) (void)hipHostFree(S.jobs);
+        S.jobs = nullptr;
+        S.jobs_cap = 0;
+        HIPCHK(hipHostMalloc((void **)&S.jobs, std::max<size_t>(sp.k, 64) * sizeof(MulJob), hipHostMallocDefault));
+        S.jobs_cap = std::max<size_t>(sp.k, 64);
+      }
+    } else {
+      work = work_layout(n, sqr).total;
+    }
+    if (!rc) rc = slot_ensure(S.mem, ba + bb + bo + work);
+    Plan *plan = nullptr;
+    if (!rc) rc = build_plan(split ? sp.nc : n, &plan);  // root tables, built and copied before this returns
+    if (rc) return rc;
+  }
+  char *base = (char *)S.mem.ptr;
+  u64 *d_a = (u64 *)base, *d_b = sqr ? d_a : (u64 *)(base + ba), *d_o = (u64 *)(base + ba + bb);
+  char *work = base + ba + bb + bo;
+  PinScope pin(S.pin);
+  int rc = h2d(S.pin, d_a, a, na * 8, S.s);
+  if (!rc && !sqr) rc = h2d(S.pin, d_b, b, nb * 8, S.s);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+    if (split) {
+      rc = split_product_at(d_a, na, d_b, nb, d_o, sp, S.s, work, (uint8_t *)S.scan.ptr, S.jobs);
+    } else {
+      JobRef jr{};
+      jr.one = make_job(d_a, na, d_b, sqr ? na : nb, d_o, n, sqr, work);
+      rc = enqueue_group(jr, 1, n, sqr, S.s);
+    }
+    if (rc) return rc;
+  }
+  rc = d2h(S.pin, out, d_o, nout * 8, S.s);
+  if (!rc) rc = pin_flush(S.pin);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(S.s));
+  return HGX_OK;
+}
+
 }  // namespace
 
 // for the library's other host translation units (hgx_series.cpp): record
@@ -2585,7 +2762,19 @@ int hgx_init(int device, size_t hbm_budget_bytes) {
 
 void hgx_shutdown(void) {
   server_stop();
+  {
+    // calls on call slots finish first (they take g_ctx.mu to queue work)
+    std::unique_lock<std::mutex> sl(g_slots.mu);
+    g_slots.closing = true;
+    g_slots.cv.notify_all();
+    g_slots.cv.wait(sl, [] { return g_slots.active == 0; });
+  }
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  slots_free();
+  {
+    std::lock_guard<std::mutex> sl(g_slots.mu);
+    g_slots.closing = false;
+  }
   if (!g_ctx.ready) return;
   (void)hipStreamSynchronize(g_ctx.stream);
   timing_drain();
@@ -2711,23 +2900,33 @@ static bool overlaps(const uint64_t *x, size_t nx, const uint64_t *y, size_t ny)
   return x0 < y1 && y0 < x1;
 }
 
+// A host-memory product on a call slot, or holding the library (large or
+// host-staged products).
+static int mul_host(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out, bool sqr) {
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+    int rc = check_ready();
+    if (rc) return rc;
+    if (log2_ceil((u64)hgx_product_length(na, nb)) > kMaxBits) return fail(HGX_ETOOBIG, "transform length above 2^32");
+    if (!slot_product(na, nb, sqr)) {
+      slots_trim();  // the product may use what idle slots hold
+      return mul_host_locked(a, na, b, nb, out, sqr);
+    }
+  }
+  return mul_host_slot(a, na, b, nb, out, sqr);
+}
+
 int hgx_mul(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out) {
-  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
-  int rc = check_ready();
-  if (rc) return rc;
   if (!a || !b || !out || na == 0 || nb == 0) return fail(HGX_EINVAL, "null operand or zero length");
   if (overlaps(out, na + nb, a, na) || overlaps(out, na + nb, b, nb))
     return fail(HGX_EINVAL, "the product storage overlaps an operand");
-  return mul_host_locked(a, na, b, nb, out, a == b && na == nb);
+  return mul_host(a, na, b, nb, out, a == b && na == nb);
 }
 
 int hgx_sqr(const uint64_t *a, size_t na, uint64_t *out) {
-  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
-  int rc = check_ready();
-  if (rc) return rc;
   if (!a || !out || na == 0) return fail(HGX_EINVAL, "null operand or zero length");
   if (overlaps(out, 2 * na, a, na)) return fail(HGX_EINVAL, "the square's storage overlaps the operand");
-  return mul_host_locked(a, na, a, na, out, true);
+  return mul_host(a, na, a, na, out, true);
 }
 
 int hgx_mul_batch(const hgx_mul_desc *descs, size_t count) {
