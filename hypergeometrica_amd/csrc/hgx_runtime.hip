@@ -1527,9 +1527,9 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
 // most a 1/lanes share of the arena), groups each admission round by
 // transform length into batched launches, and puts each round on the lane
 // with the least work in flight, so rounds overlap on the device.  A lane is
-// one stream for a round's uploads and kernels and, with split downloads, a
-// second stream for its downloads, so that the next round's uploads on the
-// lane do not queue behind this round's downloads.  A request that arrives
+// the stream of a round's kernels; the copies go to upload and download
+// streams shared by the lanes (or, in the lane layout, run on the lane's
+// stream in order with its kernels).  A request that arrives
 // while others run joins the next round as soon as memory frees up.
 // Requests larger than the arena go to a stager thread, which computes them
 // host-staged on a stream of its own and holds the library only while it
@@ -1567,25 +1567,27 @@ struct SrvReq {
   std::string err;
 };
 
-// Stream layouts.  By default every copy of a round runs on its lane's
-// stream, in order with the round's kernels: six lanes serve a saturated
-// mixed-size stream at 4.0-4.1e9 limbs/s with 32 requests outstanding.
-// HGX_SERVE_COPY_STREAMS=1 moves the copies to streams shared by the lanes:
-// all uploads of a round go to an upload stream, the round's kernels wait for
-// them on a compute lane, and each product's download is issued on the less
-// loaded of two download streams once its kernels are seen to be done.  The
-// copy engines move this mix at 48 GB/s each way with one or two streams per
-// direction, against 38 GB/s when six streams each run uploads and downloads
-// in turn; served, the layout reached 4.3-4.8e9 limbs/s with 128 requests
-// outstanding in some processes and 3.5-3.7e9 in others, so it is not the
-// default.  HGX_SERVE_SPLIT_D2H=1 gives each lane a second stream for its
-// downloads; HGX_SERVE_UPLOADS / HGX_SERVE_DOWNLOADS (1..4) set the copy
-// stream counts and HGX_SERVE_LANES (1..16) the compute lanes.
+// Stream layouts.  By default the copies run on streams shared by the
+// lanes: all uploads of a round go to an upload stream, the round's kernels
+// wait for them on a compute lane, and each product's download is issued on
+// the less loaded of two download streams once its kernels are seen to be
+// done.  The copy engines move this mix at 48 GB/s each way with one or two
+// streams per direction, against 38 GB/s when six streams each run uploads
+// and downloads in turn.  Served with 32 requests outstanding, a saturated
+// mixed-size stream runs at 4.0-4.3e9 limbs/s in this layout against
+// 3.8e9 with every copy of a round on its lane's stream, in order with the
+// round's kernels (HGX_SERVE_COPY_STREAMS=0).  Downloads queued at launch
+// behind the kernels' event (instead of issued once the kernels are seen
+// done) served 2.2e9: a download then waits in its stream for kernels that
+// end later than the next ones.  HGX_SERVE_SPLIT_D2H=1 (lane layout) gives
+// each lane a second stream for its downloads; HGX_SERVE_UPLOADS /
+// HGX_SERVE_DOWNLOADS (1..4) set the copy stream counts and HGX_SERVE_LANES
+// (1..16) the compute lanes.
 constexpr int kMaxLanes = 16;
 constexpr int kMaxCopyStreams = 4;
 constexpr int kDefaultLanes = 6;
 constexpr int kDefaultSplitD2H = 0;
-constexpr int kDefaultCopyStreams = 0;
+constexpr int kDefaultCopyStreams = 1;
 constexpr int kDefaultUploads = 1;
 constexpr int kDefaultDownloads = 2;
 constexpr size_t kServeTableBytes = 4 << 20;  // pinned pool of the server's host job tables

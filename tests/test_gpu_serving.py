@@ -64,9 +64,9 @@ def _stream_operands(oracle, count, seed):
     return reqs
 
 
-# the server's two stream layouts: every copy on its round's lane (the
-# default), and uploads and downloads on streams shared by the compute lanes
-MODES = {"lane_streams": {}, "copy_streams": {"HGX_SERVE_COPY_STREAMS": 1}}
+# the server's two stream layouts: uploads and downloads on streams shared
+# by the compute lanes (the default), and every copy on its round's lane
+MODES = {"lane_streams": {"HGX_SERVE_COPY_STREAMS": 0}, "copy_streams": {"HGX_SERVE_COPY_STREAMS": 1}}
 
 
 @pytest.mark.parametrize("mode", list(MODES))
@@ -76,8 +76,7 @@ def test_poisson_stream_equals_solo_and_gmp(hgx, oracle, server_env, mode):
     # solo run and GMP's
     from hypergeometrica_amd import _lib
     from hypergeometrica_amd import workload as W
-    if MODES[mode]:
-        server_env(**MODES[mode])
+    server_env(**MODES[mode])
     count, seed = 24, 0x48475804
     reqs = _stream_operands(oracle, count, seed)
     say("serving mix:", [(a.size, b.size) for a, b in reqs])
@@ -98,7 +97,7 @@ def test_poisson_stream_equals_solo_and_gmp(hgx, oracle, server_env, mode):
         want = _gmp.mul(a, b if b is not a else a.copy())
         assert np.array_equal(y, want), (i, a.size, b.size)
     st = _lib.server_stats()
-    assert (st["uploads"], st["downloads"]) == ((1, 2) if MODES[mode] else (0, 0))
+    assert (st["uploads"], st["downloads"]) == ((1, 2) if mode == "copy_streams" else (0, 0))
 
 
 @pytest.mark.parametrize("mode", list(MODES))
