@@ -3,6 +3,8 @@ the golden fixtures, GMP and exact integers.  Integer work: bit-exact."""
 import os
 import sys
 
+import time
+
 import numpy as np
 import pytest
 
@@ -194,6 +196,57 @@ def test_concurrent_host_calls_from_threads(hgx, oracle):
     assert len(results) == 8 * len(shapes)
     for key, (got, want) in results.items():
         assert np.array_equal(got, want), key
+
+
+def test_shutdown_during_concurrent_host_calls(hgx, oracle):
+    # hgx_mul calls on call slots from four threads while the library shuts
+    # down and starts again: every call returns either the exact product or
+    # HGX_ENOTINIT, and none hangs (hgx_shutdown waits for calls holding a
+    # slot; later calls fail until hgx_init)
+    import threading
+    from hypergeometrica_amd import _lib
+    L = _lib.lib()
+    a = oracle.splitmix64(0x5301, 1 << 15)
+    b = oracle.splitmix64(0x5302, 3 << 13)
+    want = _gmp.mul(a, b)
+    stop = threading.Event()
+    seen = {"ok": 0, "notinit": 0, "bad": []}
+
+    def worker():
+        out = np.empty(a.size + b.size, dtype=np.uint64)
+        while not stop.is_set():
+            out[:] = 0
+            rc = L.hgx_mul(a.ctypes.data, a.size, b.ctypes.data, b.size, out.ctypes.data)
+            if rc == 0:
+                if np.array_equal(out, want):
+                    seen["ok"] += 1
+                else:
+                    seen["bad"].append("wrong product")
+            elif rc == _lib.HGX_ENOTINIT:
+                seen["notinit"] += 1
+            else:
+                seen["bad"].append(rc)
+
+    import gc
+    gc.collect()  # no device block of an earlier test is released after the restarts
+    threads = [threading.Thread(target=worker) for _ in range(4)]
+    for t in threads:
+        t.start()
+    try:
+        for _ in range(3):
+            time.sleep(0.2)
+            L.hgx_shutdown()
+            _lib.generation += 1  # blocks handed out before are gone
+            time.sleep(0.05)
+            _lib.check(L.hgx_init(0, 0))
+        time.sleep(0.2)
+    finally:
+        stop.set()
+        for t in threads:
+            t.join(timeout=60)
+    assert not any(t.is_alive() for t in threads)
+    assert not seen["bad"], seen["bad"][:5]
+    assert seen["ok"] > 0
 
 
 def test_overlapping_product_storage_rejected(hgx):
