@@ -213,11 +213,14 @@ def download(_lib, ptr: int, n: int):
     return out
 
 
-def batched_leg(h, _lib, count: int = 4096, reps: int = 3) -> dict:
+def batched_leg(h, _lib, count: int = 4096, reps: int = 3, probe: dict | None = None) -> dict:
     """Binary-splitting level: COUNT independent products, operand sizes
     log-uniform in [2^8, 2^16] limbs (seed 0x48475803), operands in HBM.
     Batched (hgx_mul_batch_device, grouped by transform length) against the
-    per-call path (hgx_mul_device per product); outputs compared."""
+    per-call path (hgx_mul_device per product); outputs compared.  With the
+    ALU PROBE's rates, the time of the level's radix-2 butterflies and
+    pointwise products alone (3 primes; the twiddle products between passes
+    not counted, so a lower bound) and the batched time's fraction of it."""
     na, nb = W.batch_shapes(count, W.BATCH_SEED)
     tot = sum(na) + sum(nb)
     IN, OUT1, OUT2 = Dev(_lib, tot), Dev(_lib, tot), Dev(_lib, tot)
@@ -255,10 +258,22 @@ def batched_leg(h, _lib, count: int = 4096, reps: int = 3) -> dict:
     for d in (IN, OUT1, OUT2):
         d.free()
     b, p = min(tb), min(tp)
-    return {"workload": f"{count} products, operand limbs log-uniform in [2^8, 2^16], seed 0x48475803",
-            "limbs": tot, "batched_s": round(b, 5), "per_call_s": round(p, 5),
-            "batched_limbs_per_s": tot / b, "per_call_limbs_per_s": tot / p,
-            "speedup": round(p / b, 2), "outputs_identical": same}
+    res = {"workload": f"{count} products, operand limbs log-uniform in [2^8, 2^16], seed 0x48475803",
+           "limbs": tot, "batched_s": round(b, 5), "per_call_s": round(p, 5),
+           "batched_limbs_per_s": tot / b, "per_call_limbs_per_s": tot / p,
+           "speedup": round(p / b, 2), "outputs_identical": same}
+    if probe:
+        ops = {"fwd_butterfly": 0, "inv_butterfly": 0, "mont": 0}
+        for x, y in zip(na, nb):
+            n = (x + y - 1).bit_length()
+            L = 1 << n
+            ops["fwd_butterfly"] += 3 * 2 * (L // 2) * n
+            ops["inv_butterfly"] += 3 * (L // 2) * n
+            ops["mont"] += 3 * L
+        fl = alu_floor_s(ops, probe)
+        res["alu_floor_ms"] = round(1e3 * fl, 3)
+        res["alu_frac"] = round(fl / b, 4)
+    return res
 
 
 def square_leg(h, _lib, bits: int = 26, steps: int = 10) -> dict:
@@ -914,7 +929,7 @@ def main() -> int:
     batched = None
     if not args.no_batch:
         print("# batched vs per-call ...", file=sys.stderr, flush=True)
-        batched = batched_leg(h, _lib)
+        batched = batched_leg(h, _lib, probe=probe)
 
     latency = None
     if not args.no_latency:
