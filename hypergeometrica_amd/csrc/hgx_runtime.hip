@@ -555,7 +555,13 @@ void copy_pieces(const std::vector<Piece> &pieces) {
     }
   };
   std::vector<std::thread> th;
-  for (size_t t = 1; t < nt; t++) th.emplace_back(work, t);
+  size_t t = 1;
+  try {
+    for (; t < nt; t++) th.emplace_back(work, t);
+  } catch (const std::exception &) {
+    // no thread for part t and above: this thread copies them
+  }
+  for (size_t u = t; u < nt; u++) work(u);
   work(0);
   for (auto &x : th) x.join();
 }

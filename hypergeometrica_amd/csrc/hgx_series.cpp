@@ -25,6 +25,7 @@
 #include <array>
 #include <atomic>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <memory>
 #include <new>
@@ -355,7 +356,12 @@ int parallel_for(size_t n, const std::function<int(size_t)> &fn) {
       if (err.load() != HGX_OK) return;
       const size_t i = next.fetch_add(1);
       if (i >= n) return;
-      const int rc = fn(i);
+      int rc;
+      try {
+        rc = fn(i);
+      } catch (const std::bad_alloc &) {
+        rc = hgx_internal_fail(HGX_ENOMEM, "out of host memory in binary splitting");
+      }
       if (rc != HGX_OK) {
         int expect = HGX_OK;
         if (err.compare_exchange_strong(expect, rc) && !msg_taken.test_and_set()) msg = hgx_last_error();
@@ -397,14 +403,19 @@ int mul_many(const std::vector<const Big *> &xs, const std::vector<const Big *> 
     if (!hgx_internal_ready())
       return hgx_internal_fail(HGX_ENOTINIT, "hgx_init has not been called (a series product needs the GPU)");
     gpu_thread = std::thread([&]() {
-      std::vector<hgx_mul_desc> d(gpu.size());
-      for (size_t k = 0; k < gpu.size(); k++) {
-        const size_t i = gpu[k];
-        const Big &x = *xs[i], &y = *ys[i];
-        d[k] = hgx_mul_desc{x.m.data(), x.m.size(), y.m.data(), y.m.size(), out[i].m.data()};
+      try {
+        std::vector<hgx_mul_desc> d(gpu.size());
+        for (size_t k = 0; k < gpu.size(); k++) {
+          const size_t i = gpu[k];
+          const Big &x = *xs[i], &y = *ys[i];
+          d[k] = hgx_mul_desc{x.m.data(), x.m.size(), y.m.data(), y.m.size(), out[i].m.data()};
+        }
+        rc_gpu = hgx_mul_batch(d.data(), d.size());
+        if (rc_gpu) gpu_err = hgx_last_error();
+      } catch (const std::bad_alloc &) {
+        rc_gpu = HGX_ENOMEM;
+        gpu_err = "out of host memory in binary splitting";
       }
-      rc_gpu = hgx_mul_batch(d.data(), d.size());
-      if (rc_gpu) gpu_err = hgx_last_error();
     });
   }
   const int rc_host = parallel_for(host.size(), [&](size_t k) {
@@ -449,7 +460,19 @@ int parse_poly(const hgx_series &s, int which, Poly &f) {
 
 extern "C" {
 
+static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, hgx_partial **out);
+
 int hgx_series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, hgx_partial **out) {
+  try {
+    return series_split(desc, lower, upper, out);
+  } catch (const std::bad_alloc &) {
+    return hgx_internal_fail(HGX_ENOMEM, "out of host memory in binary splitting");
+  } catch (const std::exception &e) {
+    return hgx_internal_fail(HGX_EINVAL, e.what());
+  }
+}
+
+static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, hgx_partial **out) {
   if (!desc || !out) return hgx_internal_fail(HGX_EINVAL, "null series or result pointer");
   *out = nullptr;
   if (upper <= lower) return hgx_internal_fail(HGX_EINVAL, "the interval [lower, upper) is empty");
