@@ -37,6 +37,10 @@ sys.path.insert(0, ROOT)
 # library's, and with HIP's default of four queues streams share queues and
 # wait behind each other's copies (served rate 3.8 -> 4.0e9 limbs/s with 8)
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# the request server admits requests under a fixed HBM budget: an arena of
+# 64 GiB for operands, products and work arrays (read when the server starts)
+SERVE_ARENA_BYTES = 64 << 30
+os.environ.setdefault("HGX_SERVE_ARENA_BYTES", str(SERVE_ARENA_BYTES))
 
 from hypergeometrica_amd import workload as W  # noqa: E402
 
@@ -725,10 +729,11 @@ def serving_leg(_lib, rate: float = 2000.0, seconds: float = 10.0, slots: int = 
         stats = _lib.server_stats()
     finally:
         st.release()
+    main["arena_bytes"] = int(os.environ.get("HGX_SERVE_ARENA_BYTES", "0"))
     main["workload"] = (f"Poisson arrivals {rate:.0f}/s for {seconds} s (seed {seed:#x}), operand limbs "
-                        f"log-uniform in [2^10, 2^24], pinned host buffers, {slots} product slots, root tables "
-                        f"built before the stream, every product checked mod 2^61-1 and {main['gmp_checked']} "
-                        f"against GMP")
+                        f"log-uniform in [2^10, 2^24], pinned host buffers, {slots} product slots, server arena "
+                        f"{main['arena_bytes'] >> 30} GiB, root tables built before the stream, every product "
+                        f"checked mod 2^61-1 and {main['gmp_checked']} against GMP")
     main["sustained"] = sustained
     main["rate_search"] = tried
     main["server"] = {"lanes": stats["lanes"], "split_downloads": stats["split_downloads"],
