@@ -82,3 +82,19 @@ def test_gpu_products_need_init():
     with pytest.raises(HgxError) as e:
         S.binary_split_native(S.e_series(), 0, 3000)  # 3000! has more than 100 limbs
     assert e.value.code == HGX_ENOTINIT
+
+
+@pytest.mark.parametrize("series,lo,up", [
+    # p vanishes at k = 5: P and every product with it become zero
+    (S.Series(p=S.Poly((-5, 1), at_zero=1), q=S.Poly((0, 3), at_zero=1)), 0, 40),
+    # a vanishes everywhere: R = 0
+    (S.Series(a=S.Poly((0,)), q=S.Poly((1, 1))), 0, 50),
+    # negative b and q: signs of B, Q and R alternate
+    (S.Series(a=S.Poly((2, -7)), b=S.Poly((-3,)), p=S.Poly((1, 2, -1)), q=S.Poly((-4, 0, 5), at_zero=-1)), 0, 45),
+    # degree-7 terms with the largest coefficients
+    (S.Series(a=S.Poly((2**63 - 1,) * 8), p=S.Poly((-(2**63),) + (1,) * 7, at_zero=7), q=S.Poly((1,) * 8)), 0, 9),
+])
+def test_native_degenerate_series(series, lo, up):
+    want = SO.binary_split(series, lo, up)
+    got = S.binary_split_native(series, lo, up)
+    assert (got.lower, got.upper, got.p, got.q, got.b, got.r) == want
