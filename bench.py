@@ -222,9 +222,11 @@ def batched_leg(h, _lib, count: int = 4096, reps: int = 3, probe: dict | None = 
     log-uniform in [2^8, 2^16] limbs (seed 0x48475803), operands in HBM.
     Batched (hgx_mul_batch_device, grouped by transform length) against the
     per-call path (hgx_mul_device per product); outputs compared.  With the
-    ALU PROBE's rates, the time of the level's radix-2 butterflies and
-    pointwise products alone (3 primes; the twiddle products between passes
-    not counted, so a lower bound) and the batched time's fraction of it."""
+    ALU PROBE's rates, the time of the level's modular arithmetic (3 primes:
+    radix-2 butterflies, pointwise products, and for transforms above one
+    4096-word tile the twiddle products of the one column pass each such
+    length has up to 2^20, one per word of each forward and of the inverse
+    transform) and the batched time's fraction of it."""
     na, nb = W.batch_shapes(count, W.BATCH_SEED)
     tot = sum(na) + sum(nb)
     IN, OUT1, OUT2 = Dev(_lib, tot), Dev(_lib, tot), Dev(_lib, tot)
@@ -267,13 +269,15 @@ def batched_leg(h, _lib, count: int = 4096, reps: int = 3, probe: dict | None = 
            "batched_limbs_per_s": tot / b, "per_call_limbs_per_s": tot / p,
            "speedup": round(p / b, 2), "outputs_identical": same}
     if probe:
-        ops = {"fwd_butterfly": 0, "inv_butterfly": 0, "mont": 0}
+        ops = {"fwd_butterfly": 0, "inv_butterfly": 0, "mont": 0, "shoup": 0}
         for x, y in zip(na, nb):
             n = (x + y - 1).bit_length()
             L = 1 << n
             ops["fwd_butterfly"] += 3 * 2 * (L // 2) * n
             ops["inv_butterfly"] += 3 * (L // 2) * n
             ops["mont"] += 3 * L
+            if n > 12:
+                ops["shoup"] += 3 * 3 * L
         fl = alu_floor_s(ops, probe)
         res["alu_floor_ms"] = round(1e3 * fl, 3)
         res["alu_frac"] = round(fl / b, 4)
