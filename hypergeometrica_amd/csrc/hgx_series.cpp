@@ -361,6 +361,8 @@ int parallel_for(size_t n, const std::function<int(size_t)> &fn) {
         rc = fn(i);
       } catch (const std::bad_alloc &) {
         rc = hgx_internal_fail(HGX_ENOMEM, "out of host memory in binary splitting");
+      } catch (const std::exception &e) {
+        rc = hgx_internal_fail(HGX_EINVAL, e.what());
       }
       if (rc != HGX_OK) {
         int expect = HGX_OK;
@@ -372,9 +374,14 @@ int parallel_for(size_t n, const std::function<int(size_t)> &fn) {
   if (nt == 1) {
     work();
   } else {
+    // a thread that cannot be created leaves its share to the others (the
+    // work is taken from one counter), so creation failures cost only speed
     std::vector<std::thread> th;
-    th.reserve(nt - 1);
-    for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
+    try {
+      th.reserve(nt - 1);
+      for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
+    } catch (const std::exception &) {
+    }
     work();
     for (auto &t : th) t.join();
   }
@@ -399,24 +406,32 @@ int mul_many(const std::vector<const Big *> &xs, const std::vector<const Big *> 
   int rc_gpu = HGX_OK;
   std::string gpu_err;
   std::thread gpu_thread;
+  auto gpu_call = [&]() {
+    try {
+      std::vector<hgx_mul_desc> d(gpu.size());
+      for (size_t k = 0; k < gpu.size(); k++) {
+        const size_t i = gpu[k];
+        const Big &x = *xs[i], &y = *ys[i];
+        d[k] = hgx_mul_desc{x.m.data(), x.m.size(), y.m.data(), y.m.size(), out[i].m.data()};
+      }
+      rc_gpu = hgx_mul_batch(d.data(), d.size());
+      if (rc_gpu) gpu_err = hgx_last_error();
+    } catch (const std::bad_alloc &) {
+      rc_gpu = HGX_ENOMEM;
+      gpu_err = "out of host memory in binary splitting";
+    } catch (const std::exception &e) {
+      rc_gpu = HGX_EINVAL;
+      gpu_err = e.what();
+    }
+  };
   if (!gpu.empty()) {
     if (!hgx_internal_ready())
       return hgx_internal_fail(HGX_ENOTINIT, "hgx_init has not been called (a series product needs the GPU)");
-    gpu_thread = std::thread([&]() {
-      try {
-        std::vector<hgx_mul_desc> d(gpu.size());
-        for (size_t k = 0; k < gpu.size(); k++) {
-          const size_t i = gpu[k];
-          const Big &x = *xs[i], &y = *ys[i];
-          d[k] = hgx_mul_desc{x.m.data(), x.m.size(), y.m.data(), y.m.size(), out[i].m.data()};
-        }
-        rc_gpu = hgx_mul_batch(d.data(), d.size());
-        if (rc_gpu) gpu_err = hgx_last_error();
-      } catch (const std::bad_alloc &) {
-        rc_gpu = HGX_ENOMEM;
-        gpu_err = "out of host memory in binary splitting";
-      }
-    });
+    try {
+      gpu_thread = std::thread(gpu_call);
+    } catch (const std::exception &) {
+      gpu_call();  // no thread: the batch runs before the host products
+    }
   }
   const int rc_host = parallel_for(host.size(), [&](size_t k) {
     const size_t i = host[k];
