@@ -28,6 +28,7 @@
 #include <exception>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <utility>
 #include <string>
@@ -66,14 +67,10 @@ typedef std::vector<u64, NoInit<u64>> Limbs;
 int hgx_internal_fail(int code, const char *msg);
 bool hgx_internal_ready();
 
-struct hgx_partial {
-  int sign[4];
-  Limbs mag[4];  // P, Q, B, R: trimmed magnitudes
-};
-
 namespace {
 
 constexpr u64 kFoldTerms = 32;
+constexpr size_t kLeafChunk = 512;  // leaves folded by one task
 constexpr size_t kHostMaxLimbs = 100;  // *ntt-multiply-threshold*, src/multiply.lisp:9
 constexpr int kMaxDeg = 7;
 constexpr int kSmallLimbs = 16;  // term values and products of two of them
@@ -85,8 +82,8 @@ struct Small {
   u64 m[kSmallLimbs] = {};
 };
 
-// A signed integer of any size.
-struct Big {
+// A signed integer of any size, grown in place (a leaf's fold).
+struct Acc {
   int s = 0;
   Limbs m;
 };
@@ -160,7 +157,7 @@ void mag_sub_from(Limbs &x, const u64 *y, size_t ny) {
 }
 
 // x = x + y for signed x, y (mpz-+)
-void big_add(Big &x, const Big &y) {
+void big_add(Acc &x, const Acc &y) {
   if (y.s == 0) return;
   if (x.s == 0) {
     x = y;
@@ -185,7 +182,7 @@ void big_add(Big &x, const Big &y) {
 }
 
 // x = x * v for a small v; TMP is scratch
-void big_mul_small(Big &x, const Small &v, Limbs &tmp) {
+void big_mul_small(Acc &x, const Small &v, Limbs &tmp) {
   if (x.s == 0) return;
   if (v.s == 0) {
     x.s = 0;
@@ -199,8 +196,8 @@ void big_mul_small(Big &x, const Small &v, Limbs &tmp) {
   x.s *= v.s;
 }
 
-Big big_of(const Small &v) {
-  Big b;
+Acc big_of(const Small &v) {
+  Acc b;
   b.s = v.s;
   b.m.assign(v.m, v.m + v.n);
   return b;
@@ -293,15 +290,15 @@ struct Series {
   Poly a, b, p, q;
 };
 
-struct Part {
-  Big P, Q, B, R;
+struct AccPart {
+  Acc P, Q, B, R;
 };
 
 // The partial over [lo, up): the base case at LO (binary-split-base-case=1,
 // src/binary-splitting.lisp:100-110) combined with the base cases lo+1, ...
 // one at a time on the right: COMBINE with (p_k, q_k, b_k, p_k a_k) gives
 // R <- b_k q_k R + B P p_k a_k, P <- P p_k, Q <- Q q_k, B <- B b_k.
-int fold(const Series &s, u64 lo, u64 up, Part &out) {
+int fold(const Series &s, u64 lo, u64 up, AccPart &out) {
   Small pk, qk, bk, ak, t1, t2;
   int rc = poly_eval(s.p, lo, pk);
   if (!rc) rc = poly_eval(s.q, lo, qk);
@@ -313,7 +310,7 @@ int fold(const Series &s, u64 lo, u64 up, Part &out) {
   out.Q = big_of(qk);
   out.B = big_of(bk);
   out.R = big_of(t1);
-  Big BP = out.B;  // B * P of the terms so far
+  Acc BP = out.B;  // B * P of the terms so far
   Limbs tmp;
   big_mul_small(BP, pk, tmp);
   for (u64 k = lo + 1; k < up; k++) {
@@ -325,7 +322,7 @@ int fold(const Series &s, u64 lo, u64 up, Part &out) {
     if (!rc) rc = small_mul(pk, ak, t2);
     if (rc) return rc;
     big_mul_small(out.R, t1, tmp);
-    Big x = BP;
+    Acc x = BP;
     big_mul_small(x, t2, tmp);
     big_add(out.R, x);
     rc = small_mul(bk, pk, t1);
@@ -389,19 +386,196 @@ int parallel_for(size_t n, const std::function<int(size_t)> &fn) {
   return HGX_OK;
 }
 
-// The products OUT[i] = X[i] * Y[i]: on the GPU in one batched call when the
-// smaller magnitude has more than kHostMaxLimbs limbs, by schoolbook on the
-// host threads otherwise (while the GPU call runs).
-int mul_many(const std::vector<const Big *> &xs, const std::vector<const Big *> &ys, std::vector<Big> &out) {
+// ---- storage of the tree's values ----
+// The values of the tree live in regions: one block of limbs holds every
+// product of one round of one height, or the partials of a chunk of folded
+// leaves.  A value keeps its region alive; a region that no value refers to
+// goes back to the split's cache, and a later height takes a cached block of
+// about the size it needs.  Each height's products take about the same
+// number of limbs in all, so after the first heights the split reuses
+// memory it has written before: no allocation or free per value, and no
+// fresh pages for the kernel to fault in and zero.
+class Cache {
+ public:
+  ~Cache() {
+    for (int i = 0; i < n_; i++) std::free(base_[i]);
+  }
+  // the smallest cached block of WANT..2*WANT limbs, or null
+  u64 *take(size_t want, size_t &got) {
+    std::lock_guard<std::mutex> g(mu_);
+    int best = -1;
+    for (int i = 0; i < n_; i++)
+      if (cap_[i] >= want && cap_[i] / 2 <= want && (best < 0 || cap_[i] < cap_[best])) best = i;
+    if (best < 0) return nullptr;
+    u64 *p = base_[best];
+    got = cap_[best];
+    n_--;
+    base_[best] = base_[n_];
+    cap_[best] = cap_[n_];
+    return p;
+  }
+  // keeps P for reuse (up to kMax blocks), or frees it
+  void give(u64 *p, size_t cap) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (n_ < kMax) {
+        base_[n_] = p;
+        cap_[n_] = cap;
+        n_++;
+        return;
+      }
+    }
+    std::free(p);
+  }
+
+ private:
+  static constexpr int kMax = 64;
+  std::mutex mu_;
+  u64 *base_[kMax];
+  size_t cap_[kMax];
+  int n_ = 0;
+};
+
+struct Region {
+  u64 *base = nullptr;
+  size_t cap = 0;
+  std::weak_ptr<Cache> cache;  // where the block goes back to, while the split runs
+  ~Region() {
+    if (!base) return;
+    if (auto c = cache.lock())
+      c->give(base, cap);
+    else
+      std::free(base);
+  }
+};
+typedef std::shared_ptr<Region> RegionRef;
+
+// a region of at least LIMBS limbs (throws std::bad_alloc)
+RegionRef new_region(const std::shared_ptr<Cache> &cache, size_t limbs) {
+  RegionRef r = std::make_shared<Region>();
+  limbs = std::max<size_t>(limbs, 1);
+  size_t got = 0;
+  u64 *p = cache ? cache->take(limbs, got) : nullptr;
+  if (!p) {
+    p = static_cast<u64 *>(std::malloc(8 * limbs));
+    if (!p) throw std::bad_alloc();
+    got = limbs;
+  }
+  r->base = p;
+  r->cap = got;
+  r->cache = cache;
+  return r;
+}
+
+// A signed integer in a slot of CAP limbs of a region.
+struct Num {
+  int s = 0;
+  size_t n = 0;  // limbs in use (trimmed)
+  size_t cap = 0;
+  u64 *p = nullptr;
+  RegionRef own;
+};
+
+struct Part {
+  Num P, Q, B, R;
+};
+
+void num_trim(Num &x) {
+  while (x.n > 0 && x.p[x.n - 1] == 0) x.n--;
+  if (x.n == 0) x.s = 0;
+}
+
+// x = x + y for signed x, y (mpz-+), in x's slot, which holds at least
+// max(|x|, |y|) + 1 limbs
+int num_add(Num &x, const Num &y) {
+  if (y.s == 0) return HGX_OK;
+  const size_t xn = x.s ? x.n : 0;
+  if (x.cap < std::max(xn, y.n) + 1) return hgx_internal_fail(HGX_EINVAL, "binary splitting: sum slot too small");
+  if (xn == 0) {
+    std::copy(y.p, y.p + y.n, x.p);
+    x.n = y.n;
+    x.s = y.s;
+    return HGX_OK;
+  }
+  if (x.s == y.s) {  // |x| += |y|
+    if (x.n < y.n) {
+      std::fill(x.p + x.n, x.p + y.n, 0);
+      x.n = y.n;
+    }
+    u64 c = 0;
+    size_t i = 0;
+    for (; i < y.n; i++) {
+      const u128 t = (u128)x.p[i] + y.p[i] + c;
+      x.p[i] = (u64)t;
+      c = (u64)(t >> 64);
+    }
+    for (; c && i < x.n; i++) {
+      x.p[i] += 1;
+      c = (x.p[i] == 0);
+    }
+    if (c) x.p[x.n++] = 1;
+    return HGX_OK;
+  }
+  const int c = mag_cmp(x.p, x.n, y.p, y.n);
+  if (c == 0) {
+    x.s = 0;
+    x.n = 0;
+    return HGX_OK;
+  }
+  if (c > 0) {  // |x| -= |y|
+    u64 b = 0;
+    size_t i = 0;
+    for (; i < y.n; i++) {
+      const u64 xi = x.p[i], t = xi - y.p[i];
+      const u64 b2 = (xi < y.p[i]) || (t < b);
+      x.p[i] = t - b;
+      b = b2;
+    }
+    for (; b && i < x.n; i++) {
+      b = (x.p[i] == 0);
+      x.p[i] -= 1;
+    }
+  } else {  // |x| = |y| - |x|, with |x| < |y|
+    std::fill(x.p + x.n, x.p + y.n, 0);
+    u64 b = 0;
+    for (size_t i = 0; i < y.n; i++) {
+      const u64 yi = y.p[i], t = yi - x.p[i];
+      const u64 b2 = (yi < x.p[i]) || (t < b);
+      x.p[i] = t - b;
+      b = b2;
+    }
+    x.n = y.n;
+    x.s = y.s;
+  }
+  num_trim(x);
+  return HGX_OK;
+}
+
+// The products OUT[i] = X[i] * Y[i] in one new region, slot i of ROOM[i] >=
+// |X[i]| + |Y[i]| limbs: on the GPU in one batched call when the smaller
+// magnitude has more than kHostMaxLimbs limbs, by schoolbook on the host
+// threads otherwise (while the GPU call runs).
+int mul_many(const std::shared_ptr<Cache> &cache, const std::vector<const Num *> &xs,
+             const std::vector<const Num *> &ys, const std::vector<size_t> &room, std::vector<Num> &out) {
   const size_t n = xs.size();
-  out.assign(n, Big());
+  size_t total = 0;
+  for (size_t i = 0; i < n; i++) total += room[i];
+  const RegionRef reg = new_region(cache, total);
+  out.assign(n, Num());
   std::vector<size_t> gpu, host;
+  size_t at = 0;
   for (size_t i = 0; i < n; i++) {
-    const Big &x = *xs[i], &y = *ys[i];
+    const Num &x = *xs[i], &y = *ys[i];
+    Num &o = out[i];
+    o.p = reg->base + at;
+    o.cap = room[i];
+    o.own = reg;
+    at += room[i];
     if (x.s == 0 || y.s == 0) continue;
-    out[i].s = x.s * y.s;
-    out[i].m.resize(x.m.size() + y.m.size());
-    (std::min(x.m.size(), y.m.size()) > kHostMaxLimbs ? gpu : host).push_back(i);
+    if (o.cap < x.n + y.n) return hgx_internal_fail(HGX_EINVAL, "binary splitting: product slot too small");
+    o.s = x.s * y.s;
+    o.n = x.n + y.n;
+    (std::min(x.n, y.n) > kHostMaxLimbs ? gpu : host).push_back(i);
   }
   int rc_gpu = HGX_OK;
   std::string gpu_err;
@@ -411,8 +585,8 @@ int mul_many(const std::vector<const Big *> &xs, const std::vector<const Big *> 
       std::vector<hgx_mul_desc> d(gpu.size());
       for (size_t k = 0; k < gpu.size(); k++) {
         const size_t i = gpu[k];
-        const Big &x = *xs[i], &y = *ys[i];
-        d[k] = hgx_mul_desc{x.m.data(), x.m.size(), y.m.data(), y.m.size(), out[i].m.data()};
+        const Num &x = *xs[i], &y = *ys[i];
+        d[k] = hgx_mul_desc{x.p, x.n, y.p, y.n, out[i].p};
       }
       rc_gpu = hgx_mul_batch(d.data(), d.size());
       if (rc_gpu) gpu_err = hgx_last_error();
@@ -435,17 +609,18 @@ int mul_many(const std::vector<const Big *> &xs, const std::vector<const Big *> 
   }
   const int rc_host = parallel_for(host.size(), [&](size_t k) {
     const size_t i = host[k];
-    const Big &x = *xs[i], &y = *ys[i];
-    if (x.m.size() >= y.m.size())
-      mag_mul(x.m.data(), x.m.size(), y.m.data(), y.m.size(), out[i].m.data());
+    const Num &x = *xs[i], &y = *ys[i];
+    if (x.n >= y.n)
+      mag_mul(x.p, x.n, y.p, y.n, out[i].p);
     else
-      mag_mul(y.m.data(), y.m.size(), x.m.data(), x.m.size(), out[i].m.data());
+      mag_mul(y.p, y.n, x.p, x.n, out[i].p);
     return (int)HGX_OK;
   });
   if (gpu_thread.joinable()) gpu_thread.join();
   if (rc_gpu) return hgx_internal_fail(rc_gpu, gpu_err.c_str());
   if (rc_host) return rc_host;
-  for (size_t i = 0; i < n; i++) trim(out[i].m);
+  for (size_t i = 0; i < n; i++)
+    if (out[i].s) num_trim(out[i]);
   return HGX_OK;
 }
 
@@ -473,6 +648,11 @@ int parse_poly(const hgx_series &s, int which, Poly &f) {
 
 }  // namespace
 
+// P, Q, B, R in one region of their own
+struct hgx_partial {
+  Num v[4];
+};
+
 extern "C" {
 
 static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, hgx_partial **out);
@@ -498,11 +678,45 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
   if (!rc) rc = parse_poly(*desc, 2, s.p);
   if (!rc) rc = parse_poly(*desc, 3, s.q);
   if (rc) return rc;
+  // declared first, so that every region goes back to it before it frees
+  // its blocks
+  const std::shared_ptr<Cache> cache = std::make_shared<Cache>();
   std::vector<std::array<u64, 4>> nodes;
   std::vector<std::pair<u64, u64>> blocks;
   build_tree(lower, upper, nodes, blocks);
+  // the leaves, folded in chunks: one thread folds a chunk's leaves, copies
+  // their partials into one region and frees its own fold vectors
   std::vector<Part> leaf(blocks.size());
-  rc = parallel_for(blocks.size(), [&](size_t i) { return fold(s, blocks[i].first, blocks[i].second, leaf[i]); });
+  const size_t nchunk = (blocks.size() + kLeafChunk - 1) / kLeafChunk;
+  rc = parallel_for(nchunk, [&](size_t c) {
+    const size_t i0 = c * kLeafChunk, i1 = std::min(blocks.size(), i0 + kLeafChunk);
+    std::vector<AccPart> acc(i1 - i0);
+    size_t total = 0;
+    for (size_t i = i0; i < i1; i++) {
+      AccPart &a = acc[i - i0];
+      const int r = fold(s, blocks[i].first, blocks[i].second, a);
+      if (r) return r;
+      total += a.P.m.size() + a.Q.m.size() + a.B.m.size() + a.R.m.size();
+    }
+    const RegionRef reg = new_region(cache, total);
+    size_t at = 0;
+    for (size_t i = i0; i < i1; i++) {
+      AccPart &a = acc[i - i0];
+      const Acc *src[4] = {&a.P, &a.Q, &a.B, &a.R};
+      Num *dst[4] = {&leaf[i].P, &leaf[i].Q, &leaf[i].B, &leaf[i].R};
+      for (int k = 0; k < 4; k++) {
+        Num &d = *dst[k];
+        d.s = src[k]->s;
+        d.n = d.s ? src[k]->m.size() : 0;
+        d.cap = d.n;
+        d.p = reg->base + at;
+        d.own = reg;
+        std::copy(src[k]->m.begin(), src[k]->m.begin() + d.n, d.p);
+        at += d.n;
+      }
+    }
+    return (int)HGX_OK;
+  });
   if (rc) return rc;
   // partials by their interval's lower end (children of a node start at lo
   // and mid, and no two live partials share a lower end)
@@ -515,15 +729,18 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     auto it = std::lower_bound(live.begin(), live.end(), lo, [](const auto &x, u64 v) { return x.first < v; });
     return it->second;
   };
+  auto prod_limbs = [](const Num &x, const Num &y) -> size_t { return x.s && y.s ? x.n + y.n : 0; };
   std::stable_sort(nodes.begin(), nodes.end(), [](const auto &x, const auto &y) { return x[0] < y[0]; });
   for (size_t h0 = 0; h0 < nodes.size();) {
     size_t h1 = h0;
     while (h1 < nodes.size() && nodes[h1][0] == nodes[h0][0]) h1++;
     const size_t cnt = h1 - h0;
     // round 1: P_l P_r, Q_l Q_r, B_l B_r, Q_r R_l, P_l R_r (combine, :112-121)
-    std::vector<const Big *> xs, ys;
+    std::vector<const Num *> xs, ys;
+    std::vector<size_t> room;
     xs.reserve(5 * cnt);
     ys.reserve(5 * cnt);
+    room.reserve(5 * cnt);
     std::vector<Part *> L(cnt), R(cnt);
     for (size_t i = 0; i < cnt; i++) {
       L[i] = &find(nodes[h0 + i][1]);
@@ -533,33 +750,41 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
                       std::make_pair(&r.Q, &l.R), std::make_pair(&l.P, &r.R)}) {
         xs.push_back(pr.first);
         ys.push_back(pr.second);
+        room.push_back(prod_limbs(*pr.first, *pr.second));
       }
     }
-    std::vector<Big> o1;
-    rc = mul_many(xs, ys, o1);
+    std::vector<Num> o1;
+    rc = mul_many(cache, xs, ys, room, o1);
     if (rc) return rc;
-    // round 2: B_r (Q_r R_l), B_l (P_l R_r)
+    // round 2: B_r (Q_r R_l), B_l (P_l R_r); the first slot of each node
+    // also holds the sum R
     xs.clear();
     ys.clear();
+    room.clear();
     for (size_t i = 0; i < cnt; i++) {
-      xs.push_back(&R[i]->B);
+      const Num &br = R[i]->B, &bl = L[i]->B;
+      xs.push_back(&br);
       ys.push_back(&o1[5 * i + 3]);
-      xs.push_back(&L[i]->B);
+      xs.push_back(&bl);
       ys.push_back(&o1[5 * i + 4]);
+      const size_t a = prod_limbs(br, o1[5 * i + 3]), b = prod_limbs(bl, o1[5 * i + 4]);
+      room.push_back(std::max(a, b) + 1);
+      room.push_back(b);
     }
-    std::vector<Big> o2;
-    rc = mul_many(xs, ys, o2);
+    std::vector<Num> o2;
+    rc = mul_many(cache, xs, ys, room, o2);
     if (rc) return rc;
     // R = B_r Q_r R_l + B_l P_l R_r; the node's partial replaces its left
     // child's slot (same lower end), the right child's slot is emptied
     rc = parallel_for(cnt, [&](size_t i) {
-      Part &l = *L[i], &r = *R[i];
-      big_add(o2[2 * i], o2[2 * i + 1]);
+      const int r = num_add(o2[2 * i], o2[2 * i + 1]);
+      if (r) return r;
+      Part &l = *L[i], &rt = *R[i];
       l.P = std::move(o1[5 * i]);
       l.Q = std::move(o1[5 * i + 1]);
       l.B = std::move(o1[5 * i + 2]);
       l.R = std::move(o2[2 * i]);
-      r = Part();
+      rt = Part();
       return (int)HGX_OK;
     });
     if (rc) return rc;
@@ -573,23 +798,40 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
                live.end());
     h0 = h1;
   }
-  Part &top = find(lower);
-  hgx_partial *res = new (std::nothrow) hgx_partial;
+  // the result gets a region of its own: the last height's regions also
+  // hold that height's intermediate products
+  const Part &top = find(lower);
+  std::unique_ptr<hgx_partial> res(new (std::nothrow) hgx_partial);
   if (!res) return hgx_internal_fail(HGX_ENOMEM, "out of host memory for the series result");
-  Big *src[4] = {&top.P, &top.Q, &top.B, &top.R};
-  for (int i = 0; i < 4; i++) {
-    res->sign[i] = src[i]->s;
-    res->mag[i] = std::move(src[i]->m);
+  const Num *src[4] = {&top.P, &top.Q, &top.B, &top.R};
+  size_t total = 0;
+  for (int k = 0; k < 4; k++) total += src[k]->n;
+  const RegionRef reg = new_region(nullptr, total);
+  size_t at = 0;
+  for (int k = 0; k < 4; k++) {
+    Num &d = res->v[k];
+    d.s = src[k]->s;
+    d.n = src[k]->n;
+    d.cap = d.n;
+    d.p = reg->base + at;
+    d.own = reg;
+    at += d.n;
   }
-  *out = res;
+  rc = parallel_for(4, [&](size_t k) {
+    std::copy(src[k]->p, src[k]->p + src[k]->n, res->v[k].p);
+    return (int)HGX_OK;
+  });
+  if (rc) return rc;
+  *out = res.release();
   return HGX_OK;
 }
 
 int hgx_partial_get(const hgx_partial *x, int which, int *sign, size_t *nlimbs, uint64_t *limbs) {
   if (!x || which < 0 || which > 3) return hgx_internal_fail(HGX_EINVAL, "bad partial or component index");
-  if (sign) *sign = x->sign[which];
-  if (nlimbs) *nlimbs = x->mag[which].size();
-  if (limbs && !x->mag[which].empty()) std::memcpy(limbs, x->mag[which].data(), 8 * x->mag[which].size());
+  const Num &v = x->v[which];
+  if (sign) *sign = v.s;
+  if (nlimbs) *nlimbs = v.n;
+  if (limbs && v.n) std::memcpy(limbs, v.p, 8 * v.n);
   return HGX_OK;
 }
 
