@@ -9,6 +9,7 @@
 // recomputes each limb with its incoming carry.
 #include "hgx_kernels.hpp"
 #include "hgx_launch.hpp"
+#include "hgx_sum.h"
 
 namespace {
 
@@ -163,7 +164,188 @@ __global__ __launch_bounds__(256) void k_shl(const u64 *a, u64 n, int s, u64 *ou
   }
 }
 
+// ---- batched signed sums (HgxSumJob) ----
+// The results' blocks of kAsBlock limbs are numbered across the jobs: job j
+// owns blocks blk0 .. blk0 + ceil(n / kAsBlock) - 1.  Per job, RES holds
+// four words: 1 + the top limb where |X| and |Y| differ (subtractions), 1 +
+// the top non-zero limb of the sum, its sign (0, 1, or 2 for -1), and
+// whether a subtraction takes |Y| - |X| (1) rather than |X| - |Y| (0).
+constexpr int kSumWords = 4;
+
+// the job of block GB: the last job whose first block is at most GB (found
+// by thread 0, shared through SH[0])
+__device__ __forceinline__ u64 sum_job(const HgxSumJob *jobs, u64 njobs, u64 gb, u64 *sh) {
+  if (threadIdx.x == 0) {
+    u64 lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const u64 mid = (lo + hi + 1) / 2;
+      if (jobs[mid].blk0 <= gb)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    sh[0] = lo;
+  }
+  __syncthreads();
+  const u64 j = sh[0];
+  __syncthreads();
+  return j;
+}
+
+__device__ __forceinline__ bool sum_is_sub(const HgxSumJob &J) { return J.sx * J.sy < 0; }
+
+// top differing limb of |X| and |Y| for the subtractions
+__global__ __launch_bounds__(HGX_NT) void k_sums_top(const HgxSumJob *jobs, u64 njobs, unsigned long long *res) {
+  __shared__ u64 sh[kScanThreads];
+  const u64 j = sum_job(jobs, njobs, blockIdx.x, sh);
+  const HgxSumJob J = jobs[j];
+  if (!sum_is_sub(J)) return;  // the whole workgroup
+  const u64 i0 = ((u64)blockIdx.x - J.blk0) * kAsBlock + (u64)threadIdx.x * kAsPerThread;
+  u64 best = 0;
+#pragma unroll
+  for (int k = 0; k < kAsPerThread; k++) {
+    const u64 i = i0 + k;
+    if (i < J.n && limb(J.x, J.nx, i) != limb(J.y, J.ny, i)) best = i + 1;
+  }
+  const u64 m = block_max(best, sh);
+  if (threadIdx.x == 0 && m) atomicMax(&res[kSumWords * j], (unsigned long long)m);
+}
+
+// the operands in the order of the operation: a subtraction takes the larger
+// magnitude first (SWAP: |Y| > |X|)
+struct SumOps {
+  const u64 *a, *b;
+  u64 na, nb;
+  bool sub;
+};
+__device__ __forceinline__ SumOps sum_ops(const HgxSumJob &J, bool swap) {
+  const u64 nx = J.sx ? J.nx : 0;
+  SumOps o{J.x, J.y, nx, J.ny, sum_is_sub(J)};
+  if (o.sub && swap) {
+    o.a = J.y;
+    o.b = J.x;
+    o.na = J.ny;
+    o.nb = nx;
+  }
+  return o;
+}
+
+// each block's carry map; the job's first block records the order
+__global__ __launch_bounds__(HGX_NT) void k_sums_blocks(const HgxSumJob *jobs, u64 njobs, unsigned long long *res,
+                                                        uint8_t *agg) {
+  __shared__ u64 shj[kScanThreads];
+  const u64 j = sum_job(jobs, njobs, blockIdx.x, shj);
+  const HgxSumJob J = jobs[j];
+  bool swap = false;
+  if (sum_is_sub(J)) {
+    const u64 t = res[kSumWords * j];  // 0: equal magnitudes
+    if (t) swap = limb(J.x, J.nx, t - 1) < limb(J.y, J.ny, t - 1);
+    if (threadIdx.x == 0 && (u64)blockIdx.x == J.blk0) res[kSumWords * j + 3] = swap ? 1 : 0;
+  }
+  const SumOps o = sum_ops(J, swap);
+  u32 *sh = (u32 *)shj;
+  const u64 i0 = ((u64)blockIdx.x - J.blk0) * kAsBlock + (u64)threadIdx.x * kAsPerThread;
+  u32 summary = 2u;
+#pragma unroll
+  for (int k = 0; k < kAsPerThread; k++) {
+    const u64 i = i0 + k;
+    if (i < J.n) {
+      u32 m;
+      if (o.sub)
+        (void)limb_op<true>(limb(o.a, o.na, i), limb(o.b, o.nb, i), m);
+      else
+        (void)limb_op<false>(limb(o.a, o.na, i), limb(o.b, o.nb, i), m);
+      summary = map_combine(summary, m);
+    }
+  }
+  sh[threadIdx.x] = summary;
+  __syncthreads();
+  for (int st = 1; st < HGX_NT; st <<= 1) {
+    if ((threadIdx.x & (2 * st - 1)) == 0) sh[threadIdx.x] = map_combine(sh[threadIdx.x], sh[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) agg[blockIdx.x] = (uint8_t)sh[0];
+}
+
+// carry into every block of one job per workgroup
+__global__ __launch_bounds__(HGX_NT) void k_sums_scan(const HgxSumJob *jobs, const uint8_t *agg, uint8_t *blkin) {
+  __shared__ u32 sh[HGX_NT];
+  const HgxSumJob J = jobs[blockIdx.x];
+  const u64 nb = (J.n + kAsBlock - 1) / kAsBlock;
+  const u64 chunk = (nb + HGX_NT - 1) / HGX_NT;
+  const u64 b0 = (u64)threadIdx.x * chunk;
+  const u64 b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  u32 summary = 2u;
+  for (u64 b = b0; b < b1; b++) summary = map_combine(summary, agg[J.blk0 + b]);
+  u32 state = block_exclusive_last(summary, sh);
+  for (u64 b = b0; b < b1; b++) {
+    blkin[J.blk0 + b] = (state == 3u) ? 1 : 0;
+    state = map_combine(state, agg[J.blk0 + b]);
+  }
+}
+
+// every limb with its incoming carry, written over X; the size and sign
+__global__ __launch_bounds__(HGX_NT) void k_sums_apply(const HgxSumJob *jobs, u64 njobs, unsigned long long *res,
+                                                       const uint8_t *blkin) {
+  __shared__ u64 sh[kScanThreads];
+  const u64 j = sum_job(jobs, njobs, blockIdx.x, sh);
+  const HgxSumJob J = jobs[j];
+  const bool swap = sum_is_sub(J) && res[kSumWords * j + 3] != 0;
+  const SumOps o = sum_ops(J, swap);
+  const u64 i0 = ((u64)blockIdx.x - J.blk0) * kAsBlock + (u64)threadIdx.x * kAsPerThread;
+  u64 r[kAsPerThread];
+  u32 m[kAsPerThread];
+  u32 summary = 2u;
+#pragma unroll
+  for (int k = 0; k < kAsPerThread; k++) {
+    const u64 i = i0 + k;
+    r[k] = 0;
+    m[k] = 2u;
+    if (i < J.n) {
+      if (o.sub)
+        r[k] = limb_op<true>(limb(o.a, o.na, i), limb(o.b, o.nb, i), m[k]);
+      else
+        r[k] = limb_op<false>(limb(o.a, o.na, i), limb(o.b, o.nb, i), m[k]);
+      summary = map_combine(summary, m[k]);
+    }
+  }
+  const u32 ex = block_exclusive_last(summary, (u32 *)sh);
+  u64 c = (ex == 3u) ? 1u : (ex == 2u ? (u64)blkin[blockIdx.x] : 0u);
+  u64 best = 0;
+#pragma unroll
+  for (int k = 0; k < kAsPerThread; k++) {
+    const u64 i = i0 + k;
+    if (i < J.n) {
+      const u64 v = o.sub ? r[k] - c : r[k] + c;
+      J.x[i] = v;
+      if (v) best = i + 1;
+      c = (m[k] == 2u) ? c : (m[k] == 3u ? 1u : 0u);
+    }
+  }
+  const u64 top = block_max(best, sh);
+  if (threadIdx.x == 0) {
+    if (top) atomicMax(&res[kSumWords * j + 1], (unsigned long long)top);
+    if ((u64)blockIdx.x == J.blk0) {
+      int sg = J.sy;  // a sum of magnitudes takes Y's sign (X's, when X is not zero)
+      if (o.sub) sg = res[kSumWords * j] == 0 ? 0 : (swap ? J.sy : J.sx);
+      res[kSumWords * j + 2] = sg > 0 ? 1u : (sg < 0 ? 2u : 0u);
+    }
+  }
+}
+
 }  // namespace
+
+void hgx_launch_sums(hipStream_t s, const HgxSumJob *jobs, u64 njobs, u64 nblk, uint8_t *agg, uint8_t *blkin,
+                     u64 *res) {
+  const u64 nres = (u64)kSumWords * njobs;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((nres + 255) / 256)), dim3(256), 0, s, res, nres, (u64)0);
+  unsigned long long *r = (unsigned long long *)res;
+  hipLaunchKernelGGL(k_sums_top, dim3((unsigned)nblk), dim3(HGX_NT), 0, s, jobs, njobs, r);
+  hipLaunchKernelGGL(k_sums_blocks, dim3((unsigned)nblk), dim3(HGX_NT), 0, s, jobs, njobs, r, agg);
+  hipLaunchKernelGGL(k_sums_scan, dim3((unsigned)njobs), dim3(HGX_NT), 0, s, jobs, (const uint8_t *)agg, blkin);
+  hipLaunchKernelGGL(k_sums_apply, dim3((unsigned)nblk), dim3(HGX_NT), 0, s, jobs, njobs, r,
+                     (const uint8_t *)blkin);
+}
 
 void hgx_launch_top_diff(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 nb, u64 *res) {
   const u64 n = na > nb ? na : nb;

@@ -44,6 +44,11 @@ void hgx_launch_cmp_sign(hipStream_t s, const u64 *a, u64 na, const u64 *b, u64 
 // out[0..n] = a * 2^bits, 0 <= bits < 64 (out must not overlap a)
 void hgx_launch_shl(hipStream_t s, const u64 *a, u64 n, int bits, u64 *out);
 u64 hgx_addsub_blocks(u64 n);
+// the signed sums of NJOBS jobs (hgx_sum.h) whose blk0 and n are set, NBLK
+// blocks in all; AGG and BLKIN hold NBLK bytes each, RES 4 words per job
+struct HgxSumJob;
+void hgx_launch_sums(hipStream_t s, const HgxSumJob *jobs, u64 njobs, u64 nblk, uint8_t *agg, uint8_t *blkin,
+                     u64 *res);
 // register-only probe of one modular operation (0 Shoup product, 1
 // Montgomery product, 2 sum and difference, 3 forward butterfly, 4 inverse
 // butterfly), hgx_probe_chains() chains per thread, ITERS steps each;

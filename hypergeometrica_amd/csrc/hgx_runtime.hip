@@ -26,6 +26,7 @@
 
 #include "../../include/hgx.h"
 #include "hgx_launch.hpp"
+#include "hgx_sum.h"
 
 typedef unsigned __int128 u128;
 
@@ -2695,6 +2696,58 @@ int mul_host_slot(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, bo
 // has succeeded
 int hgx_internal_fail(int code, const char *msg) { return fail(code, msg ? msg : ""); }
 bool hgx_internal_ready() { return g_ctx.ready; }
+
+size_t hgx_internal_device_room() {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  return g_ctx.ready ? memory_for_product(true) : 0;
+}
+
+int hgx_internal_sums_device(HgxSumJob *jobs, size_t count, int *signs, uint64_t *sizes) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (count == 0) return HGX_OK;
+  if (!jobs || !signs || !sizes) return fail(HGX_EINVAL, "null sum jobs or results");
+  u64 nblk = 0;
+  for (size_t j = 0; j < count; j++) {
+    HgxSumJob &J = jobs[j];
+    if (!J.x || !J.y || J.ny == 0 || J.sy == 0) return fail(HGX_EINVAL, "a signed sum needs a non-zero Y");
+    if (J.sx == 0) J.nx = 0;
+    J.n = std::max(J.nx, J.ny) + 1;
+    J.blk0 = nblk;
+    nblk += hgx_addsub_blocks(J.n);
+  }
+  if (nblk >= ((u64)1 << 31) || count >= ((size_t)1 << 31)) return fail(HGX_ETOOBIG, "too many signed sums");
+  hipStream_t s = g_ctx.stream;
+  rc = ws_acquire(s);
+  if (rc) return rc;
+  const size_t jb = align_up(count * sizeof(HgxSumJob)), rb = align_up(count * 4 * sizeof(u64));
+  rc = ensure(g_ctx.scan, jb + rb + 2 * nblk);
+  if (rc) return rc;
+  char *base = (char *)g_ctx.scan.ptr;
+  HgxSumJob *djobs = (HgxSumJob *)base;
+  u64 *dres = (u64 *)(base + jb);
+  uint8_t *agg = (uint8_t *)(base + jb + rb);
+  std::vector<u64> hres(4 * count);
+  {
+    PinScope pin;
+    rc = h2d(djobs, jobs, count * sizeof(HgxSumJob), s);
+    if (rc) return rc;
+    hgx_launch_sums(s, djobs, count, nblk, agg, agg + nblk, dres);
+    HIPCHK(hipGetLastError());
+    rc = d2h(hres.data(), dres, hres.size() * sizeof(u64), s);
+    if (!rc) rc = pin_flush();
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  rc = ws_release(s);
+  if (rc) return rc;
+  for (size_t j = 0; j < count; j++) {
+    sizes[j] = hres[4 * j + 1];
+    signs[j] = hres[4 * j + 2] == 1 ? 1 : (hres[4 * j + 2] == 2 ? -1 : 0);
+  }
+  return HGX_OK;
+}
 
 // ===========================================================================
 // C ABI

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <functional>
@@ -36,6 +37,7 @@
 #include <vector>
 
 #include "../../include/hgx.h"
+#include "hgx_sum.h"
 
 typedef uint64_t u64;
 typedef unsigned __int128 u128;
@@ -71,6 +73,12 @@ namespace {
 
 constexpr u64 kFoldTerms = 32;
 constexpr size_t kLeafChunk = 512;  // leaves folded by one task
+// a height moves to the GPU when its round-1 products have at least this
+// many limbs, and the device has room for kDeviceRoomFactor times the live
+// values (a height's products take about twice as many limbs as its
+// children, and the children's regions live until the height is done)
+constexpr size_t kDeviceMinLimbs = (size_t)1 << 20;
+constexpr size_t kDeviceRoomFactor = 6;
 constexpr size_t kHostMaxLimbs = 100;  // *ntt-multiply-threshold*, src/multiply.lisp:9
 constexpr int kMaxDeg = 7;
 constexpr int kSmallLimbs = 16;  // term values and products of two of them
@@ -467,14 +475,31 @@ RegionRef new_region(const std::shared_ptr<Cache> &cache, size_t limbs) {
   return r;
 }
 
-// A signed integer in a slot of CAP limbs of a region.
+// A signed integer in a slot of CAP limbs of a region of host memory, or of
+// device memory once the split has moved to the GPU (DEV).  Host values are
+// trimmed; device values may keep a zero top limb from a product.
 struct Num {
   int s = 0;
-  size_t n = 0;  // limbs in use (trimmed)
+  size_t n = 0;  // limbs in use
   size_t cap = 0;
   u64 *p = nullptr;
-  RegionRef own;
+  bool dev = false;
+  std::shared_ptr<void> own;  // the region
 };
+
+// one round's products in device memory (hgx_device_alloc), released after
+// the work queued before the release
+struct DevRegion {
+  u64 *base = nullptr;
+  ~DevRegion() {
+    if (base) (void)hgx_device_free(base, nullptr);
+  }
+};
+
+int new_dev_region(size_t limbs, std::shared_ptr<DevRegion> &out) {
+  out = std::make_shared<DevRegion>();
+  return hgx_device_alloc(std::max<size_t>(limbs, 1), &out->base);
+}
 
 struct Part {
   Num P, Q, B, R;
@@ -624,6 +649,145 @@ int mul_many(const std::shared_ptr<Cache> &cache, const std::vector<const Num *>
   return HGX_OK;
 }
 
+// The products of a height whose values live in device memory: one device
+// region for all, every product in one batched call (hgx_mul_batch_device,
+// queued on the library's stream).  A product keeps all |X| + |Y| limbs.
+int mul_many_dev(const std::vector<const Num *> &xs, const std::vector<const Num *> &ys,
+                 const std::vector<size_t> &room, std::vector<Num> &out) {
+  const size_t n = xs.size();
+  size_t total = 0;
+  for (size_t i = 0; i < n; i++) total += room[i];
+  std::shared_ptr<DevRegion> reg;
+  int rc = new_dev_region(total, reg);
+  if (rc) return rc;
+  out.assign(n, Num());
+  std::vector<hgx_mul_desc> d;
+  d.reserve(n);
+  size_t at = 0;
+  for (size_t i = 0; i < n; i++) {
+    const Num &x = *xs[i], &y = *ys[i];
+    Num &o = out[i];
+    o.p = reg->base + at;
+    o.cap = room[i];
+    o.dev = true;
+    o.own = reg;
+    at += room[i];
+    if (x.s == 0 || y.s == 0) continue;
+    if (!x.dev || !y.dev || o.cap < x.n + y.n)
+      return hgx_internal_fail(HGX_EINVAL, "binary splitting: bad device product slot");
+    o.s = x.s * y.s;
+    o.n = x.n + y.n;
+    d.push_back(hgx_mul_desc{x.p, x.n, y.p, y.n, o.p});
+  }
+  return d.empty() ? HGX_OK : hgx_mul_batch_device(d.data(), d.size(), nullptr);
+}
+
+// R = B_r Q_r R_l + B_l P_l R_r for every node of a device height: O2 holds
+// the pairs (B_r Q_r R_l, B_l P_l R_r); the sums go over the first of each
+// pair (hgx_internal_sums_device), whose sign and trimmed size come back.
+int sums_dev(std::vector<Num> &o2, size_t cnt) {
+  std::vector<HgxSumJob> jobs;
+  std::vector<size_t> idx;
+  jobs.reserve(cnt);
+  idx.reserve(cnt);
+  for (size_t i = 0; i < cnt; i++) {
+    Num &x = o2[2 * i];
+    const Num &y = o2[2 * i + 1];
+    if (y.s == 0) continue;  // R is the first product as it stands
+    const size_t nx = x.s ? x.n : 0;
+    if (x.cap < std::max(nx, y.n) + 1) return hgx_internal_fail(HGX_EINVAL, "binary splitting: sum slot too small");
+    jobs.push_back(HgxSumJob{x.p, y.p, nx, y.n, x.s, y.s, 0, 0});
+    idx.push_back(i);
+  }
+  if (jobs.empty()) return HGX_OK;
+  std::vector<int> sg(jobs.size());
+  std::vector<uint64_t> sz(jobs.size());
+  const int rc = hgx_internal_sums_device(jobs.data(), jobs.size(), sg.data(), sz.data());
+  if (rc) return rc;
+  for (size_t k = 0; k < jobs.size(); k++) {
+    Num &x = o2[2 * idx[k]];
+    x.s = sz[k] ? sg[k] : 0;
+    x.n = x.s ? sz[k] : 0;
+  }
+  return HGX_OK;
+}
+
+// Moves every live value to device memory: one region, packed on the host
+// threads into pinned memory (pageable when that is not available) and
+// uploaded by one copy.
+int to_device(std::vector<std::pair<u64, Part>> &live) {
+  std::vector<Num *> vals;
+  vals.reserve(4 * live.size());
+  std::vector<size_t> off;
+  off.reserve(4 * live.size() + 1);
+  size_t total = 0;
+  for (auto &e : live)
+    for (Num *v : {&e.second.P, &e.second.Q, &e.second.B, &e.second.R}) {
+      vals.push_back(v);
+      off.push_back(total);
+      total += v->s ? v->n : 0;
+    }
+  std::shared_ptr<DevRegion> reg;
+  int rc = new_dev_region(total, reg);
+  if (rc) return rc;
+  void *pinned = nullptr;
+  std::unique_ptr<u64[]> pageable;
+  u64 *pack = nullptr;
+  if (hgx_host_alloc(8 * std::max<size_t>(total, 1), &pinned) == HGX_OK) {
+    pack = static_cast<u64 *>(pinned);
+  } else {
+    pageable.reset(new u64[std::max<size_t>(total, 1)]);
+    pack = pageable.get();
+  }
+  const size_t per = 4096;  // values packed by one task
+  rc = parallel_for((vals.size() + per - 1) / per, [&](size_t t) {
+    for (size_t k = t * per; k < std::min(vals.size(), (t + 1) * per); k++)
+      if (vals[k]->s) std::copy(vals[k]->p, vals[k]->p + vals[k]->n, pack + off[k]);
+    return (int)HGX_OK;
+  });
+  if (!rc && total) rc = hgx_upload(reg->base, pack, total, nullptr);
+  if (pinned) (void)hgx_host_free(pinned);
+  if (rc) return rc;
+  for (size_t k = 0; k < vals.size(); k++) {
+    Num &v = *vals[k];
+    v.p = reg->base + off[k];
+    v.cap = v.s ? v.n : 0;
+    v.dev = true;
+    v.own = reg;
+  }
+  return HGX_OK;
+}
+
+// Whether the split moves to the GPU before the height whose round-1
+// products are XS[i] * YS[i]: once nearly all of a height's product limbs go
+// to the GPU anyway (the smaller operand above kHostMaxLimbs), the values
+// stay in device memory from there on -- no host packing, copies or host
+// sums per height -- when the device has room for several times the live
+// values.  RC receives a failure of the move.
+// HGX_SERIES_DEVICE=0 keeps every height on the host, =1 moves the split to
+// the GPU at its first height (above the folded leaves).
+bool move_to_device(std::vector<std::pair<u64, Part>> &live, const std::vector<const Num *> &xs,
+                    const std::vector<const Num *> &ys, int &rc) {
+  rc = HGX_OK;
+  if (!hgx_internal_ready()) return false;
+  const char *env = std::getenv("HGX_SERIES_DEVICE");
+  const int mode = env && env[0] == '0' ? 0 : (env && env[0] == '1' ? 2 : 1);
+  if (mode == 0) return false;
+  size_t all = 0, gpu = 0;
+  for (size_t i = 0; i < xs.size(); i++) {
+    if (!xs[i]->s || !ys[i]->s) continue;
+    const size_t v = xs[i]->n + ys[i]->n;
+    all += v;
+    if (std::min(xs[i]->n, ys[i]->n) > kHostMaxLimbs) gpu += v;
+  }
+  if (mode == 1 && (all < kDeviceMinLimbs || gpu < all / 10 * 9)) return false;
+  size_t held = 0;
+  for (const auto &e : live) held += e.second.P.n + e.second.Q.n + e.second.B.n + e.second.R.n;
+  if (held > hgx_internal_device_room() / (8 * kDeviceRoomFactor)) return false;
+  rc = to_device(live);
+  return rc == HGX_OK;
+}
+
 // The split tree down to subtrees of at most kFoldTerms terms (BLOCKS);
 // every inner node as (height, lo, mid, up); returns the height of [lo, up).
 int build_tree(u64 lo, u64 up, std::vector<std::array<u64, 4>> &nodes, std::vector<std::pair<u64, u64>> &blocks) {
@@ -730,6 +894,7 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     return it->second;
   };
   auto prod_limbs = [](const Num &x, const Num &y) -> size_t { return x.s && y.s ? x.n + y.n : 0; };
+  bool on_dev = false;
   std::stable_sort(nodes.begin(), nodes.end(), [](const auto &x, const auto &y) { return x[0] < y[0]; });
   for (size_t h0 = 0; h0 < nodes.size();) {
     size_t h1 = h0;
@@ -753,8 +918,12 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
         room.push_back(prod_limbs(*pr.first, *pr.second));
       }
     }
+    if (!on_dev) {
+      on_dev = move_to_device(live, xs, ys, rc);
+      if (rc) return rc;
+    }
     std::vector<Num> o1;
-    rc = mul_many(cache, xs, ys, room, o1);
+    rc = on_dev ? mul_many_dev(xs, ys, room, o1) : mul_many(cache, xs, ys, room, o1);
     if (rc) return rc;
     // round 2: B_r (Q_r R_l), B_l (P_l R_r); the first slot of each node
     // also holds the sum R
@@ -772,12 +941,16 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
       room.push_back(b);
     }
     std::vector<Num> o2;
-    rc = mul_many(cache, xs, ys, room, o2);
+    rc = on_dev ? mul_many_dev(xs, ys, room, o2) : mul_many(cache, xs, ys, room, o2);
     if (rc) return rc;
     // R = B_r Q_r R_l + B_l P_l R_r; the node's partial replaces its left
     // child's slot (same lower end), the right child's slot is emptied
+    if (on_dev) {
+      rc = sums_dev(o2, cnt);
+      if (rc) return rc;
+    }
     rc = parallel_for(cnt, [&](size_t i) {
-      const int r = num_add(o2[2 * i], o2[2 * i + 1]);
+      const int r = on_dev ? (int)HGX_OK : num_add(o2[2 * i], o2[2 * i + 1]);
       if (r) return r;
       Part &l = *L[i], &rt = *R[i];
       l.P = std::move(o1[5 * i]);
@@ -817,11 +990,20 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     d.own = reg;
     at += d.n;
   }
-  rc = parallel_for(4, [&](size_t k) {
-    std::copy(src[k]->p, src[k]->p + src[k]->n, res->v[k].p);
-    return (int)HGX_OK;
-  });
-  if (rc) return rc;
+  if (on_dev) {
+    for (int k = 0; k < 4; k++) {
+      Num &d = res->v[k];
+      if (d.n) rc = hgx_download(d.p, src[k]->p, d.n, nullptr);
+      if (rc) return rc;
+      if (d.s) num_trim(d);
+    }
+  } else {
+    rc = parallel_for(4, [&](size_t k) {
+      std::copy(src[k]->p, src[k]->p + src[k]->n, res->v[k].p);
+      return (int)HGX_OK;
+    });
+    if (rc) return rc;
+  }
   *out = res.release();
   return HGX_OK;
 }

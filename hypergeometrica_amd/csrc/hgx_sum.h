@@ -1,0 +1,28 @@
+// hgx_sum.h -- batched signed sums of device limb vectors, shared by the
+// binary-splitting engine (hgx_series.cpp), the runtime and the kernels.
+// The sum R = B_r Q_r R_l + B_l P_l R_r of COMBINE
+// (src/binary-splitting.lisp:112-121) for every node of a tree height whose
+// values stay in device memory: mpz-+ on signed values
+// (src/mpz-ram.lisp:185-291), magnitudes added, or the smaller subtracted
+// from the larger.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+struct HgxSumJob {
+  uint64_t *x;        // |X|, NX limbs, in a slot of at least max(NX, NY) + 1 limbs: receives the sum
+  const uint64_t *y;  // |Y|, NY limbs (a different slot)
+  uint64_t nx, ny;
+  int sx, sy;     // signs (-1, 0, 1); SY != 0; SX = 0 reads X as zero
+  uint64_t n;     // set by the runtime: limbs written, max(NX, NY) + 1
+  uint64_t blk0;  // set by the runtime: the job's first block of the launch
+};
+
+// Every job's X slot receives SX|X| + SY|Y| (N limbs); SIGNS[j] its sign and
+// SIZES[j] its limb count up to the top non-zero limb.  Queued on the
+// library's stream after the work before it; returns when the sums are
+// done.
+int hgx_internal_sums_device(HgxSumJob *jobs, size_t count, int *signs, uint64_t *sizes);
+
+// device memory the library may still allocate under its HBM budget
+size_t hgx_internal_device_room();

@@ -51,3 +51,52 @@ def test_native_chudnovsky_pi_digits(hgx):
     mpmath.mp.dps = prec + 30
     want = int(mpmath.floor(mpmath.pi / 10 * mpmath.mpf(10) ** prec))
     assert abs(val - want) <= 1
+
+
+@pytest.mark.parametrize("mode", ["0", "1"])
+@pytest.mark.parametrize("name,lo,up", [
+    ("chudnovsky", 0, 3000), ("ramanujan", 0, 3000), ("e", 0, 5000), ("catalan", 0, 4000), ("apery", 0, 2000),
+    ("exp(7/2)", 0, 5000), ("exp(-5/3)", 17, 4017), ("chudnovsky", 1000, 2999),
+])
+def test_native_host_and_device_heights_equal_depth_first(hgx, monkeypatch, mode, name, lo, up):
+    # HGX_SERIES_DEVICE=1: every height above the folded leaves in device
+    # memory (products batched on the device, signed sums by the device);
+    # =0: every height on the host
+    monkeypatch.setenv("HGX_SERIES_DEVICE", mode)
+    s = SERIES[name]()
+    assert _tuple(S.binary_split_native(s, lo, up)) == SO.binary_split(s, lo, up)
+
+
+# a = 1, p = -1 (1 at k = 0), q = b = 1: R over [lo, up) is a sum of +-1 that
+# cancels to zero on many intervals, so device sums meet equal magnitudes
+# of opposite signs (a zero result) and zero operands
+CANCEL = S.Series(a=S.Poly((1,)), p=S.Poly((-1,), at_zero=1))
+
+
+@pytest.mark.parametrize("mode", ["0", "1"])
+@pytest.mark.parametrize("series,lo,up", [
+    (CANCEL, 0, 66), (CANCEL, 0, 130), (CANCEL, 0, 1000), (CANCEL, 5, 1000), (CANCEL, 1, 4097),
+    (S.Series(p=S.Poly((-5, 1), at_zero=1), q=S.Poly((0, 3), at_zero=1)), 0, 400),
+    (S.Series(a=S.Poly((2, -7)), b=S.Poly((-3,)), p=S.Poly((1, 2, -1)), q=S.Poly((-4, 0, 5), at_zero=-1)), 0, 900),
+])
+def test_device_sums_signs_and_cancellation(hgx, monkeypatch, mode, series, lo, up):
+    monkeypatch.setenv("HGX_SERIES_DEVICE", mode)
+    assert _tuple(S.binary_split_native(series, lo, up)) == SO.binary_split(series, lo, up)
+
+
+@pytest.mark.parametrize("name,lo,up", [("chudnovsky", 0, 60000), ("apery", 0, 30000), ("exp(7/2)", 0, 80000)])
+def test_device_heights_equal_python_levels(hgx, monkeypatch, name, lo, up):
+    monkeypatch.setenv("HGX_SERIES_DEVICE", "1")
+    s = SERIES[name]()
+    assert _tuple(S.binary_split_native(s, lo, up)) == _tuple(S.binary_split(s, lo, up))
+
+
+def test_default_move_to_device_equals_host_heights(hgx, monkeypatch):
+    # 400 000 Chudnovsky terms: the upper heights' products exceed 2^20 limbs,
+    # so the default rule moves them to device memory
+    s = S.chudnovsky_series()
+    monkeypatch.delenv("HGX_SERIES_DEVICE", raising=False)
+    dev = _tuple(S.binary_split_native(s, 0, 400000))
+    monkeypatch.setenv("HGX_SERIES_DEVICE", "0")
+    host = _tuple(S.binary_split_native(s, 0, 400000))
+    assert dev == host
