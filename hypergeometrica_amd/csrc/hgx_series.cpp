@@ -10,12 +10,17 @@
 // integers for any split, so a left fold of the base cases gives the
 // recursion's numbers), and then evaluates the tree bottom up, one height at
 // a time.  A height's products go out in two rounds (five products that need
-// only the children, then the two that need round-1 results); a product
-// whose smaller operand has more than *ntt-multiply-threshold* limbs
-// (src/multiply.lisp:9) joins one batched GPU call (hgx_mul_batch), the
-// others are multiplied on the host by schoolbook while the GPU works, as
-// the reference's mpz-* multiplies them (src/multiply.lisp:21-42).  All
-// arithmetic is exact, so every partial equals the reference's.
+// only the children, then the two that need round-1 results).  While the
+// values are in host memory, a product whose smaller operand has more than
+// *ntt-multiply-threshold* limbs (src/multiply.lisp:9) joins one batched GPU
+// call (hgx_mul_batch), the others are multiplied on the host by schoolbook
+// while the GPU works, as the reference's mpz-* multiplies them
+// (src/multiply.lisp:21-42).  A split whose heights hold enough limbs moves
+// its values to device memory at its first height and keeps them there:
+// each round is one batched device call (hgx_mul_batch_device) and the sums
+// R run on the device, with one upload of the folded leaves and one download
+// of the result.  All arithmetic is exact, so every partial equals the
+// reference's.
 //
 // Series terms are polynomials in k with 64-bit signed coefficients and an
 // optional separate value at k = 0, which covers the reference's series
@@ -759,13 +764,15 @@ int to_device(std::vector<std::pair<u64, Part>> &live) {
 }
 
 // Whether the split moves to the GPU before the height whose round-1
-// products are XS[i] * YS[i]: once nearly all of a height's product limbs go
-// to the GPU anyway (the smaller operand above kHostMaxLimbs), the values
-// stay in device memory from there on -- no host packing, copies or host
-// sums per height -- when the device has room for several times the live
-// values.  RC receives a failure of the move.
-// HGX_SERIES_DEVICE=0 keeps every height on the host, =1 moves the split to
-// the GPU at its first height (above the folded leaves).
+// products are XS[i] * YS[i]: once a height's products hold at least
+// kDeviceMinLimbs limbs, the values stay in device memory from there on --
+// every product of a height in one batched device call, small ones too, and
+// no host packing, copies or host sums per height -- when the device has
+// room for several times the live values.  (The products are exact either
+// way; a height's products take about the same number of limbs at every
+// height, so a large split moves at its first height.)  RC receives a
+// failure of the move.  HGX_SERIES_DEVICE=0 keeps every height on the host,
+// =1 moves any split at its first height.
 bool move_to_device(std::vector<std::pair<u64, Part>> &live, const std::vector<const Num *> &xs,
                     const std::vector<const Num *> &ys, int &rc) {
   rc = HGX_OK;
@@ -773,14 +780,10 @@ bool move_to_device(std::vector<std::pair<u64, Part>> &live, const std::vector<c
   const char *env = std::getenv("HGX_SERIES_DEVICE");
   const int mode = env && env[0] == '0' ? 0 : (env && env[0] == '1' ? 2 : 1);
   if (mode == 0) return false;
-  size_t all = 0, gpu = 0;
-  for (size_t i = 0; i < xs.size(); i++) {
-    if (!xs[i]->s || !ys[i]->s) continue;
-    const size_t v = xs[i]->n + ys[i]->n;
-    all += v;
-    if (std::min(xs[i]->n, ys[i]->n) > kHostMaxLimbs) gpu += v;
-  }
-  if (mode == 1 && (all < kDeviceMinLimbs || gpu < all / 10 * 9)) return false;
+  size_t all = 0;
+  for (size_t i = 0; i < xs.size(); i++)
+    if (xs[i]->s && ys[i]->s) all += xs[i]->n + ys[i]->n;
+  if (mode == 1 && all < kDeviceMinLimbs) return false;
   size_t held = 0;
   for (const auto &e : live) held += e.second.P.n + e.second.Q.n + e.second.B.n + e.second.R.n;
   if (held > hgx_internal_device_room() / (8 * kDeviceRoomFactor)) return false;

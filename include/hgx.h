@@ -323,9 +323,14 @@ int hgx_modmul_probe(int op, int iters, double *out);
 // time.  Each height's products whose smaller operand has more than 100
 // limbs (*ntt-multiply-threshold*, src/multiply.lisp:9) go to the GPU in one
 // hgx_mul_batch call; smaller ones are multiplied on the host, as mpz-*
-// does (src/multiply.lisp:37-42).  An interval whose products are all small
-// needs no hgx_init.  Replaces a Lisp-side binary-split whose term functions
-// are polynomials: a(k), b(k), p(k), q(k) are polynomials in k (index 0 a,
+// does (src/multiply.lisp:37-42).  A split whose heights' products hold at
+// least 2^20 limbs instead keeps its values in device memory from its first
+// height on (every product of a round in one hgx_mul_batch_device call, the
+// sums R on the device), with the same result; HGX_SERIES_DEVICE=0 in the
+// environment keeps every split on the host, =1 moves every split.  An
+// interval whose products are all small needs no hgx_init.  Replaces a
+// Lisp-side binary-split whose term functions are polynomials: a(k), b(k),
+// p(k), q(k) are polynomials in k (index 0 a,
 // 1 b, 2 p, 3 q; coefficients constant term first, degree 0..7) or, where
 // AT_ZERO[i] is set, VALUE0[i] at k = 0 (the reference's series give 1 there:
 // make-exp-series, make-e-series, make-ramanujan-series, :140-184;
