@@ -85,6 +85,17 @@ def to_device(x: _mpd.MPD) -> DMPD:
     return DMPD(1 if x.mantissa > 0 else -1, buf, 0, n, x.exponent)
 
 
+def from_limbs(sign: int, limbs: np.ndarray, exponent: int = 0) -> DMPD:
+    """A device value from trimmed host limbs (no Python integer)."""
+    n = int(limbs.size)
+    if sign == 0 or n == 0:
+        return DMPD(0, None, 0, 0, exponent)
+    host = np.ascontiguousarray(limbs, dtype=np.uint64)
+    buf = DBuf(n)
+    check(_L().hgx_upload(buf.ptr, host.ctypes.data, n, None))
+    return DMPD(sign, buf, 0, n, exponent)
+
+
 def to_host(x: DMPD) -> _mpd.MPD:
     if x.sign == 0:
         return _mpd.MPD(0, x.exponent)
@@ -198,16 +209,19 @@ class DeviceContext:
     def mpd_reciprocal(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
         """mpd-reciprocal (src/mpd-reciprocal.lisp:27-73) with A and every
         iterate in device memory."""
+        return to_host(self.reciprocal(to_device(a), bits))
+
+    def reciprocal(self, da: DMPD, bits: int) -> DMPD:
+        """mpd-reciprocal of a device value, left in device memory."""
         digits = max(1, -(-bits // 64))
-        da = to_device(a)
         r = _mpd._round_div(_mpd.BASE, self._top(da))
-        x = to_device(_mpd.MPD(-r if a.mantissa < 0 else r, -a.exponent - da.size))
+        x = to_device(_mpd.MPD(-r if da.sign < 0 else r, -da.exponent - da.size))
         one = self._one()
         while True:
             h = self.mpd_sub(one, self.mpd_mul(da, x))
             zeros, zero = self.mpd_oom(h)
             if zero or -zeros >= digits:
-                return to_host(x)
+                return x
             x = self.mpd_add(x, self.mpd_mul(x, h))
             if self.truncate:
                 x = self.mpd_truncate(x, digits + 2)
@@ -216,6 +230,10 @@ class DeviceContext:
     def mpd_inv_sqrt(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
         """mpd-inv-sqrt (src/mpd-sqrt.lisp:40-87) with A and every iterate in
         device memory; x*x goes to the squaring path."""
+        return to_host(self.inv_sqrt(a, bits))
+
+    def inv_sqrt(self, a: _mpd.MPD, bits: int) -> DMPD:
+        """mpd-inv-sqrt, the result left in device memory."""
         digits = max(1, -(-bits // 64))
         da = to_device(a)
         x = to_device(_mpd.Context.estimate_inv_sqrt(a))
@@ -224,28 +242,32 @@ class DeviceContext:
             h = self.mpd_sub(one, self.mpd_mul(da, self.mpd_mul(x, x)))
             zeros, zero = self.mpd_oom(h)
             if zero or -zeros >= digits:
-                return to_host(x)
+                return x
             x = self.mpd_half(self.mpd_add(self.mpd_add(x, x), self.mpd_mul(x, h)))
             if self.truncate:
                 x = self.mpd_truncate(x, digits + 2)
 
     def mpd_sqrt(self, a: _mpd.MPD, bits: int) -> _mpd.MPD:
         """mpd-sqrt (src/mpd-sqrt.lisp:89-90)."""
-        inv = to_device(self.mpd_inv_sqrt(a, bits))
-        return to_host(self.mpd_mul(to_device(a), inv))
+        return to_host(self.sqrt(a, bits))
+
+    def sqrt(self, a: _mpd.MPD, bits: int) -> DMPD:
+        """mpd-sqrt, the result left in device memory."""
+        return self.mpd_mul(to_device(a), self.inv_sqrt(a, bits))
 
     # -- src/pi.lisp
     def mpd_pi(self, prec_bits: int) -> _mpd.MPD:
         """mpd-pi (src/pi.lisp:53-74): binary splitting by the library's
-        native engine (hgx_series_split: host folds and small products in
-        C++, each tree height's large products in one batched GPU call), then
-        the Newton steps and the final products on the device."""
+        native engine (hgx_series_split), then the Newton steps and the
+        final products on the device.  The partial's limbs go to device
+        memory as they are (partial-denominator B*Q as a device product), and
+        only the result comes back as an integer."""
         from . import series as _series
         guard = prec_bits + _mpd.DIGIT_BITS
         num_terms = 2 + int(guard // _mpd.CHUD_BITS_PER_TERM)
-        comp = _series.binary_split_native(_series.chudnovsky_series(), 0, num_terms)
-        sqrt = to_device(self.mpd_sqrt(_mpd.MPD(10005, 0), guard))
-        recip = to_device(self.mpd_reciprocal(_mpd.MPD(comp.numerator(), 0), guard))
+        comp = _series.binary_split_native_limbs(_series.chudnovsky_series(), 0, num_terms)
+        sqrt = self.sqrt(_mpd.MPD(10005, 0), guard)
+        recip = self.reciprocal(from_limbs(*comp["r"]), guard)
         final = self.mpd_mul(sqrt, recip)
-        den = to_device(_mpd.MPD(comp.denominator(), 0))
+        den = self.mpd_mul(from_limbs(*comp["b"]), from_limbs(*comp["q"]))
         return to_host(self.mpd_mul(final, den))

@@ -202,9 +202,16 @@ def binary_split(series: Series, lower: int, upper: int, mul_batch=None, fold: i
 def binary_split_native(series: Series, lower: int, upper: int) -> Partial:
     """binary-split (src/binary-splitting.lisp:123-133) by the library's
     native engine (hgx_series_split): the same tree evaluated one height at
-    a time in C++, small products on host threads and every product above
-    *ntt-multiply-threshold* limbs of a height in one batched GPU call.  The
+    a time in C++, with each height's products batched on the GPU (or, for
+    small products of a small split, multiplied on host threads).  The
     series' four term functions must be Poly."""
+    v = binary_split_native_limbs(series, lower, upper)
+    return Partial(lower, upper, *(v[k][0] * _int(v[k][1]) for k in "pqbr"))
+
+
+def binary_split_native_limbs(series: Series, lower: int, upper: int) -> dict[str, tuple[int, np.ndarray]]:
+    """binary_split_native's P, Q, B, R as (sign, trimmed little-endian
+    limbs), with no conversion to Python integers."""
     import ctypes
 
     from . import _lib
@@ -223,18 +230,18 @@ def binary_split_native(series: Series, lower: int, upper: int) -> Partial:
     L = _lib.load() if not _lib._initialised else _lib.lib()
     res = ctypes.c_void_p(0)
     _lib.check(L.hgx_series_split(ctypes.byref(desc), lower, upper, ctypes.byref(res)))
+    out = {}
     try:
-        vals = []
-        for w in range(4):
+        for w, k in enumerate("pqbr"):
             sign, n = ctypes.c_int(0), ctypes.c_size_t(0)
             _lib.check(L.hgx_partial_get(res, w, ctypes.byref(sign), ctypes.byref(n), None))
             buf = np.empty(n.value, dtype=np.uint64)
             if n.value:
                 _lib.check(L.hgx_partial_get(res, w, None, None, buf.ctypes.data))
-            vals.append(sign.value * _int(buf))
+            out[k] = (sign.value, buf)
     finally:
         L.hgx_partial_free(res)
-    return Partial(lower, upper, *vals)
+    return out
 
 
 # ------------------------------------------------------------------ series

@@ -351,6 +351,21 @@ def test_mpd_pi_gpu_bit_identical_to_cpu_routing(hgx, bits, truncate):
     assert gpu == cpu
 
 
+@pytest.mark.parametrize("bits,truncate,device_heights", [
+    (1000, False, "0"), (10000, False, "1"), (1 << 16, True, "1"), (1 << 18, True, "0"), (1 << 18, True, "1"),
+])
+def test_mpd_pi_device_chain_equals_host_routing(hgx, monkeypatch, bits, truncate, device_heights):
+    # DeviceContext.mpd_pi: the native split's limbs go to device memory as
+    # they are, B*Q and the Newton steps run there; the same integers as the
+    # host iteration with exact Python products
+    from hypergeometrica_amd import mpd as M
+    from hypergeometrica_amd import mpd_device as D
+    monkeypatch.setenv("HGX_SERIES_DEVICE", device_heights)
+    dev = D.DeviceContext(truncate=truncate).mpd_pi(bits)
+    cpu = M.Context(mul=lambda x, y: x * y, truncate=truncate).mpd_pi(bits)
+    assert dev == cpu
+
+
 @pytest.mark.parametrize("k", [4096, 6000, 1 << 17, 1 << 21])
 def test_product_that_is_all_ones(hgx, k):
     # (2^(32k) - 1)(2^(32k) + 1) = 2^(64k) - 1: every limb of the product is
