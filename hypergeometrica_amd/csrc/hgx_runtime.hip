@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <new>
@@ -91,6 +92,25 @@ u64 redc_host(u128 T, u64 m, u64 nm) {
   const u64 q = (u64)T * nm;
   const u64 r = (u64)((T + (u128)q * m) >> 64);
   return r >= m ? r - m : r;
+}
+
+// fn(i) for i < n, in contiguous ranges on up to 16 host threads (ranges
+// whose thread cannot be created run on the calling thread)
+void host_parallel(size_t n, const std::function<void(size_t)> &fn) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min(16u, hw), n);
+  auto part = [&](size_t t) {
+    for (size_t i = n * t / nt; i < n * (t + 1) / nt; i++) fn(i);
+  };
+  std::vector<std::thread> th;
+  size_t t = 1;
+  try {
+    for (; t < nt; t++) th.emplace_back(part, t);
+  } catch (const std::exception &) {
+  }
+  for (size_t u = t; u < nt; u++) part(u);
+  if (nt) part(0);
+  for (auto &x : th) x.join();
 }
 
 bool is_prime_small(u64 n) {
@@ -619,20 +639,28 @@ int build_plan_variant(int n, const Choice &choice, Plan **out) {
       const u64 ktr = Linv;                   // 1 / L
       Off &o = offs[i * 3 + k];
       o = Off{};
+      const u64 pinv = inv_pow2_64(p), npinv = 0 - pinv;
+      auto mm = [&](u64 a, u64 b) { return redc_host((u128)a * b, p, npinv); };
       // direct table: row q holds SCALE * (root^q)^t for t < T, as (value,
-      // Shoup companion floor(value * 2^64 / p)) pairs
+      // Shoup companion floor(value * 2^64 / p)) pairs.  The rows are filled
+      // on host threads in Montgomery form, x~ = value * 2^64 mod p: value =
+      // REDC(x~), and since value * 2^64 = companion * p + x~ the companion
+      // is the exact quotient -x~ * p^-1 mod 2^64 (no 128-bit division).
       auto push_direct = [&](u64 root, u64 scale) {
-        size_t at = host.size();
-        u64 rq = 1;
-        for (int q = 0; q < 256; q++) {
-          u64 x = scale % p;
+        const size_t at = host.size();
+        host.resize(at + 2 * 256 * T);
+        u64 *tab = host.data() + at;
+        const u64 s0 = to_mont(scale, p);
+        host_parallel(256, [&](size_t q) {
+          const u64 rq = to_mont(powmod(root, q, p), p);
+          u64 *dst = tab + 2 * q * T;
+          u64 x = s0;
           for (size_t t = 0; t < T; t++) {
-            host.push_back(x);
-            host.push_back(shoup_companion(x, p));
-            x = mulmod(x, rq, p);
+            dst[2 * t] = mm(x, 1);
+            dst[2 * t + 1] = (0 - x) * pinv;
+            x = mm(x, rq);
           }
-          rq = mulmod(rq, root, p);
-        }
+        });
         return at;
       };
       // small table: SCALE * root^(tt q) at q*16 + tt, pairs
@@ -651,24 +679,21 @@ int build_plan_variant(int n, const Choice &choice, Plan **out) {
         return at;
       };
       // tile table: root^(16 c q) at c*256 + q, c < T/16, as (value, Shoup
-      // companion) pairs.  The loop runs in Montgomery form: x~ = value * 2^64
-      // mod p, so value * 2^64 = companion * p + x~ and the companion is the
-      // exact quotient -x~ * p^-1 mod 2^64 (no 128-bit division).
+      // companion) pairs, in Montgomery form as the direct table
       auto push_tile = [&](u64 root) {
-        size_t at = host.size();
-        const u64 pinv = inv_pow2_64(p), npinv = 0 - pinv;
-        auto mm = [&](u64 a, u64 b) { return redc_host((u128)a * b, p, npinv); };
-        const u64 r16 = to_mont(powmod(root, 16, p), p);
-        u64 base = to_mont(1, p);
-        for (size_t cidx = 0; cidx < T / 16; cidx++) {
+        const size_t at = host.size(), nc = T / 16;
+        host.resize(at + 2 * 256 * nc);
+        u64 *tab = host.data() + at;
+        host_parallel(nc, [&](size_t cidx) {
+          const u64 base = to_mont(powmod(root, 16 * cidx, p), p);
+          u64 *dst = tab + 2 * 256 * cidx;
           u64 x = to_mont(1, p);
           for (int q = 0; q < 256; q++) {
-            host.push_back(mm(x, 1));
-            host.push_back((0 - x) * pinv);
+            dst[2 * q] = mm(x, 1);
+            dst[2 * q + 1] = (0 - x) * pinv;
             x = mm(x, base);
           }
-          base = mm(base, r16);
-        }
+        });
         return at;
       };
       if (direct) {
