@@ -69,3 +69,18 @@ bool hgx_launch_row_half(bool two, int tb, int sbits, dim3 grid, hipStream_t s, 
                : launch_half<false, 10>(sbits, grid, s, ps, rp, jr, jt_bits);
   return false;
 }
+
+// small products of one length, several per tile (k_row_pack)
+bool hgx_launch_row_pack_mul(int sbits, int jbits, unsigned njobs, dim3 grid, hipStream_t s, const PrimeSet &ps,
+                           const RowParams &rp, const JobRef &jr) {
+  if (jbits < 0 || sbits + jbits > 12) return false;
+  switch (sbits) {
+  case 5: hipLaunchKernelGGL((k_row_pack<5, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;
+  case 6: hipLaunchKernelGGL((k_row_pack<6, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;
+  case 7: hipLaunchKernelGGL((k_row_pack<7, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;
+  case 8: hipLaunchKernelGGL((k_row_pack<8, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;
+  case 9: hipLaunchKernelGGL((k_row_pack<9, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;
+  case 10: hipLaunchKernelGGL((k_row_pack<10, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;
+  }
+  return false;
+}

@@ -619,6 +619,68 @@ __global__ __launch_bounds__(HGX_NT) void k_row(PrimeSet ps, RowParams rp, JobRe
   }
 }
 
+// Whole transforms of 2^S_BITS words (S_BITS <= 10) for a batch of small
+// products of one length: 2^jbits jobs per tile of 2^(S_BITS + jbits) <=
+// 4096 words, one row per job, so that a workgroup's tile is full where
+// k_row<SINGLE> would hold one job's 2^S_BITS words.  One workgroup per tile
+// and prime (grid = (ceil(njobs / 2^jbits), 3)); limbs in, the product's
+// residues out (k_crt finishes).  The same residues as k_row<SINGLE>.
+template <int S_BITS, int MODE>
+__global__ __launch_bounds__(HGX_NT) void k_row_pack(PrimeSet ps, RowParams rp, JobRef jr, int jbits, u32 njobs) {
+  constexpr bool TWO = (MODE == MODE_MUL);
+  constexpr int S = 1 << S_BITS;
+  constexpr int NJ = HGX_TILE >> S_BITS;  // the most jobs of a tile
+  __shared__ u64 ldsA[HGX_TILE_LDS];
+  __shared__ u64 ldsB[TWO ? HGX_TILE_LDS : 1];
+  __shared__ const u64 *sa[NJ];
+  __shared__ const u64 *sb[NJ];
+  __shared__ u64 *sres[NJ];
+  __shared__ u64 sna[NJ], snb[NJ];
+  const int tid = threadIdx.x;
+  const int k = blockIdx.y;
+  const Mod M = mod_of(ps, k);
+  const int nj = 1 << jbits;
+  const int TW = nj << S_BITS;
+  const u32 j0 = blockIdx.x << jbits;
+  for (int r = tid; r < nj; r += HGX_NT) {
+    const u32 j = j0 + r;
+    if (j < njobs) {
+      const MulJob J = load_job(jr, j);
+      sa[r] = J.a;
+      sb[r] = TWO ? J.b : nullptr;
+      sna[r] = J.na;
+      snb[r] = TWO ? J.nb : 0;
+      sres[r] = J.res0;
+    } else {  // past the last job: zero rows, nothing stored
+      sa[r] = nullptr;
+      sb[r] = nullptr;
+      sna[r] = 0;
+      snb[r] = 0;
+      sres[r] = nullptr;
+    }
+  }
+  __syncthreads();
+  for (int x = tid; x < TW; x += HGX_NT) {
+    const int r = x >> S_BITS, pos = x & (S - 1);
+    ldsA[HGX_PAD(x)] = ((u64)pos < sna[r]) ? mod_reduce64(sa[r][pos], M) : 0;
+    if (TWO) ldsB[HGX_PAD(x)] = ((u64)pos < snb[r]) ? mod_reduce64(sb[r][pos], M) : 0;
+  }
+  __syncthreads();
+  tile_forward<S_BITS, false>(ldsA, jbits, ps.tw_fwd[k], M);
+  if (TWO) tile_forward<S_BITS, false>(ldsB, jbits, ps.tw_fwd[k], M);
+  for (int x = tid; x < TW; x += HGX_NT) {
+    const u64 a = ldsA[HGX_PAD(x)];
+    const u64 b = TWO ? ldsB[HGX_PAD(x)] : a;
+    ldsA[HGX_PAD(x)] = mont_mul(mont_mul(a, b, M), rp.scale[k], M);
+  }
+  __syncthreads();
+  tile_inverse<S_BITS, false>(ldsA, jbits, ps.tw_inv[k], M);
+  for (int x = tid; x < TW; x += HGX_NT) {
+    u64 *dst = sres[x >> S_BITS];
+    if (dst) dst[(u64)k * rp.L + (x & (S - 1))] = ldsA[HGX_PAD(x)];
+  }
+}
+
 // Row pass of a product (TWO) or square on tiles of 2^TB words (rows of
 // 2^S_BITS <= 2^TB words): 2048-word tiles with radix-8 rounds or 1024-word
 // tiles with radix-4 rounds, two or four times the workgroups of k_row's

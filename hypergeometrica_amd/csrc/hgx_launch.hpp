@@ -23,6 +23,12 @@ bool hgx_launch_row_half(bool two, int tb, int sbits, dim3 grid, hipStream_t s, 
                          const RowParams &rp, const JobRef &jr, int jt_bits);
 bool hgx_launch_row_sqr(bool single, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
                         const JobRef &jr, int jt_bits);
+// whole small transforms (5 <= SBITS <= 10), 2^JBITS jobs per tile
+// (SBITS + JBITS <= 12), NJOBS jobs in all: products and squares
+bool hgx_launch_row_pack_mul(int sbits, int jbits, unsigned njobs, dim3 grid, hipStream_t s, const PrimeSet &ps,
+                             const RowParams &rp, const JobRef &jr);
+bool hgx_launch_row_pack_sqr(int sbits, int jbits, unsigned njobs, dim3 grid, hipStream_t s, const PrimeSet &ps,
+                             const RowParams &rp, const JobRef &jr);
 bool hgx_launch_row_tr(int mode, int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,
                        const JobRef &jr, int jt_bits);
 bool hgx_launch_row_inv_scaled(int sbits, dim3 grid, hipStream_t s, const PrimeSet &ps, const RowParams &rp,

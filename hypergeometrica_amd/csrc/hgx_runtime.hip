@@ -35,6 +35,8 @@ namespace {
 
 constexpr int kMaxBits = 32;   // largest transform: 2^32 words per residue array
 constexpr int kRowMaxBits = 12;
+// a group of whole-tile products of 2^5 .. 2^10 words packs several per tile
+constexpr int kPackMinBits = 5, kPackMaxBits = 10;
 // a multiply row pass with fewer 4096-word workgroups than this (two per CU
 // of the 256) runs on 2048-word tiles instead, and on 1024-word tiles when
 // the 2048-word ones are still fewer
@@ -938,9 +940,19 @@ int enqueue_group(const JobRef &jr, unsigned njobs, int n, bool sqr, hipStream_t
       const u64 Rm = (u64)((((u128)1) << 64) % p);
       rp.scale[k] = to_mont(mulmod(Rm, invmod(L % p, p), p), p);
     }
-    rc = sqr ? launch_row<MODE_SQR, true>(n, dim3(njobs, 3), s, rp, jr, 0)
-             : launch_row<MODE_MUL, true>(n, dim3(njobs, 3), s, rp, jr, 0);
-    if (rc) return rc;
+    // several small products per tile when the group has them
+    const int jb = (n >= kPackMinBits && n <= kPackMaxBits) ? std::min(kRowMaxBits - n, log2_ceil(njobs)) : 0;
+    if (jb > 0) {
+      HGX_TIMED(sqr ? "row_pack_sqr" : "row_pack_mul", s);
+      const dim3 grid((njobs + (1u << jb) - 1) >> jb, 3);
+      const bool ok = sqr ? hgx_launch_row_pack_sqr(n, jb, njobs, grid, s, g_ctx.ps, rp, jr)
+                          : hgx_launch_row_pack_mul(n, jb, njobs, grid, s, g_ctx.ps, rp, jr);
+      if (!ok) return fail(HGX_EINVAL, "packed row pass size out of range");
+    } else {
+      rc = sqr ? launch_row<MODE_SQR, true>(n, dim3(njobs, 3), s, rp, jr, 0)
+               : launch_row<MODE_MUL, true>(n, dim3(njobs, 3), s, rp, jr, 0);
+      if (rc) return rc;
+    }
   } else {
     const int jt = n - 12;  // tiles per job
     const unsigned grid = njobs << jt;

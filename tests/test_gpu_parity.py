@@ -280,6 +280,32 @@ def test_batch_equals_per_call(hgx, oracle):
         assert oracle.limbs_to_int(o) == oracle.limbs_to_int(a) * oracle.limbs_to_int(b)
 
 
+@pytest.mark.parametrize("n", [5, 6, 7, 8, 9, 10])
+def test_small_products_packed_per_tile(hgx, oracle, n):
+    # batches of one small transform length 2^n (several products per tile,
+    # a partial last tile): random, all-ones and square operands against
+    # exact integers, for every count from one product to several tiles
+    L = 1 << n
+    rng = np.random.default_rng(0x48475806 + n)
+    for count in (1, 2, 3, 31, 33, 4096 // L, 4096 // L + 1, 257):
+        pairs = []
+        for i in range(count):
+            na = int(rng.integers(L // 4 + 1, L // 2 + 1))
+            nb = int(rng.integers(max(1, L // 2 - na + 1), L - na + 1))
+            if i % 5 == 1:
+                a = np.full(na, np.uint64(2**64 - 1), dtype=np.uint64)
+                b = np.full(nb, np.uint64(2**64 - 1), dtype=np.uint64)
+            else:
+                a = oracle.splitmix64(1000 * n + 3 * i + 1, na)
+                b = oracle.splitmix64(1000 * n + 3 * i + 2, nb)
+            if i % 4 == 3 and na > L // 4:
+                b = a  # a square (same operand)
+            pairs.append((a, b))
+        outs = hgx.mul_batch(pairs)
+        for (a, b), o in zip(pairs, outs):
+            assert oracle.limbs_to_int(o) == oracle.limbs_to_int(a) * oracle.limbs_to_int(b), (n, count)
+
+
 def test_mpz_mirror(hgx):
     import random
     rnd = random.Random(4)
