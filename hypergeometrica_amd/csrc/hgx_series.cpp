@@ -324,6 +324,7 @@ int fold(const Series &s, u64 lo, u64 up, AccPart &out) {
   out.B = big_of(bk);
   out.R = big_of(t1);
   Acc BP = out.B;  // B * P of the terms so far
+  Acc x;           // BP p_k a_k (its capacity kept from term to term)
   Limbs tmp;
   big_mul_small(BP, pk, tmp);
   for (u64 k = lo + 1; k < up; k++) {
@@ -335,7 +336,7 @@ int fold(const Series &s, u64 lo, u64 up, AccPart &out) {
     if (!rc) rc = small_mul(pk, ak, t2);
     if (rc) return rc;
     big_mul_small(out.R, t1, tmp);
-    Acc x = BP;
+    x = BP;
     big_mul_small(x, t2, tmp);
     big_add(out.R, x);
     rc = small_mul(bk, pk, t1);

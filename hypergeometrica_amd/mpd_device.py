@@ -79,7 +79,7 @@ def to_device(x: _mpd.MPD) -> DMPD:
         return DMPD(0, None, 0, 0, x.exponent)
     m = abs(x.mantissa)
     n = (m.bit_length() + 63) // 64
-    host = np.frombuffer(m.to_bytes(8 * n, "little"), dtype="<u8").astype(np.uint64)
+    host = np.frombuffer(m.to_bytes(8 * n, "little"), dtype="<u8")  # read-only view, uploaded before it goes
     buf = DBuf(n)
     check(_L().hgx_upload(buf.ptr, host.ctypes.data, n, None))
     return DMPD(1 if x.mantissa > 0 else -1, buf, 0, n, x.exponent)
@@ -101,7 +101,7 @@ def to_host(x: DMPD) -> _mpd.MPD:
         return _mpd.MPD(0, x.exponent)
     host = np.empty(x.size, dtype=np.uint64)
     check(_L().hgx_download(host.ctypes.data, x.ptr, x.size, None))
-    return _mpd.MPD(x.sign * int.from_bytes(host.astype("<u8").tobytes(), "little"), x.exponent)
+    return _mpd.MPD(x.sign * int.from_bytes(np.ascontiguousarray(host, dtype="<u8").tobytes(), "little"), x.exponent)
 
 
 def _mag_cmp(a: DMPD, b: DMPD) -> int:

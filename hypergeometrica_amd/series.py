@@ -80,7 +80,7 @@ def _limbs(x: int) -> np.ndarray:
 
 
 def _int(limbs: np.ndarray) -> int:
-    return int.from_bytes(np.ascontiguousarray(limbs).astype("<u8").tobytes(), "little")
+    return int.from_bytes(np.ascontiguousarray(limbs, dtype="<u8").tobytes(), "little")
 
 
 def mul_many(pairs: list[tuple[int, int]], threshold: int | None = None) -> list[int]:
