@@ -100,3 +100,30 @@ def test_default_move_to_device_equals_host_heights(hgx, monkeypatch):
     monkeypatch.setenv("HGX_SERIES_DEVICE", "0")
     host = _tuple(S.binary_split_native(s, 0, 400000))
     assert dev == host
+
+
+def test_concurrent_splits_from_threads(hgx, monkeypatch):
+    # four threads split at once (device heights forced on, so their rounds,
+    # sums and regions interleave on the device); every partial equals the
+    # oracle's
+    import threading
+    monkeypatch.setenv("HGX_SERIES_DEVICE", "1")
+    jobs = [("chudnovsky", 0, 3000), ("apery", 0, 2000), ("exp(-5/3)", 17, 4017), ("catalan", 0, 4000)]
+    got, errors = {}, []
+
+    def worker(i):
+        try:
+            name, lo, up = jobs[i]
+            for _ in range(3):
+                got[i] = _tuple(S.binary_split_native(SERIES[name](), lo, up))
+        except Exception as e:  # reported below with the job
+            errors.append((jobs[i], repr(e)))
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(len(jobs))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for i, (name, lo, up) in enumerate(jobs):
+        assert got[i] == SO.binary_split(SERIES[name](), lo, up), jobs[i]
