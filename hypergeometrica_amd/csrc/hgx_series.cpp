@@ -81,9 +81,10 @@ constexpr size_t kLeafChunk = 512;  // leaves folded by one task
 // a height moves to the GPU when its round-1 products have at least this
 // many limbs, and the device has room for kDeviceRoomFactor times the live
 // values (a height's products take about twice as many limbs as its
-// children, and the children's regions live until the height is done)
+// children, the children's regions live until the height is done, and the
+// largest product's transforms take about four times the live values)
 constexpr size_t kDeviceMinLimbs = (size_t)1 << 20;
-constexpr size_t kDeviceRoomFactor = 6;
+constexpr size_t kDeviceRoomFactor = 10;
 constexpr size_t kHostMaxLimbs = 100;  // *ntt-multiply-threshold*, src/multiply.lisp:9
 constexpr int kMaxDeg = 7;
 constexpr int kSmallLimbs = 16;  // term values and products of two of them
@@ -949,22 +950,29 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     if (rc) return rc;
     // R = B_r Q_r R_l + B_l P_l R_r; the node's partial replaces its left
     // child's slot (same lower end), the right child's slot is emptied
-    if (on_dev) {
-      rc = sums_dev(o2, cnt);
-      if (rc) return rc;
-    }
-    rc = parallel_for(cnt, [&](size_t i) {
-      const int r = on_dev ? (int)HGX_OK : num_add(o2[2 * i], o2[2 * i + 1]);
-      if (r) return r;
+    auto finish = [&](size_t i) {
       Part &l = *L[i], &rt = *R[i];
       l.P = std::move(o1[5 * i]);
       l.Q = std::move(o1[5 * i + 1]);
       l.B = std::move(o1[5 * i + 2]);
       l.R = std::move(o2[2 * i]);
       rt = Part();
-      return (int)HGX_OK;
-    });
-    if (rc) return rc;
+    };
+    if (on_dev) {
+      rc = sums_dev(o2, cnt);
+      if (rc) return rc;
+      // one thread: the values of a round share their region's reference
+      // count, which threads would contend for
+      for (size_t i = 0; i < cnt; i++) finish(i);
+    } else {
+      rc = parallel_for(cnt, [&](size_t i) {
+        const int r = num_add(o2[2 * i], o2[2 * i + 1]);
+        if (r) return r;
+        finish(i);
+        return (int)HGX_OK;
+      });
+      if (rc) return rc;
+    }
     // drop the right children's slots
     std::vector<u64> gone;
     gone.reserve(cnt);
