@@ -498,6 +498,7 @@ struct Num {
 // the work queued before the release
 struct DevRegion {
   u64 *base = nullptr;
+  size_t cap = 0;  // limbs
   ~DevRegion() {
     if (base) (void)hgx_device_free(base, nullptr);
   }
@@ -505,7 +506,8 @@ struct DevRegion {
 
 int new_dev_region(size_t limbs, std::shared_ptr<DevRegion> &out) {
   out = std::make_shared<DevRegion>();
-  return hgx_device_alloc(std::max<size_t>(limbs, 1), &out->base);
+  out->cap = std::max<size_t>(limbs, 1);
+  return hgx_device_alloc(out->cap, &out->base);
 }
 
 struct Part {
@@ -790,7 +792,37 @@ bool move_to_device(std::vector<std::pair<u64, Part>> &live, const std::vector<c
   for (const auto &e : live) held += e.second.P.n + e.second.Q.n + e.second.B.n + e.second.R.n;
   if (held > hgx_internal_device_room() / (8 * kDeviceRoomFactor)) return false;
   rc = to_device(live);
+  if (rc == HGX_ENOMEM) {  // no device memory for the move: the split stays on the host
+    rc = HGX_OK;
+    return false;
+  }
   return rc == HGX_OK;
+}
+
+// Moves every live value back to host memory (one download per device
+// region that holds live values), trimmed, for a split whose device memory
+// ran out.
+int to_host(std::vector<std::pair<u64, Part>> &live, const std::shared_ptr<Cache> &cache) {
+  std::vector<std::pair<DevRegion *, RegionRef>> moved;
+  for (auto &e : live)
+    for (Num *v : {&e.second.P, &e.second.Q, &e.second.B, &e.second.R}) {
+      if (!v->dev) continue;
+      DevRegion *d = static_cast<DevRegion *>(v->own.get());
+      RegionRef h;
+      for (auto &m : moved)
+        if (m.first == d) h = m.second;
+      if (!h) {
+        h = new_region(cache, d->cap);
+        const int rc = hgx_download(h->base, d->base, d->cap, nullptr);
+        if (rc) return rc;
+        moved.emplace_back(d, h);
+      }
+      v->p = h->base + (v->p - d->base);
+      v->dev = false;
+      v->own = h;
+      if (v->s) num_trim(*v);
+    }
+  return HGX_OK;
 }
 
 // The split tree down to subtrees of at most kFoldTerms terms (BLOCKS);
@@ -899,80 +931,105 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     return it->second;
   };
   auto prod_limbs = [](const Num &x, const Num &y) -> size_t { return x.s && y.s ? x.n + y.n : 0; };
-  bool on_dev = false;
+  bool on_dev = false, dev_off = false;
+  // test hook: device rounds at this height report HGX_ENOMEM
+  const char *fail_env = std::getenv("HGX_SERIES_FAIL_HEIGHT");
+  const long fail_height = fail_env ? std::atol(fail_env) : -1;
   std::stable_sort(nodes.begin(), nodes.end(), [](const auto &x, const auto &y) { return x[0] < y[0]; });
   for (size_t h0 = 0; h0 < nodes.size();) {
     size_t h1 = h0;
     while (h1 < nodes.size() && nodes[h1][0] == nodes[h0][0]) h1++;
     const size_t cnt = h1 - h0;
-    // round 1: P_l P_r, Q_l Q_r, B_l B_r, Q_r R_l, P_l R_r (combine, :112-121)
+    std::vector<Part *> L(cnt), R(cnt);
     std::vector<const Num *> xs, ys;
     std::vector<size_t> room;
-    xs.reserve(5 * cnt);
-    ys.reserve(5 * cnt);
-    room.reserve(5 * cnt);
-    std::vector<Part *> L(cnt), R(cnt);
-    for (size_t i = 0; i < cnt; i++) {
-      L[i] = &find(nodes[h0 + i][1]);
-      R[i] = &find(nodes[h0 + i][2]);
-      const Part &l = *L[i], &r = *R[i];
-      for (auto pr : {std::make_pair(&l.P, &r.P), std::make_pair(&l.Q, &r.Q), std::make_pair(&l.B, &r.B),
-                      std::make_pair(&r.Q, &l.R), std::make_pair(&l.P, &r.R)}) {
-        xs.push_back(pr.first);
-        ys.push_back(pr.second);
-        room.push_back(prod_limbs(*pr.first, *pr.second));
+    // round 1: P_l P_r, Q_l Q_r, B_l B_r, Q_r R_l, P_l R_r (combine, :112-121)
+    auto round1 = [&]() {
+      xs.clear();
+      ys.clear();
+      room.clear();
+      xs.reserve(5 * cnt);
+      ys.reserve(5 * cnt);
+      room.reserve(5 * cnt);
+      for (size_t i = 0; i < cnt; i++) {
+        L[i] = &find(nodes[h0 + i][1]);
+        R[i] = &find(nodes[h0 + i][2]);
+        const Part &l = *L[i], &r = *R[i];
+        for (auto pr : {std::make_pair(&l.P, &r.P), std::make_pair(&l.Q, &r.Q), std::make_pair(&l.B, &r.B),
+                        std::make_pair(&r.Q, &l.R), std::make_pair(&l.P, &r.R)}) {
+          xs.push_back(pr.first);
+          ys.push_back(pr.second);
+          room.push_back(prod_limbs(*pr.first, *pr.second));
+        }
       }
-    }
-    if (!on_dev) {
+    };
+    // the height's products and sums; the nodes' partials replace the
+    // children only once all of them succeed
+    auto height = [&](bool dev) -> int {
+      std::vector<Num> o1;
+      int r = dev ? mul_many_dev(xs, ys, room, o1) : mul_many(cache, xs, ys, room, o1);
+      if (r) return r;
+      if (dev && (long)nodes[h0][0] == fail_height)
+        return hgx_internal_fail(HGX_ENOMEM, "binary splitting: device memory exhausted (test hook)");
+      // round 2: B_r (Q_r R_l), B_l (P_l R_r); the first slot of each node
+      // also holds the sum R
+      std::vector<const Num *> xs2, ys2;
+      std::vector<size_t> room2;
+      for (size_t i = 0; i < cnt; i++) {
+        const Num &br = R[i]->B, &bl = L[i]->B;
+        xs2.push_back(&br);
+        ys2.push_back(&o1[5 * i + 3]);
+        xs2.push_back(&bl);
+        ys2.push_back(&o1[5 * i + 4]);
+        const size_t a = prod_limbs(br, o1[5 * i + 3]), b = prod_limbs(bl, o1[5 * i + 4]);
+        room2.push_back(std::max(a, b) + 1);
+        room2.push_back(b);
+      }
+      std::vector<Num> o2;
+      r = dev ? mul_many_dev(xs2, ys2, room2, o2) : mul_many(cache, xs2, ys2, room2, o2);
+      if (r) return r;
+      // R = B_r Q_r R_l + B_l P_l R_r; the node's partial replaces its left
+      // child's slot (same lower end), the right child's slot is emptied
+      auto finish = [&](size_t i) {
+        Part &l = *L[i], &rt = *R[i];
+        l.P = std::move(o1[5 * i]);
+        l.Q = std::move(o1[5 * i + 1]);
+        l.B = std::move(o1[5 * i + 2]);
+        l.R = std::move(o2[2 * i]);
+        rt = Part();
+      };
+      if (dev) {
+        r = sums_dev(o2, cnt);
+        if (r) return r;
+        // one thread: the values of a round share their region's reference
+        // count, which threads would contend for
+        for (size_t i = 0; i < cnt; i++) finish(i);
+        return HGX_OK;
+      }
+      // host sums on host threads: the sums first, the partials replace the
+      // children once every sum has succeeded
+      r = parallel_for(cnt, [&](size_t i) { return num_add(o2[2 * i], o2[2 * i + 1]); });
+      if (r) return r;
+      for (size_t i = 0; i < cnt; i++) finish(i);
+      return HGX_OK;
+    };
+    round1();
+    if (!on_dev && !dev_off) {
       on_dev = move_to_device(live, xs, ys, rc);
       if (rc) return rc;
     }
-    std::vector<Num> o1;
-    rc = on_dev ? mul_many_dev(xs, ys, room, o1) : mul_many(cache, xs, ys, room, o1);
-    if (rc) return rc;
-    // round 2: B_r (Q_r R_l), B_l (P_l R_r); the first slot of each node
-    // also holds the sum R
-    xs.clear();
-    ys.clear();
-    room.clear();
-    for (size_t i = 0; i < cnt; i++) {
-      const Num &br = R[i]->B, &bl = L[i]->B;
-      xs.push_back(&br);
-      ys.push_back(&o1[5 * i + 3]);
-      xs.push_back(&bl);
-      ys.push_back(&o1[5 * i + 4]);
-      const size_t a = prod_limbs(br, o1[5 * i + 3]), b = prod_limbs(bl, o1[5 * i + 4]);
-      room.push_back(std::max(a, b) + 1);
-      room.push_back(b);
-    }
-    std::vector<Num> o2;
-    rc = on_dev ? mul_many_dev(xs, ys, room, o2) : mul_many(cache, xs, ys, room, o2);
-    if (rc) return rc;
-    // R = B_r Q_r R_l + B_l P_l R_r; the node's partial replaces its left
-    // child's slot (same lower end), the right child's slot is emptied
-    auto finish = [&](size_t i) {
-      Part &l = *L[i], &rt = *R[i];
-      l.P = std::move(o1[5 * i]);
-      l.Q = std::move(o1[5 * i + 1]);
-      l.B = std::move(o1[5 * i + 2]);
-      l.R = std::move(o2[2 * i]);
-      rt = Part();
-    };
-    if (on_dev) {
-      rc = sums_dev(o2, cnt);
+    rc = height(on_dev);
+    if (on_dev && rc == HGX_ENOMEM) {
+      // device memory ran out: the children are intact; every live value
+      // goes back to host memory and the split goes on there
+      rc = to_host(live, cache);
       if (rc) return rc;
-      // one thread: the values of a round share their region's reference
-      // count, which threads would contend for
-      for (size_t i = 0; i < cnt; i++) finish(i);
-    } else {
-      rc = parallel_for(cnt, [&](size_t i) {
-        const int r = num_add(o2[2 * i], o2[2 * i + 1]);
-        if (r) return r;
-        finish(i);
-        return (int)HGX_OK;
-      });
-      if (rc) return rc;
+      on_dev = false;
+      dev_off = true;
+      round1();  // the trimmed host values
+      rc = height(false);
     }
+    if (rc) return rc;
     // drop the right children's slots
     std::vector<u64> gone;
     gone.reserve(cnt);

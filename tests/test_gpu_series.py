@@ -127,3 +127,15 @@ def test_concurrent_splits_from_threads(hgx, monkeypatch):
     assert not errors, errors
     for i, (name, lo, up) in enumerate(jobs):
         assert got[i] == SO.binary_split(SERIES[name](), lo, up), jobs[i]
+
+
+@pytest.mark.parametrize("fail_height", [1, 2, 5])
+@pytest.mark.parametrize("name,lo,up", [("chudnovsky", 0, 3000), ("exp(-5/3)", 17, 4017)])
+def test_device_memory_exhausted_goes_on_on_the_host(hgx, monkeypatch, fail_height, name, lo, up):
+    # the device rounds of one height report HGX_ENOMEM (as when device
+    # memory runs out): every live value returns to host memory and the split
+    # goes on there, with the same partial
+    monkeypatch.setenv("HGX_SERIES_DEVICE", "1")
+    monkeypatch.setenv("HGX_SERIES_FAIL_HEIGHT", str(fail_height))
+    s = SERIES[name]()
+    assert _tuple(S.binary_split_native(s, lo, up)) == SO.binary_split(s, lo, up)
