@@ -745,7 +745,8 @@ int to_device(std::vector<std::pair<u64, Part>> &live) {
   if (hgx_host_alloc(8 * std::max<size_t>(total, 1), &pinned) == HGX_OK) {
     pack = static_cast<u64 *>(pinned);
   } else {
-    pageable.reset(new u64[std::max<size_t>(total, 1)]);
+    pageable.reset(new (std::nothrow) u64[std::max<size_t>(total, 1)]);
+    if (!pageable) return hgx_internal_fail(HGX_ENOMEM, "out of host memory for the upload of a split");
     pack = pageable.get();
   }
   const size_t per = 4096;  // values packed by one task
