@@ -27,7 +27,7 @@
 ;;;;                    stay in HBM between products and sums
 ;;;;   HGX-BINARY-SPLIT BINARY-SPLIT of a series with polynomial terms by the
 ;;;;                    library's native engine (hgx_series_split): the same
-;;;;                    PARTIAL, each tree height's large products batched
+;;;;                    PARTIAL, each tree height's products batched
 ;;;;
 ;;;; Limbs are the reference's storage: little-endian 64-bit digits in C
 ;;;; memory (src/vec.lisp:17-65, src/ram-vec.lisp:31-97); VEC-DIGIT-POINTER
@@ -529,9 +529,11 @@ dropped (MPD-TRUNCATE!'s mantissa step, src/mpd.lisp:56-70)."
 ;;; closures.  HGX-BINARY-SPLIT takes the same four term functions as
 ;;; polynomials, the form every series of the reference has, and returns
 ;;; the PARTIAL that BINARY-SPLIT returns for them: the library builds the
-;;; same tree, folds its small subtrees and multiplies small operands on
-;;; host threads, and sends each tree height's products above
-;;; *NTT-MULTIPLY-THRESHOLD* to the GPU in one batched call.
+;;; same tree and folds its small subtrees on host threads.  A large split
+;;; then keeps its values in device memory, each round of a tree height one
+;;; batched device call and the sums on the device; a small one sends each
+;;; height's products above *NTT-MULTIPLY-THRESHOLD* to the GPU in one
+;;; batched call and multiplies the others on host threads.
 
 (defun %fill-term (desc index term)
   "TERM is a list of integer coefficients, constant term first, optionally
