@@ -422,8 +422,10 @@ def pi_leg(bits: int = 1 << 28, identical_bits: int = 1 << 22, mpfr_bits: int = 
     res = {"workload": f"mpd-pi to 2^{bits.bit_length() - 1} bits, truncating Newton, GPU products "
                        f"(binary-splitting levels batched, Newton iterates in HBM)"}
     t0 = time.perf_counter()
-    r = D.DeviceContext(truncate=True).mpd_pi(bits)
+    ctx = D.DeviceContext(truncate=True)
+    r = ctx.mpd_pi(bits)
     res["gpu_newton_in_hbm_s"] = round(time.perf_counter() - t0, 3)
+    res["phases_s"] = ctx.phase_s
     if host_iterates:
         t0 = time.perf_counter()
         g = M.Context(truncate=True)

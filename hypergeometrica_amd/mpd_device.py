@@ -118,6 +118,7 @@ class DeviceContext:
         self.truncate = truncate
         self.products = 0
         self.max_product_limbs = 0
+        self.phase_s: dict[str, float] = {}  # mpd_pi's steps
 
     # -- src/mpd.lisp
     def mpd_mul(self, a: DMPD, b: DMPD) -> DMPD:
@@ -262,12 +263,23 @@ class DeviceContext:
         final products on the device.  The partial's limbs go to device
         memory as they are (partial-denominator B*Q as a device product), and
         only the result comes back as an integer."""
+        import time
+
         from . import series as _series
         guard = prec_bits + _mpd.DIGIT_BITS
         num_terms = 2 + int(guard // _mpd.CHUD_BITS_PER_TERM)
+        t = [time.perf_counter()]
         comp = _series.binary_split_native_limbs(_series.chudnovsky_series(), 0, num_terms)
+        t.append(time.perf_counter())
         sqrt = self.sqrt(_mpd.MPD(10005, 0), guard)
+        t.append(time.perf_counter())
         recip = self.reciprocal(from_limbs(*comp["r"]), guard)
+        t.append(time.perf_counter())
         final = self.mpd_mul(sqrt, recip)
         den = self.mpd_mul(from_limbs(*comp["b"]), from_limbs(*comp["q"]))
-        return to_host(self.mpd_mul(final, den))
+        out = to_host(self.mpd_mul(final, den))
+        t.append(time.perf_counter())
+        # seconds of each step (each ends with a read-back, so its device work is done)
+        self.phase_s = dict(zip(("binary_split", "sqrt_10005", "reciprocal", "products_and_result"),
+                                (round(b - a, 3) for a, b in zip(t, t[1:]))))
+        return out
