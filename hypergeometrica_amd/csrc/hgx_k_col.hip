@@ -550,6 +550,28 @@ __global__ __launch_bounds__(256) void k_direct_mont(u64 *DM, const u64 *TS, con
   }
 }
 
+// Twiddle tables on the device: entry (i, j) of a ROWS x COLS table, row
+// major as (value, Shoup companion) pairs, is SCALE * R^(i j) mod p
+// (pow_table_entry; RM, SM and ONE in Montgomery form, PINV = p^-1 mod 2^64).
+__global__ __launch_bounds__(256) void k_pow_table(u64 *tab, u32 rows, u64 cols, u64 rm, u64 sm, u64 one, u64 pinv,
+                                                   Mod M) {
+  const u64 total = (u64)rows * cols;
+  for (u64 id = (u64)blockIdx.x * 256 + threadIdx.x; id < total; id += (u64)gridDim.x * 256) {
+    const u64 i = id / cols, j = id - i * cols;
+    u64 v, q;
+    pow_table_entry(i * j, rm, sm, one, pinv, M, v, q);
+    tab[2 * id] = v;
+    tab[2 * id + 1] = q;
+  }
+}
+
+void hgx_launch_pow_table(hipStream_t s, u64 *tab, u32 rows, u64 cols, u64 rm, u64 sm, u64 one, u64 pinv,
+                          const Mod &M) {
+  const u64 total = (u64)rows * cols;
+  const u64 blocks = (total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536;
+  hipLaunchKernelGGL(k_pow_table, dim3((unsigned)blocks), dim3(256), 0, s, tab, rows, cols, rm, sm, one, pinv, M);
+}
+
 void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M, u64 rm,
                             u64 rmq) {
   hipLaunchKernelGGL(k_direct_mont, dim3(4096), dim3(256), 0, s, DM, TS, TT, tbits, M, rm, rmq);

@@ -62,5 +62,9 @@ void hgx_launch_sums(hipStream_t s, const HgxSumJob *jobs, u64 njobs, u64 nblk, 
 bool hgx_launch_probe(int op, dim3 grid, hipStream_t s, const Mod &M, u64 w, u64 wq, u64 wm, int iters, u64 *out,
                       u64 *clk);
 int hgx_probe_chains();
+// a ROWS x COLS table of (SCALE R^(i j), companion) pairs; RM, SM, ONE in
+// Montgomery form, PINV = p^-1 mod 2^64
+void hgx_launch_pow_table(hipStream_t s, u64 *tab, u32 rows, u64 cols, u64 rm, u64 sm, u64 one, u64 pinv,
+                          const Mod &M);
 void hgx_launch_direct_mont(hipStream_t s, u64 *DM, const u64 *TS, const u64 *TT, int tbits, const Mod &M, u64 rm,
                             u64 rmq);

@@ -15,7 +15,6 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
-#include <functional>
 #include <map>
 #include <memory>
 #include <new>
@@ -88,31 +87,6 @@ u64 inv_pow2_64(u64 m) {
   u64 x = m;
   for (int i = 0; i < 6; i++) x *= 2 - m * x;
   return x;
-}
-// T * 2^-64 mod m for T < m * 2^64 (REDC; nm = -m^-1 mod 2^64)
-u64 redc_host(u128 T, u64 m, u64 nm) {
-  const u64 q = (u64)T * nm;
-  const u64 r = (u64)((T + (u128)q * m) >> 64);
-  return r >= m ? r - m : r;
-}
-
-// fn(i) for i < n, in contiguous ranges on up to 16 host threads (ranges
-// whose thread cannot be created run on the calling thread)
-void host_parallel(size_t n, const std::function<void(size_t)> &fn) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>(std::min(16u, hw), n);
-  auto part = [&](size_t t) {
-    for (size_t i = n * t / nt; i < n * (t + 1) / nt; i++) fn(i);
-  };
-  std::vector<std::thread> th;
-  size_t t = 1;
-  try {
-    for (; t < nt; t++) th.emplace_back(part, t);
-  } catch (const std::exception &) {
-  }
-  for (size_t u = t; u < nt; u++) part(u);
-  if (nt) part(0);
-  for (auto &x : th) x.join();
 }
 
 bool is_prime_small(u64 n) {
@@ -621,7 +595,16 @@ int build_plan_variant(int n, const Choice &choice, Plan **out) {
   plan->ncol = ncol;
   plan->srow = n - 8 * ncol;
   const u64 L = 1ull << n;
-  std::vector<u64> host;
+  // the tables are filled on the device (k_pow_table): here each gets its
+  // offset in the plan's block, in words
+  struct Fill {
+    size_t at;
+    u32 rows;
+    u64 cols, root, scale;
+    int k;
+  };
+  std::vector<Fill> fills;
+  size_t words = 0;
   struct Off { size_t Df, Di, Dm, Dt, Sf, Si, Sm, St, Tf, Ti; };
   std::vector<Off> offs(plan->ncol * 3);
   for (int i = 0; i < plan->ncol; i++) {
@@ -632,7 +615,6 @@ int build_plan_variant(int n, const Choice &choice, Plan **out) {
     plan->col[i].direct = direct;
     for (int k = 0; k < 3; k++) {
       const u64 p = kModuli[k];
-      const Mod M{p, 0 - p, (u32)((p - 1) >> 55)};
       const u64 w = powmod(g_ctx.gen[k], (p - 1) >> b, p);
       const u64 wi = invmod(w, p);
       const u64 Linv = invmod(L % p, p);
@@ -641,63 +623,19 @@ int build_plan_variant(int n, const Choice &choice, Plan **out) {
       const u64 ktr = Linv;                   // 1 / L
       Off &o = offs[i * 3 + k];
       o = Off{};
-      const u64 pinv = inv_pow2_64(p), npinv = 0 - pinv;
-      auto mm = [&](u64 a, u64 b) { return redc_host((u128)a * b, p, npinv); };
-      // direct table: row q holds SCALE * (root^q)^t for t < T, as (value,
-      // Shoup companion floor(value * 2^64 / p)) pairs.  The rows are filled
-      // on host threads in Montgomery form, x~ = value * 2^64 mod p: value =
-      // REDC(x~), and since value * 2^64 = companion * p + x~ the companion
-      // is the exact quotient -x~ * p^-1 mod 2^64 (no 128-bit division).
-      auto push_direct = [&](u64 root, u64 scale) {
-        const size_t at = host.size();
-        host.resize(at + 2 * 256 * T);
-        u64 *tab = host.data() + at;
-        const u64 s0 = to_mont(scale, p);
-        host_parallel(256, [&](size_t q) {
-          const u64 rq = to_mont(powmod(root, q, p), p);
-          u64 *dst = tab + 2 * q * T;
-          u64 x = s0;
-          for (size_t t = 0; t < T; t++) {
-            dst[2 * t] = mm(x, 1);
-            dst[2 * t + 1] = (0 - x) * pinv;
-            x = mm(x, rq);
-          }
-        });
+      // ROWS x COLS pairs (SCALE * R^(i j), Shoup companion floor(value * 2^64 / p))
+      auto table = [&](u32 rows, u64 cols, u64 r, u64 scale) {
+        const size_t at = words;
+        fills.push_back(Fill{at, rows, cols, r, scale, k});
+        words += 2 * (size_t)rows * cols;
         return at;
       };
-      // small table: SCALE * root^(tt q) at q*16 + tt, pairs
-      auto push_small = [&](u64 root, u64 scale) {
-        size_t at = host.size();
-        u64 rq = 1;
-        for (int q = 0; q < 256; q++) {
-          u64 x = scale % p;
-          for (int tt = 0; tt < 16; tt++) {
-            host.push_back(x);
-            host.push_back(shoup_companion(x, p));
-            x = mulmod(x, rq, p);
-          }
-          rq = mulmod(rq, root, p);
-        }
-        return at;
-      };
-      // tile table: root^(16 c q) at c*256 + q, c < T/16, as (value, Shoup
-      // companion) pairs, in Montgomery form as the direct table
-      auto push_tile = [&](u64 root) {
-        const size_t at = host.size(), nc = T / 16;
-        host.resize(at + 2 * 256 * nc);
-        u64 *tab = host.data() + at;
-        host_parallel(nc, [&](size_t cidx) {
-          const u64 base = to_mont(powmod(root, 16 * cidx, p), p);
-          u64 *dst = tab + 2 * 256 * cidx;
-          u64 x = to_mont(1, p);
-          for (int q = 0; q < 256; q++) {
-            dst[2 * q] = mm(x, 1);
-            dst[2 * q + 1] = (0 - x) * pinv;
-            x = mm(x, base);
-          }
-        });
-        return at;
-      };
+      // direct table: row q holds SCALE * (root^q)^t for t < T
+      auto push_direct = [&](u64 root, u64 scale) { return table(256, T, root, scale); };
+      // small table: SCALE * root^(tt q) at q*16 + tt
+      auto push_small = [&](u64 root, u64 scale) { return table(256, 16, root, scale); };
+      // tile table: root^(16 c q) at c*256 + q, c < T/16
+      auto push_tile = [&](u64 root) { return table((u32)(T / 16), 256, powmod(root, 16, p), 1); };
       if (direct) {
         o.Df = push_direct(w, 1);
         if (i == 0) {
@@ -719,11 +657,22 @@ int build_plan_variant(int n, const Choice &choice, Plan **out) {
       }
     }
   }
-  if (!host.empty()) {
-    plan->bytes = host.size() * sizeof(u64);
+  if (words) {
+    plan->bytes = words * sizeof(u64);
     int rc = dev_alloc((void **)&plan->dev, plan->bytes);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(plan->dev, host.data(), plan->bytes, hipMemcpyHostToDevice));
+    for (const Fill &f : fills) {
+      const u64 p = kModuli[f.k];
+      const Mod M{p, 0 - p, (u32)((p - 1) >> 55)};
+      hgx_launch_pow_table(g_ctx.stream, plan->dev + f.at, f.rows, f.cols, to_mont(f.root, p), to_mont(f.scale, p),
+                           to_mont(1, p), inv_pow2_64(p), M);
+    }
+    const hipError_t e = hipGetLastError();
+    const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(g_ctx.stream) : e;
+    if (e2 != hipSuccess) {
+      dev_free(plan->dev, plan->bytes);
+      return fail(HGX_EDEVICE, std::string("root tables: ") + hipGetErrorString(e2));
+    }
     for (int i = 0; i < plan->ncol; i++)
       for (int k = 0; k < 3; k++) {
         const Off &o = offs[i * 3 + k];

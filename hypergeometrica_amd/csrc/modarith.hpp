@@ -122,6 +122,30 @@ HGX_HD u64 mont_mul(u64 a, u64 b, const Mod &M) {
   return csub(t, M);
 }
 
+// base^e in Montgomery form (BASE and ONE = 2^64 mod p in Montgomery form,
+// canonical), by squaring and multiplying
+HGX_HD u64 mont_pow(u64 base, u64 e, u64 one, const Mod &M) {
+  u64 r = one;
+  while (e) {
+    if (e & 1) r = mont_mul(r, base, M);
+    base = mont_mul(base, base, M);
+    e >>= 1;
+  }
+  return r;
+}
+
+// Entry E of a twiddle table: the value SCALE * R^E mod p and its Shoup
+// companion floor(value * 2^64 / p), from RM, SM and ONE (R, SCALE and 1 in
+// Montgomery form, x 2^64 mod p) and PINV = p^-1 mod 2^64.  With x~ the
+// Montgomery form of the value, value = REDC(x~), and since value * 2^64 =
+// companion * p + x~, the companion is the exact quotient -x~ p^-1 mod 2^64:
+// no division.
+HGX_HD void pow_table_entry(u64 e, u64 rm, u64 sm, u64 one, u64 pinv, const Mod &M, u64 &value, u64 &companion) {
+  const u64 x = mont_mul(sm, mont_pow(rm, e, one, M), M);
+  value = mont_mul(x, 1, M);
+  companion = (0 - x) * pinv;
+}
+
 // a*w mod p, canonical, by Shoup's method with the precomputed companion
 // wq = floor(w * 2^64 / p) (calc-b-inv and m*/fast2,
 // src/modular-arithmetic.lisp:206-231): q = hi64(a*wq) is exact, so

@@ -55,6 +55,10 @@ void v_reduce(const u64 *a, u64 *out, size_t n, u64 p) {
 void v_mul128(const u64 *a, const u64 *b, u64 *lo, u64 *hi, size_t n) {
   for (size_t i = 0; i < n; i++) mul64x64(a[i], b[i], lo[i], hi[i]);
 }
+void v_table(const u64 *e, u64 *value, u64 *comp, size_t n, u64 rm, u64 sm, u64 one, u64 pinv, u64 p) {
+  const Mod M = mk(p);
+  for (size_t i = 0; i < n; i++) pow_table_entry(e[i], rm, sm, one, pinv, M, value[i], comp[i]);
+}
 }
 """
 
@@ -79,6 +83,7 @@ def ma(tmp_path_factory):
     for f in ("v_csub", "v_reduce"):
         getattr(L, f).argtypes = [P, P, ctypes.c_size_t, u]
     L.v_mul128.argtypes = [P, P, P, P, ctypes.c_size_t]
+    L.v_table.argtypes = [P, P, P, ctypes.c_size_t, u, u, u, u, u]
     return L
 
 
@@ -177,3 +182,26 @@ def test_planted_mistake_in_shoup_companion_is_caught(ma):
     out = np.zeros(len(a), dtype=np.uint64)
     call(ma.v_shoup, a, w, wq, out, len(a), p)
     assert out.tolist() != [(x * y) % p for x, y in zip(a, w)]
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_twiddle_table_entries(ma, k):
+    # the root tables' entries as the device fills them (k_pow_table):
+    # SCALE * R^E mod p and its Shoup companion floor(value 2^64 / p), for
+    # exponents up to 2^27 (the largest i*j of a table) and the scales the
+    # plans use (1, the inverse of L, R/L)
+    p = PRIMES[k]
+    rnd = random.Random(800 + k)
+    g = [3, 5, 3][k]
+    mont = lambda x: (x << 64) % p
+    pinv = pow(p, -1, 1 << 64)
+    for bits, scale in [(27, 1), (20, pow(1 << 20, -1, p)), (13, ((1 << 64) * pow(1 << 13, -1, p)) % p),
+                        (8, rnd.randrange(1, p))]:
+        r = pow(g, (p - 1) >> bits, p)
+        e = [0, 1, 2, (1 << bits) - 1, 1 << bits, (1 << 27) - 1] + [rnd.randrange(1 << 27) for _ in range(3000)]
+        val = np.zeros(len(e), dtype=np.uint64)
+        comp = np.zeros(len(e), dtype=np.uint64)
+        call(ma.v_table, e, val, comp, len(e), mont(r), mont(scale), mont(1), pinv, p)
+        want = [(scale * pow(r, x, p)) % p for x in e]
+        assert val.tolist() == want
+        assert comp.tolist() == [(v << 64) // p for v in want]
