@@ -119,19 +119,34 @@ int mag_cmp(const u64 *a, size_t na, const u64 *b, size_t nb) {
 }
 
 // r[0 .. na+nb) = a * b by schoolbook, the outer loop over the shorter
-// operand B (%multiply-storage/schoolboy, src/multiply.lisp:37-42)
+// operand B (%multiply-storage/schoolboy, src/multiply.lisp:37-42).  The
+// first row is written, the later ones accumulate: row j writes r[j + na]
+// and adds into r[j .. j + na).
 void mag_mul(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *r) {
-  std::fill(r, r + na + nb, 0);
-  for (size_t j = 0; j < nb; j++) {
-    const u64 bj = b[j];
-    if (!bj) continue;
+  if (nb == 0) {
+    std::fill(r, r + na, 0);
+    return;
+  }
+  {
+    const u64 b0 = b[0];
     u64 carry = 0;
-    u64 *rj = r + j;
     for (size_t i = 0; i < na; i++) {
-      const u128 t = (u128)a[i] * bj + rj[i] + carry;
-      rj[i] = (u64)t;
+      const u128 t = (u128)a[i] * b0 + carry;
+      r[i] = (u64)t;
       carry = (u64)(t >> 64);
     }
+    r[na] = carry;
+  }
+  for (size_t j = 1; j < nb; j++) {
+    const u64 bj = b[j];
+    u64 carry = 0;
+    u64 *rj = r + j;
+    if (bj)
+      for (size_t i = 0; i < na; i++) {
+        const u128 t = (u128)a[i] * bj + rj[i] + carry;
+        rj[i] = (u64)t;
+        carry = (u64)(t >> 64);
+      }
     rj[na] = carry;
   }
 }
