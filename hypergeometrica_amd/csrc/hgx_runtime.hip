@@ -2688,6 +2688,47 @@ size_t hgx_internal_device_room() {
   return g_ctx.ready ? memory_for_product(true) : 0;
 }
 
+int hgx_internal_upload_gather(uint64_t *d_dst, const uint64_t *const *src, const size_t *nlimbs, size_t count) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (count && (!d_dst || !src || !nlimbs)) return fail(HGX_EINVAL, "null gather upload");
+  const hipStream_t s = g_ctx.stream;
+  PinStage &P = g_pin;
+  rc = pin_init(P);
+  if (rc) return rc;
+  PinScope pin;
+  size_t k = 0, in = 0;  // the next piece and its limbs already staged
+  u64 *dst = d_dst;
+  while (k < count) {
+    const int i = P.next;
+    P.next = (i + 1) % kPinBufs;
+    rc = pin_settle(P, i);
+    if (rc) return rc;
+    // fill staging buffer i from the pieces, on the copy threads
+    std::vector<Piece> pieces;
+    size_t fill = 0;
+    while (k < count && fill < kPinChunk) {
+      const size_t take = std::min((nlimbs[k] - in) * 8, kPinChunk - fill);
+      if (take) pieces.push_back(Piece{P.buf[i] + fill, src[k] + in, take});
+      fill += take;
+      in += take / 8;
+      if (in == nlimbs[k]) {
+        k++;
+        in = 0;
+      }
+    }
+    if (fill == 0) break;
+    copy_pieces(pieces);
+    HIPCHK(hipMemcpyAsync(dst, P.buf[i], fill, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(P.ev[i], s));
+    P.state[i] = 1;
+    dst += fill / 8;
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return HGX_OK;
+}
+
 int hgx_internal_sums_device(HgxSumJob *jobs, size_t count, int *signs, uint64_t *sizes) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
   int rc = check_ready();

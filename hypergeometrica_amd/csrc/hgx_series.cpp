@@ -736,47 +736,36 @@ int sums_dev(std::vector<Num> &o2, size_t cnt) {
   return HGX_OK;
 }
 
-// Moves every live value to device memory: one region, packed on the host
-// threads into pinned memory (pageable when that is not available) and
-// uploaded by one copy.
+// Moves every live value to device memory: one region, filled by one
+// gathered upload through the library's pinned staging (no pinned memory of
+// the split's own to fault in).
 int to_device(std::vector<std::pair<u64, Part>> &live) {
   std::vector<Num *> vals;
   vals.reserve(4 * live.size());
-  std::vector<size_t> off;
-  off.reserve(4 * live.size() + 1);
+  std::vector<const u64 *> src;
+  std::vector<size_t> len;
+  src.reserve(4 * live.size());
+  len.reserve(4 * live.size());
   size_t total = 0;
   for (auto &e : live)
     for (Num *v : {&e.second.P, &e.second.Q, &e.second.B, &e.second.R}) {
       vals.push_back(v);
-      off.push_back(total);
-      total += v->s ? v->n : 0;
+      const size_t n = v->s ? v->n : 0;
+      src.push_back(v->p);
+      len.push_back(n);
+      total += n;
     }
   std::shared_ptr<DevRegion> reg;
   int rc = new_dev_region(total, reg);
   if (rc) return rc;
-  void *pinned = nullptr;
-  std::unique_ptr<u64[]> pageable;
-  u64 *pack = nullptr;
-  if (hgx_host_alloc(8 * std::max<size_t>(total, 1), &pinned) == HGX_OK) {
-    pack = static_cast<u64 *>(pinned);
-  } else {
-    pageable.reset(new (std::nothrow) u64[std::max<size_t>(total, 1)]);
-    if (!pageable) return hgx_internal_fail(HGX_ENOMEM, "out of host memory for the upload of a split");
-    pack = pageable.get();
-  }
-  const size_t per = 4096;  // values packed by one task
-  rc = parallel_for((vals.size() + per - 1) / per, [&](size_t t) {
-    for (size_t k = t * per; k < std::min(vals.size(), (t + 1) * per); k++)
-      if (vals[k]->s) std::copy(vals[k]->p, vals[k]->p + vals[k]->n, pack + off[k]);
-    return (int)HGX_OK;
-  });
-  if (!rc && total) rc = hgx_upload(reg->base, pack, total, nullptr);
-  if (pinned) (void)hgx_host_free(pinned);
+  rc = hgx_internal_upload_gather(reg->base, src.data(), len.data(), src.size());
   if (rc) return rc;
+  size_t at = 0;
   for (size_t k = 0; k < vals.size(); k++) {
     Num &v = *vals[k];
-    v.p = reg->base + off[k];
-    v.cap = v.s ? v.n : 0;
+    v.p = reg->base + at;
+    at += len[k];
+    v.cap = len[k];
     v.dev = true;
     v.own = reg;
   }

@@ -1,5 +1,6 @@
-// hgx_sum.h -- batched signed sums of device limb vectors, shared by the
-// binary-splitting engine (hgx_series.cpp), the runtime and the kernels.
+// hgx_sum.h -- the runtime's internal entry points for the binary-splitting
+// engine (hgx_series.cpp): batched signed sums of device limb vectors (also
+// shared with the kernels), the device memory left, and gathered uploads.
 // The sum R = B_r Q_r R_l + B_l P_l R_r of COMBINE
 // (src/binary-splitting.lisp:112-121) for every node of a tree height whose
 // values stay in device memory: mpz-+ on signed values
@@ -26,3 +27,9 @@ int hgx_internal_sums_device(HgxSumJob *jobs, size_t count, int *signs, uint64_t
 
 // device memory the library may still allocate under its HBM budget
 size_t hgx_internal_device_room();
+
+// D_DST[...] = the COUNT host pieces SRC[k] (NLIMBS[k] limbs each, pageable
+// memory) one after the other: gathered by copy threads into the library's
+// pinned staging buffers, each moved by one DMA copy while the next fills.
+// Returns when the copies are done.
+int hgx_internal_upload_gather(uint64_t *d_dst, const uint64_t *const *src, const size_t *nlimbs, size_t count);

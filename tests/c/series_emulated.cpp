@@ -60,12 +60,11 @@ extern "C" int hgx_download(uint64_t *d, const uint64_t *s, size_t n, void *) {
   memcpy(d, s, 8 * n);
   return HGX_OK;
 }
-extern "C" int hgx_host_alloc(size_t bytes, void **p) {
-  *p = malloc(bytes);
-  return *p ? HGX_OK : HGX_ENOMEM;
-}
-extern "C" int hgx_host_free(void *p) {
-  free(p);
+int hgx_internal_upload_gather(uint64_t *d_dst, const uint64_t *const *src, const size_t *nlimbs, size_t count) {
+  for (size_t k = 0; k < count; k++) {
+    memcpy(d_dst, src[k], 8 * nlimbs[k]);
+    d_dst += nlimbs[k];
+  }
   return HGX_OK;
 }
 extern "C" int hgx_mul_batch_device(const hgx_mul_desc *d, size_t count, void *) { return hgx_mul_batch(d, count); }
