@@ -536,6 +536,9 @@ def test_batch_graph_replays_equal_eager(hgx, oracle):
         a = oracle.splitmix64(11 * i + 1, max(na, 1))
         b = a if i % 9 == 0 else oracle.splitmix64(11 * i + 2, max(nb, 1))
         pairs.append((a, b))
+    for i in range(40):  # small products: groups of several per tile (packed transforms)
+        na, nb = int(rng.integers(20, 200)), int(rng.integers(20, 200))
+        pairs.append((oracle.splitmix64(3000 + i, na), oracle.splitmix64(4000 + i, nb)))
     _lib.check(L.hgx_graphs_enable(0))
     eager = hgx.mul_batch(pairs)
     _lib.check(L.hgx_graphs_enable(1))
