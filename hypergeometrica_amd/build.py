@@ -28,7 +28,7 @@ HIPCC_FLAGS = [
 # kernel families compile in parallel; the runtime and the series engine
 # (hgx_series.cpp, host code only) hold no device code
 SOURCES = ["hgx_runtime.hip", "hgx_k_col.hip", "hgx_k_row_mul.hip", "hgx_k_row_sqr.hip", "hgx_k_row_tr.hip",
-           "hgx_k_arith.hip", "hgx_k_probe.hip", "hgx_series.cpp"]
+           "hgx_k_arith.hip", "hgx_k_fold.hip", "hgx_k_probe.hip", "hgx_series.cpp"]
 OBJDIR = os.path.join(HERE, "_build")
 
 

@@ -55,6 +55,13 @@ u64 hgx_addsub_blocks(u64 n);
 struct HgxSumJob;
 void hgx_launch_sums(hipStream_t s, const HgxSumJob *jobs, u64 njobs, u64 nblk, uint8_t *agg, uint8_t *blkin,
                      u64 *res);
+// the leaf folds of binary splitting (hgx_fold.hpp): BOUNDS holds each
+// leaf's lo and up, CODES 4 ints per leaf; then each leaf's P, Q, B, R
+// packed at DST + OFF[t]
+struct FoldSeries;
+struct FoldSlots;
+void hgx_launch_fold(hipStream_t s, const FoldSeries &S, const u64 *bounds, u64 count, const FoldSlots &F, int *codes);
+void hgx_launch_fold_pack(hipStream_t s, const FoldSlots &F, u64 count, const int *codes, const u64 *off, u64 *dst);
 // register-only probe of one modular operation (0 Shoup product, 1
 // Montgomery product, 2 sum and difference, 3 forward butterfly, 4 inverse
 // butterfly), hgx_probe_chains() chains per thread, ITERS steps each;

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/hgx.h"
+#include "hgx_fold.hpp"
 #include "hgx_launch.hpp"
 #include "hgx_sum.h"
 
@@ -2772,6 +2773,100 @@ int hgx_internal_sums_device(HgxSumJob *jobs, size_t count, int *signs, uint64_t
   for (size_t j = 0; j < count; j++) {
     sizes[j] = hres[4 * j + 1];
     signs[j] = hres[4 * j + 2] == 1 ? 1 : (hres[4 * j + 2] == 2 ? -1 : 0);
+  }
+  return HGX_OK;
+}
+
+// The leaf folds on the device, in a scratch block SCR laid out as: the
+// fold buffers (stride ST, CAP[b] limbs each), the leaves' bounds, their
+// codes, their offsets.  The packed values go to a new block *OUT.
+static int fold_device_in(void *scr, const FoldSeries &S, const uint64_t *bounds, size_t count, const uint32_t *cap,
+                          size_t st, size_t per, int32_t *codes, uint64_t **out, size_t *nlimbs) {
+  const hipStream_t s = g_ctx.stream;
+  u64 *bufs = (u64 *)scr;
+  u64 *dbounds = bufs + st * per;
+  int *dcodes = (int *)(dbounds + 2 * count);
+  u64 *doff = (u64 *)(dcodes + 4 * count);
+  FoldSlots F;
+  size_t at = 0;
+  for (int b = 0; b < kFoldBufs; b++) {
+    F.buf[b] = bufs + at * st;
+    F.cap[b] = cap[b];
+    at += cap[b];
+  }
+  F.stride = st;
+  {
+    PinScope pin;
+    int rc = h2d(dbounds, bounds, 16 * count, s);
+    if (rc) return rc;
+    hgx_launch_fold(s, S, dbounds, count, F, dcodes);
+    HIPCHK(hipGetLastError());
+    rc = d2h(codes, dcodes, 16 * count, s);
+    if (!rc) rc = pin_flush();
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  std::vector<u64> off(count);
+  size_t total = 0;
+  for (size_t t = 0; t < count; t++) {
+    if (codes[4 * t] == kFoldFailed) return fail(HGX_ETOOBIG, "a leaf outgrew the device fold's buffers");
+    off[t] = total;
+    for (int w = 0; w < 4; w++) total += (size_t)std::abs((long long)codes[4 * t + w]);
+  }
+  void *o = nullptr;
+  int rc = dev_block_alloc(std::max<size_t>(total, 1) * 8, &o);
+  if (rc) return rc;
+  {
+    PinScope pin;
+    rc = h2d(doff, off.data(), 8 * count, s);
+    if (!rc) {
+      hgx_launch_fold_pack(s, F, count, dcodes, doff, (u64 *)o);
+      const hipError_t e = hipGetLastError();
+      if (e == hipSuccess) {
+        const hipError_t e2 = hipStreamSynchronize(s);
+        if (e2 != hipSuccess)
+          rc = fail(HGX_EDEVICE, std::string("leaf pack failed: ") + hipGetErrorString(e2));
+      } else {
+        rc = fail(HGX_EDEVICE, std::string("leaf pack launch failed: ") + hipGetErrorString(e));
+      }
+    }
+  }
+  if (rc) {
+    (void)dev_block_release(o, s);
+    return rc;
+  }
+  *out = (uint64_t *)o;
+  *nlimbs = total;
+  return HGX_OK;
+}
+
+int hgx_internal_fold_device(const FoldSeries *S, const uint64_t *bounds, size_t count, const uint32_t *cap,
+                             int32_t *codes, uint64_t **d_vals, size_t *nlimbs) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!S || !bounds || !cap || !codes || !d_vals || !nlimbs || count == 0)
+    return fail(HGX_EINVAL, "bad leaf fold arguments");
+  *d_vals = nullptr;
+  *nlimbs = 0;
+  if (count >= ((size_t)1 << 28)) return fail(HGX_ETOOBIG, "too many leaves for the device fold");
+  size_t per = 0;  // limbs of one leaf's buffers
+  for (int b = 0; b < kFoldBufs; b++) {
+    if (cap[b] < (u32)kFoldTV2 + 1 || cap[b] >= (1u << 30)) return fail(HGX_EINVAL, "bad leaf fold capacity");
+    per += cap[b];
+  }
+  const size_t st = (count + 63) & ~(size_t)63;  // the stride, whole waves of leaves
+  const size_t bytes = 8 * st * per + 16 * count + 16 * count + 8 * count;
+  void *scr = nullptr;
+  rc = dev_block_alloc(bytes, &scr);
+  if (rc) return rc;
+  rc = fold_device_in(scr, *S, bounds, count, cap, st, per, codes, d_vals, nlimbs);
+  const int rc2 = dev_block_release(scr, g_ctx.stream);
+  if (rc) return rc;
+  if (rc2) {
+    (void)dev_block_release(*d_vals, g_ctx.stream);
+    *d_vals = nullptr;
+    return rc2;
   }
   return HGX_OK;
 }

@@ -6,20 +6,20 @@
 // floor((lower + upper) / 2), recurse, COMBINE (:112-121) the two partials
 // with seven products and one sum, and at one term the base case (:100-110).
 // This engine builds the same tree, folds every subtree of at most
-// kFoldTerms terms on the host (the partial of an interval is the same four
-// integers for any split, so a left fold of the base cases gives the
-// recursion's numbers), and then evaluates the tree bottom up, one height at
-// a time.  A height's products go out in two rounds (five products that need
+// kFoldTerms terms (the partial of an interval is the same four integers for
+// any split, so a left fold of the base cases gives the recursion's
+// numbers), and then evaluates the tree bottom up, one height at a time.  A height's products go out in two rounds (five products that need
 // only the children, then the two that need round-1 results).  While the
 // values are in host memory, a product whose smaller operand has more than
 // *ntt-multiply-threshold* limbs (src/multiply.lisp:9) joins one batched GPU
 // call (hgx_mul_batch), the others are multiplied on the host by schoolbook
 // while the GPU works, as the reference's mpz-* multiplies them
 // (src/multiply.lisp:21-42).  A split whose heights hold enough limbs moves
-// its values to device memory at its first height and keeps them there:
-// each round is one batched device call (hgx_mul_batch_device) and the sums
-// R run on the device, with one upload of the folded leaves and one download
-// of the result.  All arithmetic is exact, so every partial equals the
+// its values to device memory and keeps them there: each round is one
+// batched device call (hgx_mul_batch_device) and the sums R run on the
+// device, with one download of the result.  Such a split folds its leaves on
+// the device too, one thread per leaf (hgx_fold.hpp); other splits fold them
+// on host threads (and a split that moves later uploads them).  All arithmetic is exact, so every partial equals the
 // reference's.
 //
 // Series terms are polynomials in k with 64-bit signed coefficients and an
@@ -42,6 +42,7 @@
 #include <vector>
 
 #include "../../include/hgx.h"
+#include "hgx_fold.hpp"
 #include "hgx_sum.h"
 
 typedef uint64_t u64;
@@ -772,6 +773,85 @@ int to_device(std::vector<std::pair<u64, Part>> &live) {
   return HGX_OK;
 }
 
+// HGX_SERIES_DEVICE: '0' keeps a split on the host (0), '1' sends any split
+// to the device (2), unset or other lets the sizes decide (1)
+int device_mode() {
+  const char *env = std::getenv("HGX_SERIES_DEVICE");
+  return env && env[0] == '0' ? 0 : (env && env[0] == '1' ? 2 : 1);
+}
+
+// The leaves folded on the device (hgx_internal_fold_device), for a split
+// whose heights will live there: the leaves' values, bounded from the
+// series' coefficients, hold at least kDeviceMinLimbs limbs and fit the
+// device several times over (or HGX_SERIES_DEVICE=1).  DONE is set when
+// every leaf's partial is in one device region; otherwise the split folds
+// on the host -- also when a term value exceeds kFoldTV limbs, a leaf
+// outgrows its buffers or device memory is short.  TIGHT (a test hook)
+// gives the fold buffers their smallest capacities, so that leaves of more
+// than a few limbs fail on the device.
+int leaves_on_device(const Series &s, const std::vector<std::pair<u64, u64>> &blocks, u64 lower, u64 upper,
+                     bool tight, std::vector<Part> &leaf, bool &done) {
+  done = false;
+  if (!hgx_internal_ready() || blocks.empty()) return HGX_OK;
+  const int mode = device_mode();
+  if (mode == 0) return HGX_OK;
+  FoldSeries fs;
+  std::memset(&fs, 0, sizeof fs);
+  const Poly *polys[4] = {&s.a, &s.b, &s.p, &s.q};
+  for (int i = 0; i < 4; i++) {
+    FoldPoly &f = fs.f[i];
+    f.deg = polys[i]->deg;
+    f.at_zero = polys[i]->at_zero ? 1 : 0;
+    f.v0 = polys[i]->v0;
+    for (int j = 0; j <= f.deg; j++) f.c[j] = polys[i]->c[j];
+  }
+  u64 terms = 0;
+  for (const auto &b : blocks) terms = std::max(terms, b.second - b.first);
+  u32 cap[kFoldBufs];
+  u64 leaf_limbs = 0;
+  if (!fold_caps(fs, lower, upper, terms, cap, leaf_limbs)) return HGX_OK;
+  const size_t count = blocks.size();
+  u64 per = 0;
+  for (int b = 0; b < kFoldBufs; b++) per += cap[b];
+  const size_t est = leaf_limbs * count;
+  if (mode == 1 && est < kDeviceMinLimbs) return HGX_OK;
+  const size_t room = hgx_internal_device_room();
+  if (est > room / (8 * kDeviceRoomFactor) || 8 * per * count > room / 2) return HGX_OK;
+  if (tight)
+    for (int b = 0; b < kFoldBufs; b++) cap[b] = kFoldTV2 + 1;
+  std::vector<u64> bounds(2 * count);
+  for (size_t t = 0; t < count; t++) {
+    bounds[2 * t] = blocks[t].first;
+    bounds[2 * t + 1] = blocks[t].second;
+  }
+  std::vector<int32_t> codes(4 * count);
+  u64 *dv = nullptr;
+  size_t total = 0;
+  const int rc = hgx_internal_fold_device(&fs, bounds.data(), count, cap, codes.data(), &dv, &total);
+  if (rc == HGX_ETOOBIG || rc == HGX_ENOMEM) return HGX_OK;  // the host folds
+  if (rc) return rc;
+  std::shared_ptr<DevRegion> reg = std::make_shared<DevRegion>();
+  reg->base = dv;
+  reg->cap = std::max<size_t>(total, 1);
+  size_t at = 0;
+  for (size_t t = 0; t < count; t++) {
+    Num *dst[4] = {&leaf[t].P, &leaf[t].Q, &leaf[t].B, &leaf[t].R};
+    for (int w = 0; w < 4; w++) {
+      const int32_t c = codes[4 * t + w];
+      Num &d = *dst[w];
+      d.s = c > 0 ? 1 : (c < 0 ? -1 : 0);
+      d.n = (size_t)(c < 0 ? -(int64_t)c : c);
+      d.cap = d.n;
+      d.p = dv + at;
+      d.dev = true;
+      d.own = reg;
+      at += d.n;
+    }
+  }
+  done = true;
+  return HGX_OK;
+}
+
 // Whether the split moves to the GPU before the height whose round-1
 // products are XS[i] * YS[i]: once a height's products hold at least
 // kDeviceMinLimbs limbs, the values stay in device memory from there on --
@@ -786,8 +866,7 @@ bool move_to_device(std::vector<std::pair<u64, Part>> &live, const std::vector<c
                     const std::vector<const Num *> &ys, int &rc) {
   rc = HGX_OK;
   if (!hgx_internal_ready()) return false;
-  const char *env = std::getenv("HGX_SERIES_DEVICE");
-  const int mode = env && env[0] == '0' ? 0 : (env && env[0] == '1' ? 2 : 1);
+  const int mode = device_mode();
   if (mode == 0) return false;
   size_t all = 0;
   for (size_t i = 0; i < xs.size(); i++)
@@ -890,10 +969,18 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
   std::vector<std::array<u64, 4>> nodes;
   std::vector<std::pair<u64, u64>> blocks;
   build_tree(lower, upper, nodes, blocks);
-  // the leaves, folded in chunks: one thread folds a chunk's leaves, copies
+  // test hook: device work at this height reports a failure (height 0: the
+  // leaves' folds get buffers too small for them; others: HGX_ENOMEM)
+  const char *fail_env = std::getenv("HGX_SERIES_FAIL_HEIGHT");
+  const long fail_height = fail_env ? std::atol(fail_env) : -1;
+  // the leaves, folded on the device for a split whose heights go there;
+  // else on the host in chunks: one thread folds a chunk's leaves, copies
   // their partials into one region and frees its own fold vectors
   std::vector<Part> leaf(blocks.size());
-  const size_t nchunk = (blocks.size() + kLeafChunk - 1) / kLeafChunk;
+  bool leaves_dev = false;
+  rc = leaves_on_device(s, blocks, lower, upper, fail_height == 0, leaf, leaves_dev);
+  if (rc) return rc;
+  const size_t nchunk = leaves_dev ? 0 : (blocks.size() + kLeafChunk - 1) / kLeafChunk;
   rc = parallel_for(nchunk, [&](size_t c) {
     const size_t i0 = c * kLeafChunk, i1 = std::min(blocks.size(), i0 + kLeafChunk);
     std::vector<AccPart> acc(i1 - i0);
@@ -936,10 +1023,7 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     return it->second;
   };
   auto prod_limbs = [](const Num &x, const Num &y) -> size_t { return x.s && y.s ? x.n + y.n : 0; };
-  bool on_dev = false, dev_off = false;
-  // test hook: device rounds at this height report HGX_ENOMEM
-  const char *fail_env = std::getenv("HGX_SERIES_FAIL_HEIGHT");
-  const long fail_height = fail_env ? std::atol(fail_env) : -1;
+  bool on_dev = leaves_dev, dev_off = false;
   std::stable_sort(nodes.begin(), nodes.end(), [](const auto &x, const auto &y) { return x[0] < y[0]; });
   for (size_t h0 = 0; h0 < nodes.size();) {
     size_t h1 = h0;

@@ -33,3 +33,15 @@ size_t hgx_internal_device_room();
 // pinned staging buffers, each moved by one DMA copy while the next fills.
 // Returns when the copies are done.
 int hgx_internal_upload_gather(uint64_t *d_dst, const uint64_t *const *src, const size_t *nlimbs, size_t count);
+
+// The binary-splitting leaves folded on the device (hgx_fold.hpp): leaf t
+// covers [BOUNDS[2t], BOUNDS[2t+1]), CAP holds the fold buffers'
+// capacities.  On success CODES[4t + w] = sign * limbs of leaf t's P, Q, B,
+// R (w = 0..3), and *D_VALS is a block of hgx_device_alloc memory (freed by
+// hgx_device_free) that holds them leaf after leaf, P, Q, B, R each
+// |CODES[...]| limbs, *NLIMBS in all.  HGX_ETOOBIG when a leaf outgrew its
+// capacities, HGX_ENOMEM when device memory is short: then nothing is left
+// allocated.  Returns when the values are in place.
+struct FoldSeries;
+int hgx_internal_fold_device(const FoldSeries *S, const uint64_t *bounds, size_t count, const uint32_t *cap,
+                             int32_t *codes, uint64_t **d_vals, size_t *nlimbs);

@@ -8,13 +8,16 @@
 //
 // Usage: series_emulated WHICH LOWER UPPER.  WHICH 0: the Chudnovsky series;
 // 1: a series with negative a, b, p and q; 2: a = 1, p = -1 (1 at k = 0),
-// whose partial sums cancel.  Prints P, Q, B, R as SIGN:HEX, one per line.
+// whose partial sums cancel; 3: p = 2^40 k^7, q = k + 3, whose term values
+// near k = 2^39 exceed what the device fold takes.  Prints P, Q, B, R as
+// SIGN:HEX, one per line.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 
+#include "../../hypergeometrica_amd/csrc/hgx_fold.hpp"
 #include "../../hypergeometrica_amd/csrc/hgx_sum.h"
 #include "../../include/hgx.h"
 
@@ -69,6 +72,36 @@ int hgx_internal_upload_gather(uint64_t *d_dst, const uint64_t *const *src, cons
 }
 extern "C" int hgx_mul_batch_device(const hgx_mul_desc *d, size_t count, void *) { return hgx_mul_batch(d, count); }
 size_t hgx_internal_device_room() { return (size_t)1 << 40; }
+
+// the device leaf folds (hgx_sum.h): the kernels' own fold_leaf and
+// fold_pack_leaf, one leaf after another, each fold buffer a separate
+// allocation of exactly its capacity (so that the sanitizer sees a write
+// past it); reports on stderr that it ran
+int hgx_internal_fold_device(const FoldSeries *S, const uint64_t *bounds, size_t count, const uint32_t *cap,
+                             int32_t *codes, uint64_t **d_vals, size_t *nlimbs) {
+  const size_t st = (count + 63) & ~(size_t)63;
+  FoldSlots F;
+  for (int b = 0; b < kFoldBufs; b++) {
+    F.buf[b] = (uint64_t *)malloc(8 * st * cap[b]);
+    F.cap[b] = cap[b];
+  }
+  F.stride = st;
+  bool ok = true;
+  for (size_t t = 0; t < count && ok; t++) ok = fold_leaf(*S, bounds[2 * t], bounds[2 * t + 1], t, F, codes + 4 * t);
+  size_t total = 0;
+  for (size_t k = 0; ok && k < 4 * count; k++) total += (size_t)llabs(codes[k]);
+  uint64_t *out = ok ? (uint64_t *)malloc(8 * (total + 1)) : nullptr;
+  for (size_t t = 0, at = 0; ok && t < count; t++) {
+    fold_pack_leaf(F, t, codes + 4 * t, out + at);
+    for (int w = 0; w < 4; w++) at += (size_t)llabs(codes[4 * t + w]);
+  }
+  for (int b = 0; b < kFoldBufs; b++) free(F.buf[b]);
+  fprintf(stderr, "emulated device fold: %zu leaves, %s\n", count, ok ? "done" : "a leaf failed");
+  if (!ok) return HGX_ETOOBIG;
+  *d_vals = out;
+  *nlimbs = total;
+  return HGX_OK;
+}
 
 // the device signed sums (hgx_sum.h), one limb at a time
 static int mag_cmp(const std::vector<uint64_t> &a, const std::vector<uint64_t> &b) {
@@ -150,13 +183,21 @@ int main(int argc, char **argv) {
     s.coef[3][2] = 5;
     s.at_zero[3] = 1;
     s.value0[3] = -1;
-  } else {
+  } else if (which == 2) {
     s.coef[0][0] = 1;
     s.coef[1][0] = 1;
     s.coef[2][0] = -1;
     s.at_zero[2] = 1;
     s.value0[2] = 1;
     s.coef[3][0] = 1;
+  } else {  // p(k) = 2^40 k^7: term values beyond 256 bits near k = 2^39
+    s.coef[0][0] = 1;
+    s.coef[1][0] = 1;
+    s.degree[2] = 7;
+    s.coef[2][7] = 1LL << 40;
+    s.degree[3] = 1;
+    s.coef[3][0] = 3;
+    s.coef[3][1] = 1;
   }
   hgx_partial *p = nullptr;
   const int rc = hgx_series_split(&s, lo, up, &p);

@@ -139,3 +139,24 @@ def test_device_memory_exhausted_goes_on_on_the_host(hgx, monkeypatch, fail_heig
     monkeypatch.setenv("HGX_SERIES_FAIL_HEIGHT", str(fail_height))
     s = SERIES[name]()
     assert _tuple(S.binary_split_native(s, lo, up)) == SO.binary_split(s, lo, up)
+
+
+@pytest.mark.parametrize("name,lo,up", [("chudnovsky", 0, 3000), ("catalan", 0, 4000), ("exp(-5/3)", 17, 4017)])
+def test_leaves_too_large_for_the_device_fold_fold_on_the_host(hgx, monkeypatch, name, lo, up):
+    # HGX_SERIES_FAIL_HEIGHT=0: the device fold gets buffers of its smallest
+    # capacity, which leaves of more than a few limbs outgrow; the kernel
+    # reports them and the split folds every leaf on the host instead
+    monkeypatch.setenv("HGX_SERIES_DEVICE", "1")
+    monkeypatch.setenv("HGX_SERIES_FAIL_HEIGHT", "0")
+    s = SERIES[name]()
+    assert _tuple(S.binary_split_native(s, lo, up)) == SO.binary_split(s, lo, up)
+
+
+@pytest.mark.parametrize("name,lo,up", [("chudnovsky", 0, 1), ("chudnovsky", 0, 32), ("apery", 5, 20),
+                                        ("exp(7/2)", 3, 36), ("e", 0, 65)])
+def test_single_and_few_leaves_folded_on_the_device(hgx, monkeypatch, name, lo, up):
+    # one or a few leaves (at most 32 terms each) folded on the device, the
+    # result read back from the leaves' own region or after a height or two
+    monkeypatch.setenv("HGX_SERIES_DEVICE", "1")
+    s = SERIES[name]()
+    assert _tuple(S.binary_split_native(s, lo, up)) == SO.binary_split(s, lo, up)
