@@ -846,8 +846,8 @@ static __device__ __align__(256) uint64_t scratch[kNoAllocBytes / 8];
 
 namespace {
 
-MulJob make_job(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, int n, bool sqr, char *base) {
-  const WorkLayout wl = work_layout(n, sqr);
+MulJob make_job(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, bool sqr, const WorkLayout &wl,
+                char *base) {
   MulJob J{};
   J.a = a;
   J.na = na;
@@ -863,6 +863,9 @@ MulJob make_job(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, int 
   J.agg = (uint8_t *)(base + wl.agg);
   J.blkin = (uint8_t *)(base + wl.blkin);
   return J;
+}
+MulJob make_job(const u64 *a, size_t na, const u64 *b, size_t nb, u64 *out, int n, bool sqr, char *base) {
+  return make_job(a, na, b, nb, out, sqr, work_layout(n, sqr), base);
 }
 
 // Enqueue NJOBS products of transform length 2^n (all squares or all
@@ -1412,6 +1415,11 @@ int launch_graph(const std::vector<u64> &key, hipStream_t s, Prepare prepare, En
 // whose work memory exceeds the budget are split.
 int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) {
   struct Item { size_t idx; int n; bool sqr; size_t bytes; };
+  // one work layout per (length, square) key: key = 2 n + sqr
+  constexpr int kKeys = 2 * (kMaxBits + 1);
+  WorkLayout layout[kKeys];
+  bool have[kKeys] = {};
+  size_t per_key[kKeys + 1] = {};
   std::vector<Item> items;
   std::vector<std::pair<size_t, Split>> splits;  // unbalanced products, done first
   items.reserve(count);
@@ -1426,11 +1434,22 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
       splits.push_back({i, sp});
       continue;
     }
-    items.push_back({i, n, sqr, work_layout(n, sqr).total});
+    const int k = 2 * n + (sqr ? 1 : 0);
+    if (!have[k]) {
+      layout[k] = work_layout(n, sqr);
+      have[k] = true;
+    }
+    per_key[k + 1]++;
+    items.push_back({i, n, sqr, layout[k].total});
   }
-  std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
-    return x.n != y.n ? x.n < y.n : x.sqr < y.sqr;
-  });
+  {
+    // items in order of (length, square), stable: a counting sort over the
+    // keys (linear in the batch, which can hold ~10^6 small products)
+    for (int k = 0; k < kKeys; k++) per_key[k + 1] += per_key[k];
+    std::vector<Item> sorted(items.size());
+    for (const Item &it : items) sorted[per_key[2 * it.n + (it.sqr ? 1 : 0)]++] = it;
+    items.swap(sorted);
+  }
   int rc = ws_acquire(s);
   if (rc) return rc;
   for (auto &sv : splits) {
@@ -1462,7 +1481,8 @@ int batch_device_locked(const hgx_mul_desc *descs, size_t count, hipStream_t s) 
     char *base = (char *)g_ctx.ws.ptr;
     for (size_t i = pos; i < end; i++) {
       const hgx_mul_desc &d = descs[items[i].idx];
-      g_ctx.jobs_host[i - pos] = make_job(d.a, d.na, d.b, d.nb, d.out, items[i].n, items[i].sqr, base);
+      g_ctx.jobs_host[i - pos] =
+          make_job(d.a, d.na, d.b, d.nb, d.out, items[i].sqr, layout[2 * items[i].n + (items[i].sqr ? 1 : 0)], base);
       base += items[i].bytes;
     }
     MulJob *table = (MulJob *)g_ctx.jobs_dev.ptr;

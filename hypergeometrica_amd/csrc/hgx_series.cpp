@@ -1040,9 +1040,20 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
       xs.reserve(5 * cnt);
       ys.reserve(5 * cnt);
       room.reserve(5 * cnt);
+      // the children by a cursor over the live partials: a height's nodes
+      // come left to right (post order, then a stable sort by height), and
+      // a node's right child directly follows its left one
+      size_t cur = 0;
       for (size_t i = 0; i < cnt; i++) {
-        L[i] = &find(nodes[h0 + i][1]);
-        R[i] = &find(nodes[h0 + i][2]);
+        const u64 lo = nodes[h0 + i][1], mid = nodes[h0 + i][2];
+        while (cur < live.size() && live[cur].first < lo) cur++;
+        if (cur + 1 < live.size() && live[cur].first == lo && live[cur + 1].first == mid) {
+          L[i] = &live[cur].second;
+          R[i] = &live[cur + 1].second;
+        } else {
+          L[i] = &find(lo);
+          R[i] = &find(mid);
+        }
         const Part &l = *L[i], &r = *R[i];
         for (auto pr : {std::make_pair(&l.P, &r.P), std::make_pair(&l.Q, &r.Q), std::make_pair(&l.B, &r.B),
                         std::make_pair(&r.Q, &l.R), std::make_pair(&l.P, &r.R)}) {
@@ -1124,9 +1135,17 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     gone.reserve(cnt);
     for (size_t i = 0; i < cnt; i++) gone.push_back(nodes[h0 + i][2]);
     std::sort(gone.begin(), gone.end());
-    live.erase(std::remove_if(live.begin(), live.end(),
-                              [&](const auto &x) { return std::binary_search(gone.begin(), gone.end(), x.first); }),
-               live.end());
+    {
+      // both sorted: one merge pass keeps the others in order
+      size_t keep = 0, g = 0;
+      for (size_t k = 0; k < live.size(); k++) {
+        while (g < gone.size() && gone[g] < live[k].first) g++;
+        if (g < gone.size() && gone[g] == live[k].first) continue;
+        if (keep != k) live[keep] = std::move(live[k]);
+        keep++;
+      }
+      live.erase(live.begin() + keep, live.end());
+    }
     h0 = h1;
   }
   // the result gets a region of its own: the last height's regions also
