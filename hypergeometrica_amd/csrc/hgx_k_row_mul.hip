@@ -75,6 +75,7 @@ bool hgx_launch_row_pack_mul(int sbits, int jbits, unsigned njobs, dim3 grid, hi
                            const RowParams &rp, const JobRef &jr) {
   if (jbits < 0 || sbits + jbits > 12) return false;
   switch (sbits) {
+  case 4: hipLaunchKernelGGL((k_row_pack<4, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;
   case 5: hipLaunchKernelGGL((k_row_pack<5, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;
   case 6: hipLaunchKernelGGL((k_row_pack<6, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;
   case 7: hipLaunchKernelGGL((k_row_pack<7, MODE_MUL>), grid, dim3(HGX_NT), 0, s, ps, rp, jr, jbits, njobs); return true;

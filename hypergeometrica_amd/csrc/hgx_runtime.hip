@@ -35,8 +35,8 @@ namespace {
 
 constexpr int kMaxBits = 32;   // largest transform: 2^32 words per residue array
 constexpr int kRowMaxBits = 12;
-// a group of whole-tile products of 2^5 .. 2^10 words packs several per tile
-constexpr int kPackMinBits = 5, kPackMaxBits = 10;
+// a group of whole-tile products of 2^4 .. 2^10 words packs several per tile
+constexpr int kPackMinBits = 4, kPackMaxBits = 10;
 // a multiply row pass with fewer 4096-word workgroups than this (two per CU
 // of the 256) runs on 2048-word tiles instead, and on 1024-word tiles when
 // the 2048-word ones are still fewer

@@ -280,7 +280,7 @@ def test_batch_equals_per_call(hgx, oracle):
         assert oracle.limbs_to_int(o) == oracle.limbs_to_int(a) * oracle.limbs_to_int(b)
 
 
-@pytest.mark.parametrize("n", [5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("n", [4, 5, 6, 7, 8, 9, 10])
 def test_small_products_packed_per_tile(hgx, oracle, n):
     # batches of one small transform length 2^n (several products per tile,
     # a partial last tile): random, all-ones and square operands against
