@@ -9,7 +9,8 @@
 // Usage: series_emulated WHICH LOWER UPPER.  WHICH 0: the Chudnovsky series;
 // 1: a series with negative a, b, p and q; 2: a = 1, p = -1 (1 at k = 0),
 // whose partial sums cancel; 3: p = 2^40 k^7, q = k + 3, whose term values
-// near k = 2^39 exceed what the device fold takes.  Prints P, Q, B, R as
+// near k = 2^39 exceed what the device fold takes; 4: p = 2^20 k^7, q = k + 3,
+// whose term values near k = 2^33 take 4 limbs.  Prints P, Q, B, R as
 // SIGN:HEX, one per line.
 #include <cstdint>
 #include <cstdio>
@@ -190,11 +191,11 @@ int main(int argc, char **argv) {
     s.at_zero[2] = 1;
     s.value0[2] = 1;
     s.coef[3][0] = 1;
-  } else {  // p(k) = 2^40 k^7: term values beyond 256 bits near k = 2^39
+  } else {  // 3: p(k) = 2^40 k^7, beyond 256 bits near k = 2^39; 4: p(k) = 2^20 k^7, under 256 bits near 2^33
     s.coef[0][0] = 1;
     s.coef[1][0] = 1;
     s.degree[2] = 7;
-    s.coef[2][7] = 1LL << 40;
+    s.coef[2][7] = which == 3 ? 1LL << 40 : 1LL << 20;
     s.degree[3] = 1;
     s.coef[3][0] = 3;
     s.coef[3][1] = 1;

@@ -160,3 +160,15 @@ def test_single_and_few_leaves_folded_on_the_device(hgx, monkeypatch, name, lo, 
     monkeypatch.setenv("HGX_SERIES_DEVICE", "1")
     s = SERIES[name]()
     assert _tuple(S.binary_split_native(s, lo, up)) == SO.binary_split(s, lo, up)
+
+
+@pytest.mark.parametrize("series,lo,up", [
+    (S.chudnovsky_series(), 1 << 39, (1 << 39) + 300),  # q(k) ~ 2^170: 3-limb term values
+    (S.Series(p=S.Poly((0, 0, 0, 0, 0, 0, 0, 1 << 20)), q=S.Poly((3, 1))), 1 << 33, (1 << 33) + 300),  # 4 limbs
+    (S.Series(p=S.Poly((0, 0, 0, 0, 0, 0, 0, 1 << 40)), q=S.Poly((3, 1))), 1 << 39, (1 << 39) + 300),  # > 256 bits
+])
+def test_device_fold_of_wide_term_values(hgx, monkeypatch, series, lo, up):
+    # term values of 3 and 4 limbs fold on the device; wider ones (the
+    # last series) are folded on the host
+    monkeypatch.setenv("HGX_SERIES_DEVICE", "1")
+    assert _tuple(S.binary_split_native(series, lo, up)) == SO.binary_split(series, lo, up)

@@ -21,8 +21,9 @@ SERIES = {
     1: lambda: S.Series(a=S.Poly((2, -7)), b=S.Poly((-3,)), p=S.Poly((1, 2, -1)), q=S.Poly((-4, 0, 5), at_zero=-1)),
     2: lambda: S.Series(a=S.Poly((1,)), p=S.Poly((-1,), at_zero=1)),
     3: lambda: S.Series(p=S.Poly((0, 0, 0, 0, 0, 0, 0, 1 << 40)), q=S.Poly((3, 1))),
+    4: lambda: S.Series(p=S.Poly((0, 0, 0, 0, 0, 0, 0, 1 << 20)), q=S.Poly((3, 1))),
 }
-K39 = 1 << 39
+K39, K33 = 1 << 39, 1 << 33
 
 
 @pytest.fixture(scope="module")
@@ -57,16 +58,17 @@ def _run(exe, which, lo, up, env):
     {"HGX_SERIES_DEVICE": "1", "HGX_SERIES_FAIL_HEIGHT": "6"},
 ], ids=["host", "device", "fail-h0", "fail-h1", "fail-h3", "fail-h6"])
 @pytest.mark.parametrize("which,lo,up", [(0, 0, 3000), (0, 1000, 2999), (1, 3, 5000), (2, 0, 1000), (2, 5, 4097),
-                                         (3, K39, K39 + 300)])
+                                         (3, K39, K39 + 300), (0, K39, K39 + 300), (4, K33, K33 + 300)])
 def test_emulated_device_heights_equal_depth_first(engine, env, which, lo, up):
     """Leaves fold on the device whenever the split goes there and their
-    term values fit 256 bits (series 3's do not: the host folds them); with
-    buffers made too small (fail-h0) a leaf fails and the host folds."""
+    term values fit 256 bits (series 3's do not: the host folds them; the
+    Chudnovsky terms near k = 2^39 take 3 limbs, series 4's 4); with buffers
+    made too small (fail-h0) a leaf fails and the host folds."""
     vals, err = _run(engine, which, lo, up, env)
     assert vals == SO.binary_split(SERIES[which](), lo, up)[2:]
     if env["HGX_SERIES_DEVICE"] == "0" or which == 3:
         assert "emulated device fold" not in err
-    elif env.get("HGX_SERIES_FAIL_HEIGHT") == "0" and which in (0, 1):
+    elif env.get("HGX_SERIES_FAIL_HEIGHT") == "0" and which in (0, 1, 4):
         assert "a leaf failed" in err
     else:
         assert "emulated device fold" in err and "done" in err
