@@ -182,6 +182,8 @@ HGX_HD bool fold_mul(const FoldBig &x, const FoldSmall<N> &v, FoldBig &o, u32 ca
   const u32 n = x.n + (u32)vn;
   if (n > cap) return false;
   u64 carry = 0;
+  // (unrolled: a thread issues several strided loads ahead of its carry chain)
+#pragma unroll 4
   for (u32 i = 0; i < x.n; i++) {  // the first row writes
     u64 lo, hi;
     mul64x64(x.p[i * st], v.m[0], lo, hi);
@@ -196,6 +198,7 @@ HGX_HD bool fold_mul(const FoldBig &x, const FoldSmall<N> &v, FoldBig &o, u32 ca
     if (j < vn) {
       const u64 vj = v.m[j];
       carry = 0;
+#pragma unroll 4
       for (u32 i = 0; i < x.n; i++) {
         u64 lo, hi;
         mul64x64(x.p[i * st], vj, lo, hi);

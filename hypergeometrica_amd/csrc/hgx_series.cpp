@@ -790,7 +790,7 @@ int device_mode() {
 // gives the fold buffers their smallest capacities, so that leaves of more
 // than a few limbs fail on the device.
 int leaves_on_device(const Series &s, const std::vector<std::pair<u64, u64>> &blocks, u64 lower, u64 upper,
-                     bool tight, std::vector<Part> &leaf, bool &done) {
+                     bool tight, std::vector<std::pair<u64, Part>> &leaf, bool &done) {
   done = false;
   if (!hgx_internal_ready() || blocks.empty()) return HGX_OK;
   const int mode = device_mode();
@@ -835,7 +835,8 @@ int leaves_on_device(const Series &s, const std::vector<std::pair<u64, u64>> &bl
   reg->cap = std::max<size_t>(total, 1);
   size_t at = 0;
   for (size_t t = 0; t < count; t++) {
-    Num *dst[4] = {&leaf[t].P, &leaf[t].Q, &leaf[t].B, &leaf[t].R};
+    Part &pt = leaf[t].second;
+    Num *dst[4] = {&pt.P, &pt.Q, &pt.B, &pt.R};
     for (int w = 0; w < 4; w++) {
       const int32_t c = codes[4 * t + w];
       Num &d = *dst[w];
@@ -973,12 +974,15 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
   // leaves' folds get buffers too small for them; others: HGX_ENOMEM)
   const char *fail_env = std::getenv("HGX_SERIES_FAIL_HEIGHT");
   const long fail_height = fail_env ? std::atol(fail_env) : -1;
-  // the leaves, folded on the device for a split whose heights go there;
-  // else on the host in chunks: one thread folds a chunk's leaves, copies
-  // their partials into one region and frees its own fold vectors
-  std::vector<Part> leaf(blocks.size());
+  // the live partials by their interval's lower end (children of a node
+  // start at lo and mid, and no two live partials share a lower end),
+  // first the leaves: folded on the device for a split whose heights go
+  // there, else on the host in chunks -- one thread folds a chunk's leaves,
+  // copies their partials into one region and frees its own fold vectors
+  std::vector<std::pair<u64, Part>> live(blocks.size());
+  for (size_t i = 0; i < blocks.size(); i++) live[i].first = blocks[i].first;
   bool leaves_dev = false;
-  rc = leaves_on_device(s, blocks, lower, upper, fail_height == 0, leaf, leaves_dev);
+  rc = leaves_on_device(s, blocks, lower, upper, fail_height == 0, live, leaves_dev);
   if (rc) return rc;
   const size_t nchunk = leaves_dev ? 0 : (blocks.size() + kLeafChunk - 1) / kLeafChunk;
   rc = parallel_for(nchunk, [&](size_t c) {
@@ -996,7 +1000,8 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     for (size_t i = i0; i < i1; i++) {
       AccPart &a = acc[i - i0];
       const Acc *src[4] = {&a.P, &a.Q, &a.B, &a.R};
-      Num *dst[4] = {&leaf[i].P, &leaf[i].Q, &leaf[i].B, &leaf[i].R};
+      Part &pt = live[i].second;
+      Num *dst[4] = {&pt.P, &pt.Q, &pt.B, &pt.R};
       for (int k = 0; k < 4; k++) {
         Num &d = *dst[k];
         d.s = src[k]->s;
@@ -1011,13 +1016,9 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     return (int)HGX_OK;
   });
   if (rc) return rc;
-  // partials by their interval's lower end (children of a node start at lo
-  // and mid, and no two live partials share a lower end)
-  std::vector<std::pair<u64, Part>> live;
-  live.reserve(blocks.size());
-  for (size_t i = 0; i < blocks.size(); i++) live.emplace_back(blocks[i].first, std::move(leaf[i]));
-  leaf.clear();
-  std::sort(live.begin(), live.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  // build_tree lists the leaves left to right
+  auto by_lower = [](const auto &x, const auto &y) { return x.first < y.first; };
+  if (!std::is_sorted(live.begin(), live.end(), by_lower)) std::sort(live.begin(), live.end(), by_lower);
   auto find = [&](u64 lo) -> Part & {
     auto it = std::lower_bound(live.begin(), live.end(), lo, [](const auto &x, u64 v) { return x.first < v; });
     return it->second;
