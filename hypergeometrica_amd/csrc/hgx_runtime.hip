@@ -2876,6 +2876,7 @@ int hgx_internal_fold_device(const FoldSeries *S, const uint64_t *bounds, size_t
     per += cap[b];
   }
   const size_t st = (count + 63) & ~(size_t)63;  // the stride, whole waves of leaves
+  if (per > ((size_t)1 << 40) / st) return fail(HGX_ETOOBIG, "leaf fold buffers above 8 TiB");
   const size_t bytes = 8 * st * per + 16 * count + 16 * count + 8 * count;
   void *scr = nullptr;
   rc = dev_block_alloc(bytes, &scr);

@@ -813,10 +813,10 @@ int leaves_on_device(const Series &s, const std::vector<std::pair<u64, u64>> &bl
   const size_t count = blocks.size();
   u64 per = 0;
   for (int b = 0; b < kFoldBufs; b++) per += cap[b];
-  const size_t est = leaf_limbs * count;
-  if (mode == 1 && est < kDeviceMinLimbs) return HGX_OK;
+  // (divisions, not products, so that no size can overflow)
+  if (mode == 1 && leaf_limbs < (kDeviceMinLimbs + count - 1) / count) return HGX_OK;
   const size_t room = hgx_internal_device_room();
-  if (est > room / (8 * kDeviceRoomFactor) || 8 * per * count > room / 2) return HGX_OK;
+  if (leaf_limbs > room / (8 * kDeviceRoomFactor) / count || per > room / 16 / count) return HGX_OK;
   if (tight)
     for (int b = 0; b < kFoldBufs; b++) cap[b] = kFoldTV2 + 1;
   std::vector<u64> bounds(2 * count);
