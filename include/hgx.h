@@ -324,9 +324,10 @@ int hgx_modmul_probe(int op, int iters, double *out);
 // limbs (*ntt-multiply-threshold*, src/multiply.lisp:9) go to the GPU in one
 // hgx_mul_batch call; smaller ones are multiplied on the host, as mpz-*
 // does (src/multiply.lisp:37-42).  A split whose heights' products hold at
-// least 2^20 limbs instead keeps its values in device memory from its first
-// height on (every product of a round in one hgx_mul_batch_device call, the
-// sums R on the device), with the same result; HGX_SERIES_DEVICE=0 in the
+// least 2^20 limbs instead keeps its values in device memory throughout (its
+// subtrees of at most 32 terms folded by one thread each on the device,
+// every product of a round in one hgx_mul_batch_device call, the sums R on
+// the device), with the same result; HGX_SERIES_DEVICE=0 in the
 // environment keeps every split on the host, =1 moves every split.  An
 // interval whose products are all small needs no hgx_init.  Replaces a
 // Lisp-side binary-split whose term functions are polynomials: a(k), b(k),

@@ -529,11 +529,12 @@ dropped (MPD-TRUNCATE!'s mantissa step, src/mpd.lisp:56-70)."
 ;;; closures.  HGX-BINARY-SPLIT takes the same four term functions as
 ;;; polynomials, the form every series of the reference has, and returns
 ;;; the PARTIAL that BINARY-SPLIT returns for them: the library builds the
-;;; same tree and folds its small subtrees on host threads.  A large split
-;;; then keeps its values in device memory, each round of a tree height one
-;;; batched device call and the sums on the device; a small one sends each
-;;; height's products above *NTT-MULTIPLY-THRESHOLD* to the GPU in one
-;;; batched call and multiplies the others on host threads.
+;;; same tree and folds its small subtrees.  A large split keeps its values
+;;; in device memory -- the subtrees folded there, one thread each, each round
+;;; of a tree height one batched device call and the sums on the device; a
+;;; small one folds its subtrees on host threads, sends each height's
+;;; products above *NTT-MULTIPLY-THRESHOLD* to the GPU in one batched call
+;;; and multiplies the others on host threads.
 
 (defun %fill-term (desc index term)
   "TERM is a list of integer coefficients, constant term first, optionally
