@@ -27,7 +27,8 @@ EXPORTS = (
     "hgx_device_alloc", "hgx_device_free", "hgx_upload", "hgx_download", "hgx_device_copy", "hgx_device_fill",
     "hgx_device_significant", "hgx_device_compare", "hgx_shl_device", "hgx_host_alloc", "hgx_host_free",
     "hgx_synchronize", "hgx_stream_create", "hgx_stream_destroy", "hgx_modmul_probe",
-    "hgx_kernel_set", "hgx_kernel_get", "hgx_series_split", "hgx_partial_get", "hgx_partial_free",
+    "hgx_kernel_set", "hgx_kernel_get", "hgx_series_split", "hgx_partial_get", "hgx_partial_get_device",
+    "hgx_partial_free",
 )
 
 
@@ -127,6 +128,8 @@ def load() -> ctypes.CDLL:
         L.hgx_modmul_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.hgx_series_split.argtypes = [ctypes.POINTER(SeriesDesc), _u64, _u64, ctypes.POINTER(_vp)]
         L.hgx_partial_get.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_sz), _vp]
+        L.hgx_partial_get_device.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_sz), _vp,
+                                             _vp]
         L.hgx_partial_free.argtypes = [_vp]
         L.hgx_partial_free.restype = None
         _lib = L

@@ -2860,6 +2860,17 @@ static int fold_device_in(void *scr, const FoldSeries &S, const uint64_t *bounds
   return HGX_OK;
 }
 
+int hgx_internal_copy_device(uint64_t *d_dst, const uint64_t *d_src, size_t n) {
+  std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);
+  int rc = check_ready();
+  if (rc) return rc;
+  if (!n) return HGX_OK;
+  if (!d_dst || !d_src) return fail(HGX_EINVAL, "null device copy");
+  HIPCHK(hipMemcpyAsync(d_dst, d_src, 8 * n, hipMemcpyDeviceToDevice, g_ctx.stream));
+  HIPCHK(hipStreamSynchronize(g_ctx.stream));
+  return HGX_OK;
+}
+
 int hgx_internal_fold_device(const FoldSeries *S, const uint64_t *bounds, size_t count, const uint32_t *cap,
                              int32_t *codes, uint64_t **d_vals, size_t *nlimbs) {
   std::lock_guard<std::recursive_mutex> lk(g_ctx.mu);

@@ -1149,14 +1149,42 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     }
     h0 = h1;
   }
-  // the result gets a region of its own: the last height's regions also
-  // hold that height's intermediate products
+  // the result gets a region of its own (the last height's regions also
+  // hold that height's intermediate products), in device memory for a split
+  // that ran there
   const Part &top = find(lower);
   std::unique_ptr<hgx_partial> res(new (std::nothrow) hgx_partial);
   if (!res) return hgx_internal_fail(HGX_ENOMEM, "out of host memory for the series result");
   const Num *src[4] = {&top.P, &top.Q, &top.B, &top.R};
   size_t total = 0;
   for (int k = 0; k < 4; k++) total += src[k]->n;
+  if (on_dev) {
+    std::shared_ptr<DevRegion> dreg;
+    rc = new_dev_region(total, dreg);
+    if (rc) return rc;
+    size_t at = 0;
+    for (int k = 0; k < 4; k++) {
+      Num &d = res->v[k];
+      d.n = src[k]->n;
+      d.cap = d.n;
+      d.p = dreg->base + at;
+      d.dev = true;
+      d.own = dreg;
+      at += d.n;
+      rc = hgx_internal_copy_device(d.p, src[k]->p, d.n);
+      if (rc) return rc;
+      // device products keep their operands' full lengths, so a top value
+      // can carry many zero limbs (~10^5 at 2^28 bits): one device search
+      // for the top non-zero limb trims it
+      size_t sig = 0;
+      if (d.n) rc = hgx_device_significant(d.p, d.n, &sig, nullptr);
+      if (rc) return rc;
+      d.n = sig;
+      d.s = d.n ? src[k]->s : 0;
+    }
+    *out = res.release();
+    return HGX_OK;
+  }
   const RegionRef reg = new_region(nullptr, total);
   size_t at = 0;
   for (int k = 0; k < 4; k++) {
@@ -1168,20 +1196,11 @@ static int series_split(const hgx_series *desc, uint64_t lower, uint64_t upper, 
     d.own = reg;
     at += d.n;
   }
-  if (on_dev) {
-    for (int k = 0; k < 4; k++) {
-      Num &d = res->v[k];
-      if (d.n) rc = hgx_download(d.p, src[k]->p, d.n, nullptr);
-      if (rc) return rc;
-      if (d.s) num_trim(d);
-    }
-  } else {
-    rc = parallel_for(4, [&](size_t k) {
-      std::copy(src[k]->p, src[k]->p + src[k]->n, res->v[k].p);
-      return (int)HGX_OK;
-    });
-    if (rc) return rc;
-  }
+  rc = parallel_for(4, [&](size_t k) {
+    std::copy(src[k]->p, src[k]->p + src[k]->n, res->v[k].p);
+    return (int)HGX_OK;
+  });
+  if (rc) return rc;
   *out = res.release();
   return HGX_OK;
 }
@@ -1191,7 +1210,20 @@ int hgx_partial_get(const hgx_partial *x, int which, int *sign, size_t *nlimbs, 
   const Num &v = x->v[which];
   if (sign) *sign = v.s;
   if (nlimbs) *nlimbs = v.n;
-  if (limbs && v.n) std::memcpy(limbs, v.p, 8 * v.n);
+  if (limbs && v.n) {
+    if (v.dev) return hgx_download(limbs, v.p, v.n, nullptr);
+    std::memcpy(limbs, v.p, 8 * v.n);
+  }
+  return HGX_OK;
+}
+
+int hgx_partial_get_device(const hgx_partial *x, int which, int *sign, size_t *nlimbs, uint64_t *d_limbs,
+                           void *stream) {
+  if (!x || which < 0 || which > 3) return hgx_internal_fail(HGX_EINVAL, "bad partial or component index");
+  const Num &v = x->v[which];
+  if (sign) *sign = v.s;
+  if (nlimbs) *nlimbs = v.n;
+  if (d_limbs && v.n) return v.dev ? hgx_internal_copy_device(d_limbs, v.p, v.n) : hgx_upload(d_limbs, v.p, v.n, stream);
   return HGX_OK;
 }
 

@@ -45,3 +45,7 @@ int hgx_internal_upload_gather(uint64_t *d_dst, const uint64_t *const *src, cons
 struct FoldSeries;
 int hgx_internal_fold_device(const FoldSeries *S, const uint64_t *bounds, size_t count, const uint32_t *cap,
                              int32_t *codes, uint64_t **d_vals, size_t *nlimbs);
+
+// D_DST[0..N) = D_SRC[0..N), device to device on the library's stream;
+// returns when the copy is done
+int hgx_internal_copy_device(uint64_t *d_dst, const uint64_t *d_src, size_t n);

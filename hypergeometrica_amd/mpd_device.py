@@ -96,6 +96,33 @@ def from_limbs(sign: int, limbs: np.ndarray, exponent: int = 0) -> DMPD:
     return DMPD(sign, buf, 0, n, exponent)
 
 
+def split_to_device(series, lower: int, upper: int, which: str = "pqbr") -> dict[str, DMPD]:
+    """The native split's (hgx_series_split) P, Q, B, R named in WHICH as
+    device values: copied device to device when the split ran in device
+    memory (hgx_partial_get_device), uploaded when it ran on the host."""
+    from . import series as _series
+    L = _L()
+    desc = _series.native_desc(series)
+    res = ctypes.c_void_p(0)
+    check(L.hgx_series_split(ctypes.byref(desc), lower, upper, ctypes.byref(res)))
+    out = {}
+    try:
+        for w, k in enumerate("pqbr"):
+            if k not in which:
+                continue
+            sign, n = ctypes.c_int(0), ctypes.c_size_t(0)
+            check(L.hgx_partial_get_device(res, w, ctypes.byref(sign), ctypes.byref(n), None, None))
+            if sign.value == 0 or n.value == 0:
+                out[k] = DMPD(0, None, 0, 0, 0)
+                continue
+            buf = DBuf(n.value)
+            check(L.hgx_partial_get_device(res, w, None, None, buf.ptr, None))
+            out[k] = DMPD(sign.value, buf, 0, n.value, 0)
+    finally:
+        L.hgx_partial_free(res)
+    return out
+
+
 def to_host(x: DMPD) -> _mpd.MPD:
     if x.sign == 0:
         return _mpd.MPD(0, x.exponent)
@@ -261,10 +288,11 @@ class DeviceContext:
         """mpd-pi (src/pi.lisp:53-74): binary splitting by the library's
         native engine (hgx_series_split), then the Newton steps and the
         final products on the device.  The partial's limbs go to device
-        memory as they are (partial-denominator B*Q as a device product), and
-        only the result comes back as an integer.  The square root of 10005
+        memory device to device when the split ran there, and are uploaded
+        otherwise (partial-denominator B*Q as a device product); only the
+        result comes back as an integer.  The square root of 10005
         does not depend on the series: it runs on a second thread while the
-        split folds its leaves on the host (the same integers either way)."""
+        split runs (the same integers either way)."""
         import threading
         import time
 
@@ -285,16 +313,16 @@ class DeviceContext:
         t = [time.perf_counter()]
         side.start()
         try:
-            comp = _series.binary_split_native_limbs(_series.chudnovsky_series(), 0, num_terms)
+            comp = split_to_device(_series.chudnovsky_series(), 0, num_terms, "bqr")
         finally:
             side.join()
         if "error" in box:
             raise box["error"]
         t.append(time.perf_counter())
-        recip = self.reciprocal(from_limbs(*comp["r"]), guard)
+        recip = self.reciprocal(comp["r"], guard)
         t.append(time.perf_counter())
         final = self.mpd_mul(box["sqrt"], recip)
-        den = self.mpd_mul(from_limbs(*comp["b"]), from_limbs(*comp["q"]))
+        den = self.mpd_mul(comp["b"], comp["q"])
         out = to_host(self.mpd_mul(final, den))
         t.append(time.perf_counter())
         # seconds of each step (each ends with a read-back, so its device work

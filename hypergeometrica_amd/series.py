@@ -209,11 +209,8 @@ def binary_split_native(series: Series, lower: int, upper: int) -> Partial:
     return Partial(lower, upper, *(v[k][0] * _int(v[k][1]) for k in "pqbr"))
 
 
-def binary_split_native_limbs(series: Series, lower: int, upper: int) -> dict[str, tuple[int, np.ndarray]]:
-    """binary_split_native's P, Q, B, R as (sign, trimmed little-endian
-    limbs), with no conversion to Python integers."""
-    import ctypes
-
+def native_desc(series: Series):
+    """The hgx_series descriptor of a series whose term functions are Poly."""
     from . import _lib
     fns = (series.a, series.b, series.p, series.q)
     if not all(isinstance(f, Poly) for f in fns):
@@ -227,6 +224,16 @@ def binary_split_native_limbs(series: Series, lower: int, upper: int) -> dict[st
             desc.coef[i][j] = c
         desc.at_zero[i] = 0 if f.at_zero is None else 1
         desc.value0[i] = 0 if f.at_zero is None else f.at_zero
+    return desc
+
+
+def binary_split_native_limbs(series: Series, lower: int, upper: int) -> dict[str, tuple[int, np.ndarray]]:
+    """binary_split_native's P, Q, B, R as (sign, trimmed little-endian
+    limbs), with no conversion to Python integers."""
+    import ctypes
+
+    from . import _lib
+    desc = native_desc(series)
     L = _lib.load() if not _lib._initialised else _lib.lib()
     res = ctypes.c_void_p(0)
     _lib.check(L.hgx_series_split(ctypes.byref(desc), lower, upper, ctypes.byref(res)))

@@ -340,6 +340,15 @@ int hgx_modmul_probe(int op, int iters, double *out);
 // *OUT receives a result that the caller reads with hgx_partial_get (WHICH 0
 // P, 1 Q, 2 B, 3 R: SIGN in {-1, 0, 1}, NLIMBS of the trimmed magnitude,
 // copied to LIMBS when it is not null) and releases with hgx_partial_free.
+// A split that ran in device memory keeps its result there until
+// hgx_partial_free (so that result needs the library initialised until
+// then): hgx_partial_get downloads it into LIMBS, and hgx_partial_get_device
+// copies it device to device into D_LIMBS (device memory of at least
+// *NLIMBS limbs), uploading instead a result held in host memory; both
+// return when the limbs are in place.  hgx_partial_get_device has no
+// counterpart in the reference: it serves device-resident callers such as
+// the Newton steps of mpd-pi (src/pi.lisp:53-74), which take B, Q and R
+// without a round trip through host memory.
 typedef struct {
   int degree[4];
   int64_t coef[4][8];
@@ -353,6 +362,8 @@ int hgx_series_split(const hgx_series *series, uint64_t lower, uint64_t upper,
                      hgx_partial **out);
 int hgx_partial_get(const hgx_partial *x, int which, int *sign, size_t *nlimbs,
                     uint64_t *limbs);
+int hgx_partial_get_device(const hgx_partial *x, int which, int *sign,
+                           size_t *nlimbs, uint64_t *d_limbs, void *stream);
 void hgx_partial_free(hgx_partial *x);
 
 #ifdef __cplusplus

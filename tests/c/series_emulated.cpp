@@ -72,6 +72,15 @@ int hgx_internal_upload_gather(uint64_t *d_dst, const uint64_t *const *src, cons
   return HGX_OK;
 }
 extern "C" int hgx_mul_batch_device(const hgx_mul_desc *d, size_t count, void *) { return hgx_mul_batch(d, count); }
+extern "C" int hgx_device_significant(const uint64_t *d, size_t n, size_t *out, void *) {
+  while (n && !d[n - 1]) n--;
+  *out = n;
+  return HGX_OK;
+}
+int hgx_internal_copy_device(uint64_t *d, const uint64_t *s, size_t n) {
+  memcpy(d, s, 8 * n);
+  return HGX_OK;
+}
 size_t hgx_internal_device_room() { return (size_t)1 << 40; }
 
 // the device leaf folds (hgx_sum.h): the kernels' own fold_leaf and

@@ -172,3 +172,17 @@ def test_device_fold_of_wide_term_values(hgx, monkeypatch, series, lo, up):
     # last series) are folded on the host
     monkeypatch.setenv("HGX_SERIES_DEVICE", "1")
     assert _tuple(S.binary_split_native(series, lo, up)) == SO.binary_split(series, lo, up)
+
+
+@pytest.mark.parametrize("mode", ["0", "1"])
+def test_partial_read_into_device_memory(hgx, monkeypatch, mode):
+    # hgx_partial_get_device: a split that ran in device memory (=1) hands
+    # its partial over device to device, one that ran on the host (=0)
+    # uploads it; both equal the oracle's
+    from hypergeometrica_amd import mpd_device as MD
+    monkeypatch.setenv("HGX_SERIES_DEVICE", mode)
+    s = S.chudnovsky_series()
+    got = MD.split_to_device(s, 0, 3000)
+    want = SO.binary_split(s, 0, 3000)[2:]
+    for k, w in zip("pqbr", want):
+        assert MD.to_host(got[k]).mantissa == w, k
