@@ -16,11 +16,12 @@
 // while the GPU works, as the reference's mpz-* multiplies them
 // (src/multiply.lisp:21-42).  A split whose heights hold enough limbs moves
 // its values to device memory and keeps them there: each round is one
-// batched device call (hgx_mul_batch_device) and the sums R run on the
-// device, with one download of the result.  Such a split folds its leaves on
-// the device too, one thread per leaf (hgx_fold.hpp); other splits fold them
-// on host threads (and a split that moves later uploads them).  All arithmetic is exact, so every partial equals the
-// reference's.
+// batched device call (hgx_mul_batch_device), the sums R run on the device,
+// and the result stays there (hgx_partial_get downloads it,
+// hgx_partial_get_device copies it device to device).  Such a split folds its
+// leaves on the device too, one thread per leaf (hgx_fold.hpp); other splits
+// fold them on host threads (and a split that moves later uploads them).
+// All arithmetic is exact, so every partial equals the reference's.
 //
 // Series terms are polynomials in k with 64-bit signed coefficients and an
 // optional separate value at k = 0, which covers the reference's series
